@@ -1,0 +1,287 @@
+"""Python mirror of the reference's HBDIA API, backed by libhbdia.so.
+
+Names, argument meaning and error behaviour follow the reference's C++ API
+(include/Format/HBDIA.hpp, include/Format/HBDIAVector.hpp,
+include/Operations/HBDIASpMV.hpp), so tests and scripts read like the
+reference's own drivers:
+
+    A = HBDIA(); A.create3DStencil27Point(8, 8, 8, 0.0)
+    A.convertToHBDIAFormat(32, 2)
+    x = HBDIAVector(np.arange(1, 513, dtype=np.float64))
+    y = HBDIAVector(np.zeros(512))
+    hbdiaSpMV(A, x, y, execCOOCPU=True, execCOOGPU=False)
+    A.synchronize(); y.download()
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from ctypes import byref, c_void_p
+
+import numpy as np
+
+from ._native import HBDIAError, capi, check_capi
+
+DEFAULT_BLOCK_WIDTH = 32
+DEFAULT_THRESHOLD = 1
+MAX_COO_ENTRIES = 1_000_000_000
+RAND_SEED = 42
+
+F64, F32 = 0, 1
+_INFO = ["rows", "cols", "nnz", "has_coo", "has_hbdia", "block_width", "threshold", "num_blocks", "num_values",
+         "num_offsets", "num_coo", "num_diagonals", "hist_len", "on_device", "device_coo", "stored_bytes"]
+
+
+class ExecutionMode(enum.IntEnum):
+    CPU_COO = 0
+    GPU_COO = 1
+
+
+def _code(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return F64
+    if dt == np.float32:
+        return F32
+    raise TypeError(f"unsupported dtype {dt} (float64 or float32)")
+
+
+def _map_error(e: HBDIAError, what: str):
+    msg = str(e)
+    if "invalid_argument" in msg:
+        return ValueError(msg)
+    if "out_of_range" in msg:
+        return IndexError(msg)
+    return e
+
+
+class HBDIA:
+    """Hybrid blocked-diagonal matrix (reference HBDIA<T>)."""
+
+    def __init__(self, rowIndices=None, colIndices=None, values=None, numRows: int | None = None,
+                 numCols: int | None = None, dtype=np.float64, _handle=None):
+        self.dtype = np.dtype(dtype if values is None else np.asarray(values).dtype)
+        if _handle is not None:
+            self._h = _handle
+            self.dtype = np.dtype(dtype)
+            return
+        h = c_void_p(None)
+        if rowIndices is None:
+            check_capi(capi.hbdia_matrix_create(byref(h), _code(self.dtype)))
+        else:
+            r = np.ascontiguousarray(rowIndices, dtype=np.int32)
+            c = np.ascontiguousarray(colIndices, dtype=np.int32)
+            v = np.ascontiguousarray(values, dtype=self.dtype)
+            if not (len(r) == len(c) == len(v)):
+                raise ValueError("Row indices, column indices, and values vectors must have the same size")
+            nr = -1 if numRows is None else int(numRows)
+            nc = -1 if numCols is None else int(numCols)
+            if (numRows is None) != (numCols is None):
+                raise ValueError("give both numRows and numCols or neither")
+            if numRows is not None and (nr <= 0 or nc <= 0):
+                raise ValueError("Matrix dimensions must be positive")
+            try:
+                check_capi(capi.hbdia_matrix_create_coo(byref(h), _code(self.dtype), r.ctypes.data, c.ctypes.data,
+                                                        v.ctypes.data, len(v), nr, nc))
+            except HBDIAError as e:
+                raise _map_error(e, "HBDIA") from None
+        self._h = h.value
+
+    # ---- generators -------------------------------------------------------
+    @classmethod
+    def stencil(cls, n: int, points: int = 27, scatter: float = 0.0, seed: int = 0, dtype=np.float64) -> "HBDIA":
+        """Seeded stencil + scatter benchmark matrix (see include/Workloads.hpp)."""
+        h = c_void_p(None)
+        check_capi(capi.hbdia_gen_stencil(byref(h), _code(dtype), int(n), int(points), float(scatter), int(seed)))
+        return cls(dtype=dtype, _handle=h.value)
+
+    @classmethod
+    def banded(cls, n: int, noffsets: int, max_offset: int = 64, offband: float = 0.01, seed: int = 0,
+               dtype=np.float64) -> "HBDIA":
+        h = c_void_p(None)
+        check_capi(capi.hbdia_gen_banded(byref(h), _code(dtype), int(n), int(noffsets), int(max_offset),
+                                         float(offband), int(seed)))
+        return cls(dtype=dtype, _handle=h.value)
+
+    def create3DStencil27Point(self, nx: int, ny: int, nz: int, noise: float = 0.0, iteration: int = 0) -> None:
+        check_capi(capi.hbdia_matrix_stencil27(self._h, nx, ny, nz, float(noise), iteration))
+
+    # ---- conversion -------------------------------------------------------
+    def convertToHBDIAFormat(self, blockWidth: int = DEFAULT_BLOCK_WIDTH, threshold: int = DEFAULT_THRESHOLD,
+                             max_coo_entries: int = MAX_COO_ENTRIES, COOisUnique: bool = False,
+                             execMode: ExecutionMode = ExecutionMode.GPU_COO) -> None:
+        try:
+            check_capi(capi.hbdia_matrix_convert(self._h, int(blockWidth), int(threshold), int(max_coo_entries),
+                                                 int(bool(COOisUnique)), int(execMode == ExecutionMode.GPU_COO)))
+        except HBDIAError as e:
+            raise _map_error(e, "convert") from None
+
+    def convertToHBDIAFormatOnHost(self, blockWidth: int = DEFAULT_BLOCK_WIDTH,
+                                   threshold: int = DEFAULT_THRESHOLD, max_coo_entries: int = MAX_COO_ENTRIES,
+                                   COOisUnique: bool = False,
+                                   execMode: ExecutionMode = ExecutionMode.GPU_COO) -> None:
+        """The conversion without the device upload (prepareForGPU uploads later)."""
+        try:
+            check_capi(capi.hbdia_matrix_convert_host(self._h, int(blockWidth), int(threshold),
+                                                      int(max_coo_entries), int(bool(COOisUnique)),
+                                                      int(execMode == ExecutionMode.GPU_COO)))
+        except HBDIAError as e:
+            raise _map_error(e, "convert") from None
+
+    def removeCOODuplicates(self) -> None:
+        check_capi(capi.hbdia_matrix_remove_duplicates(self._h))
+
+    def prepareForGPU(self) -> None:
+        check_capi(capi.hbdia_matrix_prepare_gpu(self._h))
+
+    def synchronize(self) -> None:
+        check_capi(capi.hbdia_matrix_synchronize(self._h))
+
+    # ---- getters ----------------------------------------------------------
+    def info(self) -> dict:
+        arr = np.zeros(16, dtype=np.int64)
+        check_capi(capi.hbdia_matrix_info(self._h, arr.ctypes.data))
+        return dict(zip(_INFO, (int(v) for v in arr)))
+
+    def getNumRows(self) -> int: return self.info()["rows"]
+    def getNumCols(self) -> int: return self.info()["cols"]
+    def getNumNonZeros(self) -> int: return self.info()["nnz"]
+    def getBlockWidth(self) -> int: return self.info()["block_width"]
+    def getThreshold(self) -> int: return self.info()["threshold"]
+    def getNumBlocks(self) -> int: return self.info()["num_blocks"]
+    def getNumberDiagonals(self) -> int: return self.info()["num_diagonals"]
+    def isHBDIAFormat(self) -> bool: return bool(self.info()["has_hbdia"])
+    def isCOOFormat(self) -> bool: return bool(self.info()["has_coo"])
+
+    def getCOO(self):
+        nnz = self.info()["nnz"]
+        r = np.empty(nnz, np.int32)
+        c = np.empty(nnz, np.int32)
+        v = np.empty(nnz, self.dtype)
+        check_capi(capi.hbdia_matrix_get_coo(self._h, r.ctypes.data, c.ctypes.data, v.ctypes.data))
+        return r, c, v
+
+    def getRowIndices(self): return self.getCOO()[0]
+    def getColIndices(self): return self.getCOO()[1]
+    def getValues(self): return self.getCOO()[2]
+
+    def getHBDIA(self) -> dict:
+        """All blocked-DIA host arrays: values, offsets (flattened), block_start,
+        coo_rows/cols/values, row_ptr, hist_blocks, hist_nnz."""
+        i = self.info()
+        out = {
+            "values": np.empty(i["num_values"], self.dtype),
+            "offsets": np.empty(i["num_offsets"], np.int32),
+            "block_start": np.empty(i["num_blocks"] + 1, np.int32),
+            "coo_rows": np.empty(i["num_coo"], np.int32),
+            "coo_cols": np.empty(i["num_coo"], np.int32),
+            "coo_values": np.empty(i["num_coo"], self.dtype),
+            "row_ptr": np.empty(i["rows"] + 1 if i["has_hbdia"] else 0, np.int32),
+            "hist_blocks": np.empty(i["hist_len"], np.int32),
+            "hist_nnz": np.empty(i["hist_len"], np.int32),
+        }
+        check_capi(capi.hbdia_matrix_get_hbdia(self._h, *(a.ctypes.data for a in out.values())))
+        return out
+
+    def getOffsetsPerBlock(self) -> list[np.ndarray]:
+        hb = self.getHBDIA()
+        bs = hb["block_start"]
+        return [hb["offsets"][bs[b]:bs[b + 1]] for b in range(len(bs) - 1)]
+
+    def device_ptrs(self) -> dict:
+        arr = (ctypes.c_void_p * 10)()
+        check_capi(capi.hbdia_matrix_device_ptrs(self._h, arr))
+        keys = ["values", "offsets", "block_start", "block_sizes", "coo_rows", "coo_cols", "coo_values",
+                "coo_row_ptr", "coo_group_ptr", "bdia_stream"]
+        return dict(zip(keys, (arr[k] for k in range(10))))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            check_capi(capi.hbdia_matrix_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                capi.hbdia_matrix_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class HBDIAVector:
+    """Device-resident vector (reference HBDIAVector<T>)."""
+
+    def __init__(self, localData, unifiedMemory: bool = False, unifiedMemoryMalloc: bool = False,
+                 unifiedMemoryManagedMalloc: bool = False, unifiedNumaAllocOnNode: bool = False, dtype=None):
+        a = np.ascontiguousarray(localData, dtype=dtype or np.asarray(localData).dtype)
+        self.dtype = a.dtype
+        self.size = int(a.size)
+        self.unified = bool(unifiedMemory)
+        h = c_void_p(None)
+        check_capi(capi.hbdia_vector_create(byref(h), _code(a.dtype), a.ctypes.data, a.size, int(unifiedMemory),
+                                            int(unifiedMemoryMalloc), int(unifiedMemoryManagedMalloc),
+                                            int(unifiedNumaAllocOnNode)))
+        self._h = h.value
+
+    def getLocalSize(self) -> int:
+        return self.size
+
+    def upload(self, data) -> None:
+        a = np.ascontiguousarray(data, dtype=self.dtype)
+        check_capi(capi.hbdia_vector_upload(self._h, a.ctypes.data, a.size))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.size, self.dtype)
+        check_capi(capi.hbdia_vector_download(self._h, out.ctypes.data, self.size))
+        return out
+
+    def download_coo(self) -> np.ndarray:
+        out = np.empty(self.size, self.dtype)
+        check_capi(capi.hbdia_vector_download_coo(self._h, out.ctypes.data, self.size))
+        return out
+
+    def device_ptrs(self) -> tuple[int, int]:
+        a, b = c_void_p(None), c_void_p(None)
+        check_capi(capi.hbdia_vector_device_ptrs(self._h, byref(a), byref(b)))
+        return a.value, b.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            check_capi(capi.hbdia_vector_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                capi.hbdia_vector_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def hbdiaSpMV(matrix: HBDIA, inputVector: HBDIAVector, outputVector: HBDIAVector, execCOOCPU: bool,
+              execCOOGPU: bool) -> bool:
+    """y = A x (reference hbdiaSpMV, src/Operations/HBDIASpMV.cpp:186-251).
+
+    Asynchronous: call matrix.synchronize() before reading the output.  Raises
+    RuntimeError("Invalid COO execution flags") when the matrix has fallback
+    entries and both flags are false."""
+    try:
+        check_capi(capi.hbdia_spmv(matrix._h, inputVector._h, outputVector._h, int(bool(execCOOCPU)),
+                                   int(bool(execCOOGPU))))
+    except HBDIAError as e:
+        if "Invalid COO execution flags" in str(e):
+            raise RuntimeError("Invalid COO execution flags") from None
+        raise _map_error(e, "hbdiaSpMV") from None
+    return True
+
+
+def gen_vector(n: int, seed: int, lo: float = -1.0, hi: float = 1.0, dtype=np.float64) -> np.ndarray:
+    out = np.empty(int(n), np.dtype(dtype))
+    check_capi(capi.hbdia_gen_vector(_code(dtype), int(n), int(seed), float(lo), float(hi), out.ctypes.data))
+    return out
+
+
+def set_num_threads(n: int) -> None:
+    check_capi(capi.hbdia_set_num_threads(int(n)))

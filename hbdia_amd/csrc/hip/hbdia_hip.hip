@@ -1,0 +1,512 @@
+// hbdia_hip.hip -- device side of the HBDIA SpMV path for MI355X (gfx950),
+// exported through the C ABI declared in include/hbdia_hip.h.
+//
+// Kernel design (HBM-bound; no MFMA, SpMV is ~0.2 flop/byte):
+//   * one 64-lane wavefront per group of 64 consecutive rows; four waves per
+//     256-thread workgroup;
+//   * workgroups are remapped so that each of the 8 XCDs (which receive
+//     workgroups round-robin) works on one contiguous eighth of the rows, so
+//     that the x window and its +-offset halo are reused from that XCD's L2;
+//   * when the block width is a multiple of 64 the block, its offset list and
+//     its start are wave-uniform (scalar loads); values of one (block, offset)
+//     pair are one contiguous 512-byte FP64 line per wave, loaded with the
+//     nontemporal hint so that the value stream does not evict x;
+//   * offsets are consumed in chunks of 8 so that 8 value loads and 8 x loads
+//     are in flight per wave before the first FMA;
+//   * the COO fallback is fused: the entries of the 64-row group are loaded
+//     cooperatively (one per lane), x is gathered in parallel, and each lane
+//     then sums its own row's entries sequentially in entry order out of LDS.
+//     This keeps the reference CPU summation order, is deterministic, and
+//     writes y exactly once with no separate vector-add pass.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "hbdia_hip.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return HBDIA_SUCCESS;
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory) return fail(HBDIA_ERROR_OUT_OF_MEMORY, m);
+  return fail(HBDIA_ERROR_DEVICE, m);
+}
+
+inline hipStream_t hs(hbdia_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int kWave = 64;
+constexpr int kWavesPerWG = 4;
+constexpr int kWG = kWave * kWavesPerWG;
+constexpr int kChunk = 8;
+constexpr unsigned kNumXcd = 8;
+
+// Workgroup i is dispatched to XCD i % 8.  Give XCD x a contiguous run of
+// logical workgroups: XCDs 0..r-1 get q+1, the rest q (q = n / 8, r = n % 8).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
+  const uint32_t q = n / kNumXcd, r = n % kNumXcd;
+  const uint32_t xcd = bid % kNumXcd, idx = bid / kNumXcd;
+  const uint32_t start = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
+  return start + idx;
+}
+
+template <typename T> __device__ __forceinline__ T fma_t(T a, T b, T c);
+template <> __device__ __forceinline__ double fma_t<double>(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+template <> __device__ __forceinline__ float fma_t<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum of the COO entries of this wave's 64-row group for the calling lane's
+// row, in entry order, as sequential FMAs (the reference CPU COO order).
+// Entries [p0, p1) are row-sorted and all lie in rows [row0, row0 + 64).
+template <typename T>
+__device__ __forceinline__ T coo_group_sum(const int* __restrict__ cr, const int* __restrict__ cc,
+                                           const T* __restrict__ cv, const T* __restrict__ x,
+                                           int p0, int p1, int64_t row0, int lane, T* s_v,
+                                           T* s_x, int* s_lo, int* s_hi, bool* any) {
+  T cs = T(0);
+  bool has = false;
+  for (int base = p0; base < p1; base += kWave) {
+    const int i = base + lane;
+    const bool ok = i < p1;
+    int r = -1;
+    T vv = T(0), xx = T(0);
+    if (ok) {
+      r = cr[i] - static_cast<int>(row0);
+      vv = cv[i];
+      xx = x[cc[i]];
+      if (r < 0 || r >= kWave) r = -1;  // unsorted input: drop rather than write outside LDS
+    }
+    const int rp = __shfl_up(r, 1);
+    const int rn = __shfl_down(r, 1);
+    s_lo[lane] = 0;
+    s_hi[lane] = 0;
+    s_v[lane] = vv;
+    s_x[lane] = xx;
+    wave_sync();
+    if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
+    if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p1 || rn != r)) s_hi[r] = lane + 1;
+    wave_sync();
+    const int a = s_lo[lane], e = s_hi[lane];
+    for (int k = a; k < e; ++k) cs = fma_t<T>(s_v[k], s_x[k], cs);
+    has |= (e > a);
+    wave_sync();
+  }
+  *any = has;
+  return cs;
+}
+
+// Blocked-DIA sum for one row over `cnt` offsets, chunked for memory-level
+// parallelism.  `vp` points at the row's lane in the block's first offset
+// line; consecutive offsets are `bw` elements apart.
+template <typename T, bool UNIFORM>
+__device__ __forceinline__ T bdia_row_sum(const T* __restrict__ vp, const int* __restrict__ op,
+                                          int cnt, int64_t bw, const T* __restrict__ x,
+                                          int64_t row, int64_t nrows, bool active) {
+  T sum = T(0);
+  for (int o0 = 0; o0 < cnt; o0 += kChunk) {
+    T v[kChunk], xv[kChunk];
+    int off[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      v[k] = T(0);
+      xv[k] = T(0);
+      off[k] = 0;
+      if (o0 + k < cnt) {
+        off[k] = op[o0 + k];
+        v[k] = __builtin_nontemporal_load(vp + static_cast<int64_t>(o0 + k) * bw);
+        const int64_t c = row + off[k];
+        if (active && c >= 0 && c < nrows) xv[k] = x[c];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const int64_t c = row + off[k];
+      const bool ok = (o0 + k < cnt) && active && v[k] != T(0) && c >= 0 && c < nrows;
+      sum = ok ? fma_t<T>(v[k], xv[k], sum) : sum;
+    }
+  }
+  (void)UNIFORM;
+  return sum;
+}
+
+// Fused HBDIA SpMV.  UNIFORM: block_width % 64 == 0, so a wave's 64 rows lie
+// in one block.  HAS_COO: the matrix has a non-empty COO fallback, and then
+// every row receives y = (bdia or old y) + coo, as the reference's
+// vector_add_kernel does for all rows.
+template <typename T, bool UNIFORM, bool HAS_COO>
+__global__ __launch_bounds__(kWG) void k_hbdia(
+    const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const T* __restrict__ x, T* __restrict__ y, int64_t nrows, int nb, int bw,
+    const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
+    const int* __restrict__ gptr) {
+  __shared__ T s_v[kWavesPerWG][kWave];
+  __shared__ T s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t grp = static_cast<int64_t>(xcd_remap(blockIdx.x, gridDim.x)) * kWavesPerWG + wv;
+  const int64_t row0 = grp * kWave;
+  if (row0 >= nrows) return;
+  const int64_t row = row0 + lane;
+  const bool active = row < nrows;
+
+  int64_t b;
+  if constexpr (UNIFORM) b = row0 / bw;
+  else b = row / bw;
+  int start = 0, cnt = 0;
+  if (b < nb && (UNIFORM || active)) {
+    start = blk[b];
+    cnt = blk[b + 1] - start;
+  }
+  T sum = T(0);
+  if (cnt > 0) {
+    const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
+    sum = bdia_row_sum<T, UNIFORM>(vp, offs + start, cnt, bw, x, row, nrows, active);
+  }
+  if constexpr (HAS_COO) {
+    const int p0 = gptr[grp], p1 = gptr[grp + 1];
+    T cs = T(0);
+    bool any = false;
+    if (p0 < p1)
+      cs = coo_group_sum<T>(cr, cc, cv, x, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv],
+                            s_hi[wv], &any);
+    if (active) {
+      const T base = cnt > 0 ? sum : y[row];
+      __builtin_nontemporal_store(base + cs, y + row);
+    }
+  } else {
+    if (active && cnt > 0) __builtin_nontemporal_store(sum, y + row);
+  }
+}
+
+// COO fallback alone: res[r] = row sum for rows with entries (others keep res).
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_coo(const int* __restrict__ cr, const int* __restrict__ cc,
+                                             const T* __restrict__ cv, const int* __restrict__ gptr,
+                                             const T* __restrict__ x, T* __restrict__ res,
+                                             int64_t nrows) {
+  __shared__ T s_v[kWavesPerWG][kWave];
+  __shared__ T s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
+  const int64_t row0 = grp * kWave;
+  if (row0 >= nrows) return;
+  const int p0 = gptr[grp], p1 = gptr[grp + 1];
+  if (p0 >= p1) return;
+  bool any = false;
+  const T cs = coo_group_sum<T>(cr, cc, cv, x, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv],
+                                s_hi[wv], &any);
+  const int64_t row = row0 + lane;
+  if (any && row < nrows) res[row] = cs;
+}
+
+// group_ptr[g] = lower_bound(rows, 64 g) for g in [0, G].
+__global__ void k_group_ptr(const int* __restrict__ rows, int64_t nnz, int64_t ngroups,
+                            int* __restrict__ gptr) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g > ngroups) return;
+  const int64_t key = g * kWave;
+  int64_t lo = 0, hi = nnz;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (static_cast<int64_t>(rows[mid]) < key) lo = mid + 1;
+    else hi = mid;
+  }
+  gptr[g] = static_cast<int>(lo);
+}
+
+template <typename T>
+__global__ void k_vector_add(T* __restrict__ dst, const T* __restrict__ src, int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] += src[i];
+}
+
+int64_t num_groups(int64_t nrows) { return (nrows + kWave - 1) / kWave; }
+
+template <typename T>
+int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t nb, int32_t bw,
+                 const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr,
+                 const T* x, T* y, int64_t nrows, hbdia_stream_t stream, bool with_coo,
+                 const char* name) {
+  if (nrows < 0 || nb < 0 || bw <= 0)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative size or block_width <= 0");
+  if (nrows == 0) return HBDIA_SUCCESS;
+  if (!x || !y || !blk || (nb > 0 && !offs))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
+  if (with_coo && (!cr || !cc || !cv || !gptr))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null COO pointer");
+  const int64_t groups = num_groups(nrows);
+  const int64_t wgs = (groups + kWavesPerWG - 1) / kWavesPerWG;
+  if (wgs > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows");
+  const dim3 grid(static_cast<unsigned>(wgs)), block(kWG);
+  const bool uniform = (bw % kWave) == 0;
+  hipStream_t s = hs(stream);
+  if (uniform && with_coo)
+    hipLaunchKernelGGL((k_hbdia<T, true, true>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+  else if (uniform)
+    hipLaunchKernelGGL((k_hbdia<T, true, false>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+  else if (with_coo)
+    hipLaunchKernelGGL((k_hbdia<T, false, true>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+  else
+    hipLaunchKernelGGL((k_hbdia<T, false, false>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+  return check_hip(hipGetLastError(), name);
+}
+
+template <typename T>
+int launch_coo(const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr, const T* x,
+               T* res, int64_t nrows, hbdia_stream_t stream, const char* name) {
+  if (nrows < 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative size");
+  if (nrows == 0) return HBDIA_SUCCESS;
+  if (!cr || !cc || !cv || !gptr || !x || !res)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
+  const int64_t groups = num_groups(nrows);
+  const int64_t wgs = (groups + kWavesPerWG - 1) / kWavesPerWG;
+  if (wgs > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows");
+  hipLaunchKernelGGL((k_coo<T>), dim3(static_cast<unsigned>(wgs)), dim3(kWG), 0, hs(stream), cr, cc, cv, gptr, x, res, nrows);
+  return check_hip(hipGetLastError(), name);
+}
+
+template <typename T>
+int launch_vector_add(T* dst, const T* src, int64_t n, hbdia_stream_t stream, const char* name) {
+  if (n < 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative size");
+  if (n == 0) return HBDIA_SUCCESS;
+  if (!dst || !src) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL((k_vector_add<T>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, hs(stream), dst, src, n);
+  return check_hip(hipGetLastError(), name);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hbdia_last_error(void) { return g_last_error.c_str(); }
+
+int hbdia_get_device_count(int* count) {
+  if (!count) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_get_device_count: null");
+  return check_hip(hipGetDeviceCount(count), "hipGetDeviceCount");
+}
+int hbdia_set_device(int device) { return check_hip(hipSetDevice(device), "hipSetDevice"); }
+int hbdia_get_device(int* device) {
+  if (!device) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_get_device: null");
+  return check_hip(hipGetDevice(device), "hipGetDevice");
+}
+int hbdia_device_synchronize(void) { return check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+int hbdia_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_mem_info: null");
+  return check_hip(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
+}
+int hbdia_device_name(char* buf, size_t len) {
+  if (!buf || len == 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_device_name: null");
+  int dev = 0;
+  int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  hipDeviceProp_t p;
+  rc = check_hip(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+  if (rc) return rc;
+  std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+  return HBDIA_SUCCESS;
+}
+
+int hbdia_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_malloc: null");
+  *ptr = nullptr;
+  if (bytes == 0) return HBDIA_SUCCESS;
+  return check_hip(hipMalloc(ptr, bytes), "hipMalloc");
+}
+int hbdia_free(void* ptr) { return ptr ? check_hip(hipFree(ptr), "hipFree") : HBDIA_SUCCESS; }
+int hbdia_malloc_host(void** ptr, size_t bytes) {
+  if (!ptr) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_malloc_host: null");
+  *ptr = nullptr;
+  if (bytes == 0) return HBDIA_SUCCESS;
+  return check_hip(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+}
+int hbdia_free_host(void* ptr) { return ptr ? check_hip(hipHostFree(ptr), "hipHostFree") : HBDIA_SUCCESS; }
+int hbdia_malloc_managed(void** ptr, size_t bytes) {
+  if (!ptr) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_malloc_managed: null");
+  *ptr = nullptr;
+  if (bytes == 0) return HBDIA_SUCCESS;
+  return check_hip(hipMallocManaged(ptr, bytes, hipMemAttachGlobal), "hipMallocManaged");
+}
+int hbdia_memcpy_h2d(void* dst, const void* src, size_t bytes, hbdia_stream_t stream) {
+  if (bytes == 0) return HBDIA_SUCCESS;
+  if (!dst || !src) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_memcpy_h2d: null");
+  return check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, hs(stream)), "hipMemcpyAsync(H2D)");
+}
+int hbdia_memcpy_d2h(void* dst, const void* src, size_t bytes, hbdia_stream_t stream) {
+  if (bytes == 0) return HBDIA_SUCCESS;
+  if (!dst || !src) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_memcpy_d2h: null");
+  return check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, hs(stream)), "hipMemcpyAsync(D2H)");
+}
+int hbdia_memcpy_d2d(void* dst, const void* src, size_t bytes, hbdia_stream_t stream) {
+  if (bytes == 0) return HBDIA_SUCCESS;
+  if (!dst || !src) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_memcpy_d2d: null");
+  return check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, hs(stream)), "hipMemcpyAsync(D2D)");
+}
+int hbdia_memcpy(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return HBDIA_SUCCESS;
+  if (!dst || !src) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_memcpy: null");
+  return check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDefault), "hipMemcpy");
+}
+int hbdia_memset(void* dst, int value, size_t bytes, hbdia_stream_t stream) {
+  if (bytes == 0) return HBDIA_SUCCESS;
+  if (!dst) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_memset: null");
+  return check_hip(hipMemsetAsync(dst, value, bytes, hs(stream)), "hipMemsetAsync");
+}
+
+int hbdia_stream_create(hbdia_stream_t* stream) {
+  if (!stream) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_stream_create: null");
+  hipStream_t s = nullptr;
+  int rc = check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  *stream = reinterpret_cast<hbdia_stream_t>(s);
+  return rc;
+}
+int hbdia_stream_destroy(hbdia_stream_t stream) {
+  return stream ? check_hip(hipStreamDestroy(hs(stream)), "hipStreamDestroy") : HBDIA_SUCCESS;
+}
+int hbdia_stream_synchronize(hbdia_stream_t stream) {
+  return check_hip(hipStreamSynchronize(hs(stream)), "hipStreamSynchronize");
+}
+int hbdia_event_create(hbdia_event_t* event) {
+  if (!event) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_event_create: null");
+  hipEvent_t e = nullptr;
+  int rc = check_hip(hipEventCreate(&e), "hipEventCreate");
+  *event = reinterpret_cast<hbdia_event_t>(e);
+  return rc;
+}
+int hbdia_event_destroy(hbdia_event_t event) {
+  return event ? check_hip(hipEventDestroy(reinterpret_cast<hipEvent_t>(event)), "hipEventDestroy")
+               : HBDIA_SUCCESS;
+}
+int hbdia_event_record(hbdia_event_t event, hbdia_stream_t stream) {
+  if (!event) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_event_record: null");
+  return check_hip(hipEventRecord(reinterpret_cast<hipEvent_t>(event), hs(stream)), "hipEventRecord");
+}
+int hbdia_event_synchronize(hbdia_event_t event) {
+  if (!event) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_event_synchronize: null");
+  return check_hip(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)), "hipEventSynchronize");
+}
+int hbdia_stream_wait_event(hbdia_stream_t stream, hbdia_event_t event) {
+  if (!event) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_stream_wait_event: null");
+  return check_hip(hipStreamWaitEvent(hs(stream), reinterpret_cast<hipEvent_t>(event), 0),
+                   "hipStreamWaitEvent");
+}
+int hbdia_event_elapsed_ms(float* ms, hbdia_event_t start, hbdia_event_t stop) {
+  if (!ms || !start || !stop) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_event_elapsed_ms: null");
+  return check_hip(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),
+                                       reinterpret_cast<hipEvent_t>(stop)),
+                   "hipEventElapsedTime");
+}
+int hbdia_graph_begin_capture(hbdia_stream_t stream) {
+  if (!stream) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_graph_begin_capture: null stream");
+  return check_hip(hipStreamBeginCapture(hs(stream), hipStreamCaptureModeThreadLocal),
+                   "hipStreamBeginCapture");
+}
+int hbdia_graph_end_capture(hbdia_stream_t stream, hbdia_graph_t* graph) {
+  if (!stream || !graph) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_graph_end_capture: null");
+  hipGraph_t g = nullptr;
+  int rc = check_hip(hipStreamEndCapture(hs(stream), &g), "hipStreamEndCapture");
+  if (rc) return rc;
+  hipGraphExec_t ge = nullptr;
+  rc = check_hip(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(g);
+  *graph = reinterpret_cast<hbdia_graph_t>(ge);
+  return rc;
+}
+int hbdia_graph_launch(hbdia_graph_t graph, hbdia_stream_t stream) {
+  if (!graph) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_graph_launch: null");
+  return check_hip(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph), hs(stream)), "hipGraphLaunch");
+}
+int hbdia_graph_destroy(hbdia_graph_t graph) {
+  return graph ? check_hip(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph)), "hipGraphExecDestroy")
+               : HBDIA_SUCCESS;
+}
+
+int hbdia_bdia_spmv_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                        const double* x, double* y, int64_t num_rows, int32_t num_blocks,
+                        int32_t block_width, hbdia_stream_t stream) {
+  return launch_hbdia<double>(values, offsets, block_start, num_blocks, block_width, nullptr, nullptr,
+                              nullptr, nullptr, x, y, num_rows, stream, false, "hbdia_bdia_spmv_f64");
+}
+int hbdia_bdia_spmv_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                        const float* x, float* y, int64_t num_rows, int32_t num_blocks,
+                        int32_t block_width, hbdia_stream_t stream) {
+  return launch_hbdia<float>(values, offsets, block_start, num_blocks, block_width, nullptr, nullptr,
+                             nullptr, nullptr, x, y, num_rows, stream, false, "hbdia_bdia_spmv_f32");
+}
+int hbdia_hbdia_spmv_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                         int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                         const int32_t* coo_cols, const double* coo_values,
+                         const int32_t* coo_group_ptr, const double* x, double* y,
+                         int64_t num_rows, hbdia_stream_t stream) {
+  return launch_hbdia<double>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols,
+                              coo_values, coo_group_ptr, x, y, num_rows, stream, true,
+                              "hbdia_hbdia_spmv_f64");
+}
+int hbdia_hbdia_spmv_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                         int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                         const int32_t* coo_cols, const float* coo_values,
+                         const int32_t* coo_group_ptr, const float* x, float* y,
+                         int64_t num_rows, hbdia_stream_t stream) {
+  return launch_hbdia<float>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols,
+                             coo_values, coo_group_ptr, x, y, num_rows, stream, true,
+                             "hbdia_hbdia_spmv_f32");
+}
+int hbdia_coo_spmv_f64(const int32_t* coo_rows, const int32_t* coo_cols, const double* coo_values,
+                       const int32_t* coo_group_ptr, const double* x, double* res,
+                       int64_t num_rows, hbdia_stream_t stream) {
+  return launch_coo<double>(coo_rows, coo_cols, coo_values, coo_group_ptr, x, res, num_rows, stream,
+                            "hbdia_coo_spmv_f64");
+}
+int hbdia_coo_spmv_f32(const int32_t* coo_rows, const int32_t* coo_cols, const float* coo_values,
+                       const int32_t* coo_group_ptr, const float* x, float* res,
+                       int64_t num_rows, hbdia_stream_t stream) {
+  return launch_coo<float>(coo_rows, coo_cols, coo_values, coo_group_ptr, x, res, num_rows, stream,
+                           "hbdia_coo_spmv_f32");
+}
+int hbdia_coo_group_ptr(const int32_t* coo_rows, int64_t coo_nnz, int64_t num_rows,
+                        int32_t* group_ptr, hbdia_stream_t stream) {
+  if (coo_nnz < 0 || num_rows < 0 || coo_nnz > 0x7fffffffLL)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_coo_group_ptr: bad size");
+  if (!group_ptr || (coo_nnz > 0 && !coo_rows))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_coo_group_ptr: null pointer");
+  const int64_t groups = num_groups(num_rows);
+  const int64_t threads = groups + 1;
+  const int64_t blocks = (threads + 255) / 256;
+  hipLaunchKernelGGL(k_group_ptr, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, hs(stream),
+                     coo_rows, coo_nnz, groups, group_ptr);
+  return check_hip(hipGetLastError(), "hbdia_coo_group_ptr");
+}
+int hbdia_vector_add_f64(double* dst, const double* src, int64_t n, hbdia_stream_t stream) {
+  return launch_vector_add<double>(dst, src, n, stream, "hbdia_vector_add_f64");
+}
+int hbdia_vector_add_f32(float* dst, const float* src, int64_t n, hbdia_stream_t stream) {
+  return launch_vector_add<float>(dst, src, n, stream, "hbdia_vector_add_f32");
+}
+
+}  // extern "C"

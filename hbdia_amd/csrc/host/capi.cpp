@@ -1,0 +1,391 @@
+// capi.cpp -- C handle API over the C++ HBDIA classes (see hbdia_capi.h).
+#include "hbdia_capi.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "Format/HBDIA.hpp"
+#include "Format/HBDIAVector.hpp"
+#include "Operations/HBDIASpMV.hpp"
+#include "Workloads.hpp"
+
+struct hbdia_matrix_opaque {
+  int dtype = HBDIA_DTYPE_F64;
+  std::unique_ptr<HBDIA<double>> d;
+  std::unique_ptr<HBDIA<float>> f;
+};
+
+struct hbdia_vector_opaque {
+  int dtype = HBDIA_DTYPE_F64;
+  std::unique_ptr<HBDIAVector<double>> d;
+  std::unique_ptr<HBDIAVector<float>> f;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F>
+int guard(const char* what, F&& fn) {
+  try {
+    fn();
+    return 0;
+  } catch (const hbdia::DeviceError& e) {
+    g_err = std::string(what) + ": " + e.what();
+    return e.code() ? e.code() : 2;
+  } catch (const std::invalid_argument& e) {
+    g_err = std::string(what) + ": invalid_argument: " + e.what();
+    return 1;
+  } catch (const std::out_of_range& e) {
+    g_err = std::string(what) + ": out_of_range: " + e.what();
+    return 1;
+  } catch (const std::exception& e) {
+    g_err = std::string(what) + ": " + e.what();
+    return 5;
+  } catch (...) {
+    g_err = std::string(what) + ": unknown exception";
+    return 5;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) throw std::invalid_argument(msg);
+}
+
+template <typename T>
+void info_of(const HBDIA<T>& m, int64_t* info) {
+  info[HBDIA_INFO_ROWS] = m.getNumRows();
+  info[HBDIA_INFO_COLS] = m.getNumCols();
+  info[HBDIA_INFO_NNZ] = m.getNumNonZeros();
+  info[HBDIA_INFO_HAS_COO] = m.getHasCOO();
+  info[HBDIA_INFO_HAS_HBDIA] = m.getHasHBDIA();
+  info[HBDIA_INFO_BLOCK_WIDTH] = m.getBlockWidth();
+  info[HBDIA_INFO_THRESHOLD] = m.getThreshold();
+  info[HBDIA_INFO_NUM_BLOCKS] = m.getNumBlocks();
+  info[HBDIA_INFO_NUM_VALUES] = static_cast<int64_t>(m.getHBDIAData().size());
+  int64_t noff = 0;
+  for (const auto& o : m.getOffsetsPerBlock()) noff += static_cast<int64_t>(o.size());
+  info[HBDIA_INFO_NUM_OFFSETS] = noff;
+  info[HBDIA_INFO_NUM_COO] = static_cast<int64_t>(m.getCpuValues().size());
+  info[HBDIA_INFO_NUM_DIAGONALS] = m.getNumberDiagonals();
+  info[HBDIA_INFO_HIST_LEN] = static_cast<int64_t>(m.getHistogramBlocks().size());
+  info[HBDIA_INFO_ON_DEVICE] = m.areStreamsInitialized();
+  info[HBDIA_INFO_DEVICE_COO] = m.hasDeviceCOO();
+  info[HBDIA_INFO_STORED_BYTES] = static_cast<int64_t>(m.storedBytes());
+}
+
+template <typename T>
+void get_hbdia(const HBDIA<T>& m, void* values, int32_t* offsets, int32_t* blockStart, int32_t* cr, int32_t* cc,
+               void* cv, int32_t* rp, int32_t* hb, int32_t* hn) {
+  if (values) std::copy(m.getHBDIAData().begin(), m.getHBDIAData().end(), static_cast<T*>(values));
+  const auto& ob = m.getOffsetsPerBlock();
+  int64_t pos = 0;
+  for (size_t b = 0; b < ob.size(); ++b) {
+    if (blockStart) blockStart[b] = static_cast<int32_t>(pos);
+    if (offsets) std::copy(ob[b].begin(), ob[b].end(), offsets + pos);
+    pos += static_cast<int64_t>(ob[b].size());
+  }
+  if (blockStart) blockStart[ob.size()] = static_cast<int32_t>(pos);
+  if (cr) std::copy(m.getCpuRowIndices().begin(), m.getCpuRowIndices().end(), cr);
+  if (cc) std::copy(m.getCpuColIndices().begin(), m.getCpuColIndices().end(), cc);
+  if (cv) std::copy(m.getCpuValues().begin(), m.getCpuValues().end(), static_cast<T*>(cv));
+  if (rp) std::copy(m.getCpuRowPtr().begin(), m.getCpuRowPtr().end(), rp);
+  if (hb) std::copy(m.getHistogramBlocks().begin(), m.getHistogramBlocks().end(), hb);
+  if (hn) std::copy(m.getHistogramNnz().begin(), m.getHistogramNnz().end(), hn);
+}
+
+template <typename T>
+void dev_ptrs(const HBDIA<T>& m, void** p) {
+  p[0] = const_cast<T*>(m.getHBDIADataDevice());
+  p[1] = const_cast<int*>(m.getFlattenedOffsetsDevice());
+  p[2] = const_cast<int*>(m.getBlockStartIndicesDevice());
+  p[3] = const_cast<int*>(m.getBlockSizesDevice());
+  p[4] = const_cast<int*>(m.getCpuRowIndicesDevice());
+  p[5] = const_cast<int*>(m.getCpuColIndicesDevice());
+  p[6] = const_cast<T*>(m.getCpuValuesDevice());
+  p[7] = const_cast<int*>(m.getCpuRowPtrDevice());
+  p[8] = const_cast<int*>(m.getCooGroupPtrDevice());
+  p[9] = m.getBDIAStream();
+}
+
+template <typename T>
+std::unique_ptr<HBDIA<T>> from_coo(const int32_t* r, const int32_t* c, const void* v, int64_t nnz, int32_t nr,
+                                   int32_t nc) {
+  need(nnz >= 0, "nnz must be non-negative");
+  need(nnz == 0 || (r && c && v), "null COO pointer");
+  std::vector<int> rr(r, r + nnz), cc(c, c + nnz);
+  std::vector<T> vv(static_cast<const T*>(v), static_cast<const T*>(v) + nnz);
+  if (nr <= 0 && nc <= 0) return std::make_unique<HBDIA<T>>(rr, cc, vv);
+  return std::make_unique<HBDIA<T>>(std::move(rr), std::move(cc), std::move(vv), nr, nc);
+}
+
+template <typename T>
+std::unique_ptr<HBDIA<T>> from_gen(hbdia::CooMatrix<T>& g) {
+  const int nr = g.numRows, nc = g.numCols;
+  return std::make_unique<HBDIA<T>>(std::move(g.rows), std::move(g.cols), std::move(g.vals), nr, nc);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hbdia_capi_last_error(void) { return g_err.c_str(); }
+
+int hbdia_set_num_threads(int n) {
+  return guard("hbdia_set_num_threads", [&] {
+    need(n > 0, "thread count must be positive");
+    omp_set_num_threads(n);
+  });
+}
+int hbdia_get_num_threads(void) { return omp_get_max_threads(); }
+
+int hbdia_matrix_create(hbdia_matrix_t* m, int dtype) {
+  return guard("hbdia_matrix_create", [&] {
+    need(m != nullptr, "null handle pointer");
+    need(dtype == HBDIA_DTYPE_F64 || dtype == HBDIA_DTYPE_F32, "dtype must be F64 or F32");
+    auto h = std::make_unique<hbdia_matrix_opaque>();
+    h->dtype = dtype;
+    if (dtype == HBDIA_DTYPE_F64) h->d = std::make_unique<HBDIA<double>>();
+    else h->f = std::make_unique<HBDIA<float>>();
+    *m = h.release();
+  });
+}
+
+int hbdia_matrix_create_coo(hbdia_matrix_t* m, int dtype, const int32_t* rows, const int32_t* cols,
+                            const void* values, int64_t nnz, int32_t num_rows, int32_t num_cols) {
+  return guard("hbdia_matrix_create_coo", [&] {
+    need(m != nullptr, "null handle pointer");
+    need(dtype == HBDIA_DTYPE_F64 || dtype == HBDIA_DTYPE_F32, "dtype must be F64 or F32");
+    auto h = std::make_unique<hbdia_matrix_opaque>();
+    h->dtype = dtype;
+    if (dtype == HBDIA_DTYPE_F64) h->d = from_coo<double>(rows, cols, values, nnz, num_rows, num_cols);
+    else h->f = from_coo<float>(rows, cols, values, nnz, num_rows, num_cols);
+    *m = h.release();
+  });
+}
+
+int hbdia_matrix_destroy(hbdia_matrix_t m) {
+  return guard("hbdia_matrix_destroy", [&] { delete m; });
+}
+
+#define DISPATCH(m, expr_d, expr_f)             \
+  do {                                          \
+    need((m) != nullptr, "null matrix handle"); \
+    if ((m)->dtype == HBDIA_DTYPE_F64) {        \
+      auto& M = *(m)->d;                        \
+      expr_d;                                   \
+    } else {                                    \
+      auto& M = *(m)->f;                        \
+      expr_f;                                   \
+    }                                           \
+  } while (0)
+
+int hbdia_matrix_stencil27(hbdia_matrix_t m, int nx, int ny, int nz, double noise, int iteration) {
+  return guard("hbdia_matrix_stencil27", [&] {
+    need(nx > 0 && ny > 0 && nz > 0, "grid dimensions must be positive");
+    DISPATCH(m, M.create3DStencil27Point(nx, ny, nz, noise, iteration),
+             M.create3DStencil27Point(nx, ny, nz, noise, iteration));
+  });
+}
+
+int hbdia_matrix_convert(hbdia_matrix_t m, int bw, int thr, int max_coo, int coo_unique, int gpu_coo) {
+  return guard("hbdia_matrix_convert", [&] {
+    const ExecutionMode em = gpu_coo ? ExecutionMode::GPU_COO : ExecutionMode::CPU_COO;
+    DISPATCH(m, M.convertToHBDIAFormat(bw, thr, max_coo, coo_unique != 0, em),
+             M.convertToHBDIAFormat(bw, thr, max_coo, coo_unique != 0, em));
+  });
+}
+
+int hbdia_matrix_convert_host(hbdia_matrix_t m, int bw, int thr, int max_coo, int coo_unique, int gpu_coo) {
+  return guard("hbdia_matrix_convert_host", [&] {
+    const ExecutionMode em = gpu_coo ? ExecutionMode::GPU_COO : ExecutionMode::CPU_COO;
+    DISPATCH(m, M.convertToHBDIAFormatOnHost(bw, thr, max_coo, coo_unique != 0, em),
+             M.convertToHBDIAFormatOnHost(bw, thr, max_coo, coo_unique != 0, em));
+  });
+}
+
+int hbdia_matrix_remove_duplicates(hbdia_matrix_t m) {
+  return guard("hbdia_matrix_remove_duplicates", [&] { DISPATCH(m, M.removeCOODuplicates(), M.removeCOODuplicates()); });
+}
+
+int hbdia_matrix_prepare_gpu(hbdia_matrix_t m) {
+  return guard("hbdia_matrix_prepare_gpu", [&] { DISPATCH(m, M.prepareForGPU(), M.prepareForGPU()); });
+}
+
+int hbdia_matrix_synchronize(hbdia_matrix_t m) {
+  return guard("hbdia_matrix_synchronize", [&] { DISPATCH(m, M.synchronize(), M.synchronize()); });
+}
+
+int hbdia_matrix_info(hbdia_matrix_t m, int64_t* info) {
+  return guard("hbdia_matrix_info", [&] {
+    need(info != nullptr, "null info");
+    DISPATCH(m, info_of(M, info), info_of(M, info));
+  });
+}
+
+int hbdia_matrix_get_coo(hbdia_matrix_t m, int32_t* rows, int32_t* cols, void* values) {
+  return guard("hbdia_matrix_get_coo", [&] {
+    DISPATCH(
+        m,
+        {
+          if (rows) std::copy(M.getRowIndices().begin(), M.getRowIndices().end(), rows);
+          if (cols) std::copy(M.getColIndices().begin(), M.getColIndices().end(), cols);
+          if (values) std::copy(M.getValues().begin(), M.getValues().end(), static_cast<double*>(values));
+        },
+        {
+          if (rows) std::copy(M.getRowIndices().begin(), M.getRowIndices().end(), rows);
+          if (cols) std::copy(M.getColIndices().begin(), M.getColIndices().end(), cols);
+          if (values) std::copy(M.getValues().begin(), M.getValues().end(), static_cast<float*>(values));
+        });
+  });
+}
+
+int hbdia_matrix_get_hbdia(hbdia_matrix_t m, void* values, int32_t* offsets, int32_t* block_start,
+                           int32_t* coo_rows, int32_t* coo_cols, void* coo_values, int32_t* row_ptr,
+                           int32_t* hist_blocks, int32_t* hist_nnz) {
+  return guard("hbdia_matrix_get_hbdia", [&] {
+    DISPATCH(m, get_hbdia(M, values, offsets, block_start, coo_rows, coo_cols, coo_values, row_ptr, hist_blocks, hist_nnz),
+             get_hbdia(M, values, offsets, block_start, coo_rows, coo_cols, coo_values, row_ptr, hist_blocks, hist_nnz));
+  });
+}
+
+int hbdia_matrix_device_ptrs(hbdia_matrix_t m, void** ptrs) {
+  return guard("hbdia_matrix_device_ptrs", [&] {
+    need(ptrs != nullptr, "null output");
+    DISPATCH(m, dev_ptrs(M, ptrs), dev_ptrs(M, ptrs));
+  });
+}
+
+int hbdia_vector_create(hbdia_vector_t* v, int dtype, const void* data, int64_t n, int unified, int umalloc,
+                        int managed, int numa) {
+  return guard("hbdia_vector_create", [&] {
+    need(v != nullptr, "null handle pointer");
+    need(n >= 0 && (n == 0 || data), "bad data");
+    need(dtype == HBDIA_DTYPE_F64 || dtype == HBDIA_DTYPE_F32, "dtype must be F64 or F32");
+    auto h = std::make_unique<hbdia_vector_opaque>();
+    h->dtype = dtype;
+    if (dtype == HBDIA_DTYPE_F64) {
+      std::vector<double> d(static_cast<const double*>(data), static_cast<const double*>(data) + n);
+      h->d = std::make_unique<HBDIAVector<double>>(d, unified != 0, umalloc != 0, managed != 0, numa != 0);
+    } else {
+      std::vector<float> d(static_cast<const float*>(data), static_cast<const float*>(data) + n);
+      h->f = std::make_unique<HBDIAVector<float>>(d, unified != 0, umalloc != 0, managed != 0, numa != 0);
+    }
+    *v = h.release();
+  });
+}
+
+int hbdia_vector_destroy(hbdia_vector_t v) {
+  return guard("hbdia_vector_destroy", [&] { delete v; });
+}
+
+int hbdia_vector_upload(hbdia_vector_t v, const void* data, int64_t n) {
+  return guard("hbdia_vector_upload", [&] {
+    need(v != nullptr && (n == 0 || data), "bad argument");
+    if (v->dtype == HBDIA_DTYPE_F64)
+      v->d->upload(std::vector<double>(static_cast<const double*>(data), static_cast<const double*>(data) + n));
+    else
+      v->f->upload(std::vector<float>(static_cast<const float*>(data), static_cast<const float*>(data) + n));
+  });
+}
+
+int hbdia_vector_download(hbdia_vector_t v, void* out, int64_t n) {
+  return guard("hbdia_vector_download", [&] {
+    need(v != nullptr && out != nullptr, "bad argument");
+    const size_t es = v->dtype == HBDIA_DTYPE_F64 ? sizeof(double) : sizeof(float);
+    const size_t local = v->dtype == HBDIA_DTYPE_F64 ? v->d->getLocalSize() : v->f->getLocalSize();
+    need(static_cast<size_t>(n) == local, "size mismatch");
+    void* src = v->dtype == HBDIA_DTYPE_F64 ? static_cast<void*>(v->d->getDeviceLocalPtr())
+                                            : static_cast<void*>(v->f->getDeviceLocalPtr());
+    CHECK_HBDIA(hbdia_memcpy(out, src, local * es));
+  });
+}
+
+int hbdia_vector_download_coo(hbdia_vector_t v, void* out, int64_t n) {
+  return guard("hbdia_vector_download_coo", [&] {
+    need(v != nullptr && out != nullptr, "bad argument");
+    const size_t es = v->dtype == HBDIA_DTYPE_F64 ? sizeof(double) : sizeof(float);
+    const size_t local = v->dtype == HBDIA_DTYPE_F64 ? v->d->getLocalSize() : v->f->getLocalSize();
+    need(static_cast<size_t>(n) == local, "size mismatch");
+    void* src = v->dtype == HBDIA_DTYPE_F64 ? static_cast<void*>(v->d->getDeviceCOOResultsPtr())
+                                            : static_cast<void*>(v->f->getDeviceCOOResultsPtr());
+    CHECK_HBDIA(hbdia_memcpy(out, src, local * es));
+  });
+}
+
+int hbdia_vector_device_ptrs(hbdia_vector_t v, void** local, void** coo) {
+  return guard("hbdia_vector_device_ptrs", [&] {
+    need(v != nullptr, "null vector handle");
+    if (v->dtype == HBDIA_DTYPE_F64) {
+      if (local) *local = v->d->getDeviceLocalPtr();
+      if (coo) *coo = v->d->getDeviceCOOResultsPtr();
+    } else {
+      if (local) *local = v->f->getDeviceLocalPtr();
+      if (coo) *coo = v->f->getDeviceCOOResultsPtr();
+    }
+  });
+}
+
+int hbdia_spmv(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_coo_cpu, int exec_coo_gpu) {
+  return guard("hbdia_spmv", [&] {
+    need(m && x && y, "null handle");
+    need(m->dtype == x->dtype && m->dtype == y->dtype, "dtype mismatch between matrix and vectors");
+    if (m->dtype == HBDIA_DTYPE_F64) hbdiaSpMV<double>(*m->d, *x->d, *y->d, exec_coo_cpu != 0, exec_coo_gpu != 0);
+    else hbdiaSpMV<float>(*m->f, *x->f, *y->f, exec_coo_cpu != 0, exec_coo_gpu != 0);
+  });
+}
+
+int hbdia_gen_stencil(hbdia_matrix_t* m, int dtype, int n, int points, double scatter, uint64_t seed) {
+  return guard("hbdia_gen_stencil", [&] {
+    need(m != nullptr, "null handle pointer");
+    need(dtype == HBDIA_DTYPE_F64 || dtype == HBDIA_DTYPE_F32, "dtype must be F64 or F32");
+    auto h = std::make_unique<hbdia_matrix_opaque>();
+    h->dtype = dtype;
+    if (dtype == HBDIA_DTYPE_F64) {
+      hbdia::CooMatrix<double> g;
+      hbdia::generateStencilScatter<double>(n, points, scatter, seed, g);
+      h->d = from_gen(g);
+    } else {
+      hbdia::CooMatrix<float> g;
+      hbdia::generateStencilScatter<float>(n, points, scatter, seed, g);
+      h->f = from_gen(g);
+    }
+    *m = h.release();
+  });
+}
+
+int hbdia_gen_banded(hbdia_matrix_t* m, int dtype, int n, int noffsets, int max_offset, double offband,
+                     uint64_t seed) {
+  return guard("hbdia_gen_banded", [&] {
+    need(m != nullptr, "null handle pointer");
+    need(dtype == HBDIA_DTYPE_F64 || dtype == HBDIA_DTYPE_F32, "dtype must be F64 or F32");
+    auto h = std::make_unique<hbdia_matrix_opaque>();
+    h->dtype = dtype;
+    if (dtype == HBDIA_DTYPE_F64) {
+      hbdia::CooMatrix<double> g;
+      hbdia::generateBanded<double>(n, noffsets, max_offset, offband, seed, g);
+      h->d = from_gen(g);
+    } else {
+      hbdia::CooMatrix<float> g;
+      hbdia::generateBanded<float>(n, noffsets, max_offset, offband, seed, g);
+      h->f = from_gen(g);
+    }
+    *m = h.release();
+  });
+}
+
+int hbdia_gen_vector(int dtype, int64_t n, uint64_t seed, double lo, double hi, void* out) {
+  return guard("hbdia_gen_vector", [&] {
+    need(n >= 0 && (n == 0 || out), "bad argument");
+    if (dtype == HBDIA_DTYPE_F64) hbdia::generateUniformVector<double>(n, seed, lo, hi, static_cast<double*>(out));
+    else if (dtype == HBDIA_DTYPE_F32) hbdia::generateUniformVector<float>(n, seed, lo, hi, static_cast<float*>(out));
+    else throw std::invalid_argument("dtype must be F64 or F32");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+"""Python callers of the device kernels exported by libhbdia_hip.so.
+
+Each function launches one C-ABI entry point on DeviceArray operands; the
+semantics are documented in include/hbdia_hip.h.  Launches are asynchronous
+on `stream` (None = null stream).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._native import check_hip, hip
+from .device import DeviceArray, Stream
+
+ROW_GROUP = 64
+
+
+def _sfx(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return "f64"
+    if dt == np.float32:
+        return "f32"
+    raise TypeError(f"unsupported value dtype {dt}")
+
+
+def _h(stream):
+    return stream.handle if stream is not None else None
+
+
+def num_groups(num_rows: int) -> int:
+    return (int(num_rows) + ROW_GROUP - 1) // ROW_GROUP
+
+
+def bdia_spmv(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArray, x: DeviceArray,
+              y: DeviceArray, num_rows: int, num_blocks: int, block_width: int,
+              stream: Stream | None = None) -> None:
+    fn = getattr(hip, "hbdia_bdia_spmv_" + _sfx(values.dtype))
+    check_hip(fn(values.ptr, offsets.ptr, block_start.ptr, x.ptr, y.ptr, int(num_rows), int(num_blocks),
+                 int(block_width), _h(stream)))
+
+
+def hbdia_spmv(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArray, num_blocks: int,
+               block_width: int, coo_rows: DeviceArray, coo_cols: DeviceArray, coo_values: DeviceArray,
+               coo_group_ptr: DeviceArray, x: DeviceArray, y: DeviceArray, num_rows: int,
+               stream: Stream | None = None) -> None:
+    fn = getattr(hip, "hbdia_hbdia_spmv_" + _sfx(values.dtype))
+    check_hip(fn(values.ptr, offsets.ptr, block_start.ptr, int(num_blocks), int(block_width), coo_rows.ptr,
+                 coo_cols.ptr, coo_values.ptr, coo_group_ptr.ptr, x.ptr, y.ptr, int(num_rows), _h(stream)))
+
+
+def coo_spmv(coo_rows: DeviceArray, coo_cols: DeviceArray, coo_values: DeviceArray, coo_group_ptr: DeviceArray,
+             x: DeviceArray, res: DeviceArray, num_rows: int, stream: Stream | None = None) -> None:
+    fn = getattr(hip, "hbdia_coo_spmv_" + _sfx(coo_values.dtype))
+    check_hip(fn(coo_rows.ptr, coo_cols.ptr, coo_values.ptr, coo_group_ptr.ptr, x.ptr, res.ptr, int(num_rows),
+                 _h(stream)))
+
+
+def coo_group_ptr(coo_rows: DeviceArray, nnz: int, num_rows: int, out: DeviceArray,
+                  stream: Stream | None = None) -> None:
+    if out.size < num_groups(num_rows) + 1:
+        raise ValueError("group pointer buffer too small")
+    check_hip(hip.hbdia_coo_group_ptr(coo_rows.ptr if coo_rows is not None else None, int(nnz), int(num_rows),
+                                      out.ptr, _h(stream)))
+
+
+def vector_add(dst: DeviceArray, src: DeviceArray, n: int, stream: Stream | None = None) -> None:
+    fn = getattr(hip, "hbdia_vector_add_" + _sfx(dst.dtype))
+    check_hip(fn(dst.ptr, src.ptr, int(n), _h(stream)))
+
+
+class DeviceLayout:
+    """Host HBDIA arrays uploaded for the raw kernels (no C++ objects).
+
+    values/offsets/block_start describe the blocked-DIA part; coo_* the
+    row-sorted fallback.  The group pointer is built on the device.
+    """
+
+    def __init__(self, values: np.ndarray, offsets: np.ndarray, block_start: np.ndarray, num_rows: int,
+                 block_width: int, coo_rows=None, coo_cols=None, coo_values=None, stream: Stream | None = None):
+        self.num_rows = int(num_rows)
+        self.block_width = int(block_width)
+        self.num_blocks = int(len(block_start) - 1)
+        self.dtype = np.dtype(values.dtype)
+        self.values = DeviceArray.from_host(values if values.size else np.zeros(1, self.dtype))
+        self.offsets = DeviceArray.from_host((offsets if offsets.size else np.zeros(1, np.int32)).astype(np.int32))
+        self.block_start = DeviceArray.from_host(np.asarray(block_start, dtype=np.int32))
+        self.coo_nnz = 0 if coo_rows is None else int(len(coo_rows))
+        if self.coo_nnz:
+            rows = np.asarray(coo_rows, dtype=np.int32)
+            if np.any(np.diff(rows) < 0):
+                raise ValueError("COO rows must be sorted")
+            self.coo_rows = DeviceArray.from_host(rows)
+            self.coo_cols = DeviceArray.from_host(np.asarray(coo_cols, dtype=np.int32))
+            self.coo_values = DeviceArray.from_host(np.asarray(coo_values, dtype=self.dtype))
+            self.group_ptr = DeviceArray(num_groups(self.num_rows) + 1, np.int32)
+            coo_group_ptr(self.coo_rows, self.coo_nnz, self.num_rows, self.group_ptr, stream)
+            if stream is not None:
+                stream.synchronize()
+            else:
+                from .device import synchronize
+                synchronize()
+
+    def spmv(self, x: DeviceArray, y: DeviceArray, stream: Stream | None = None) -> None:
+        """y = A x with the fused kernel (blocked-DIA only when there is no fallback)."""
+        if self.coo_nnz:
+            hbdia_spmv(self.values, self.offsets, self.block_start, self.num_blocks, self.block_width,
+                       self.coo_rows, self.coo_cols, self.coo_values, self.group_ptr, x, y, self.num_rows, stream)
+        else:
+            bdia_spmv(self.values, self.offsets, self.block_start, x, y, self.num_rows, self.num_blocks,
+                      self.block_width, stream)

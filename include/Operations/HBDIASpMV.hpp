@@ -1,0 +1,45 @@
+// HBDIASpMV.hpp -- the HBDIA sparse matrix-vector product y = A x.
+//
+// Drop-in for the reference's include/Operations/HBDIASpMV.hpp:8-21.
+//   hbdiaSpMV(A, x, y, execCOOCPU, execCOOGPU)
+//     * no COO fallback: one blocked-DIA launch on A's BDIA stream;
+//     * execCOOCPU (wins when both flags are set): the reference sequence --
+//       BDIA launch, x copied to the host, OpenMP COO SpMV into the vector's
+//       COO buffer, copy back, y += coo on the ADD stream after both events;
+//     * execCOOGPU: one fused launch (blocked-DIA + COO + merge) on the BDIA
+//       stream; needs ExecutionMode::GPU_COO at conversion time;
+//     * COO present and both flags false: the BDIA part is launched, then
+//       std::runtime_error("Invalid COO execution flags") is thrown.
+//   Asynchronous like the reference: call A.synchronize() (or
+//   hbdia_device_synchronize()) before reading y.  Returns true.
+#ifndef HBDIASPMV_HPP
+#define HBDIASPMV_HPP
+
+#include "../Format/HBDIA.hpp"
+#include "../Format/HBDIAVector.hpp"
+
+template <typename T>
+bool hbdiaSpMV(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector, HBDIAVector<T>& outputVector,
+               bool execCooCPU, bool execCooGPU);
+
+// COO fallback on the host (reference HBDIASpMV.cpp:101-131).
+template <typename T>
+void executeCOOOnCPU(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector,
+                     HBDIAVector<T>& outputVector, int numRows, hbdia_stream_t sD2H,
+                     hbdia_stream_t sH2D, hbdia_stream_t sCOO, hbdia_event_t cooEvent);
+
+// COO fallback on the device into the vector's COO buffer (reference
+// HBDIASpMV.cpp:133-183, which calls cuSPARSE there).
+template <typename T>
+void executeCOOOnGPU(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector,
+                     HBDIAVector<T>& outputVector, int numRows, hbdia_stream_t sCOO,
+                     hbdia_event_t cooEvent);
+
+// OpenMP COO kernel of the CPU path (reference HBDIASpMV.cpp:35-99): rows with
+// fallback entries get their sequential sum in entry order; others are left.
+template <typename T>
+void hbdia_cpu_coo_spmv(const std::vector<int>& cpuRowIndices, const std::vector<int>& cpuColIndices,
+                        const std::vector<T>& cpuValues, const T* inputVector, int numRows,
+                        T* cpuResults);
+
+#endif  // HBDIASPMV_HPP

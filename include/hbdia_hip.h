@@ -1,0 +1,141 @@
+/* hbdia_hip.h -- C ABI of libhbdia_hip.so, the device side of the HBDIA SpMV
+ * path on AMD Instinct MI355X (gfx950).
+ *
+ * Every entry point returns an int status (HBDIA_SUCCESS == 0) and never exits
+ * or throws; hbdia_last_error() returns a description of the last failure on
+ * the calling thread.  Plain pointers and sizes only.  Device pointers come
+ * from hbdia_malloc (or any HIP device allocation).  A NULL stream means the
+ * default (null) stream.
+ *
+ * Reference interfaces each group replaces (paths relative to the reference
+ * checkout):
+ *   memory / copies  -> cudaMalloc / cudaMemcpy / cudaMemset / cudaMallocHost
+ *                       calls in src/Format/HBDIA.cpp:768-814 and
+ *                       src/Format/HBDIAVector.cpp:84-153
+ *   streams / events -> HBDIA<T>::initializeStreams, src/Format/HBDIA.cpp:858-876,
+ *                       and the event waits in src/Operations/HBDIASpMV.cpp:221,236-239
+ *   BDIA kernel      -> launch_bdia_spmv_kernel<T>, src/Operations/HBDIASpMV.cu:91-117
+ *   vector add       -> launch_vector_add_kernel<T>, src/Operations/HBDIASpMV.cu:120-135
+ *   COO fallback     -> executeCOOOnGPU<T> (cusparseSpMV), src/Operations/HBDIASpMV.cpp:133-183
+ */
+#ifndef HBDIA_HIP_H
+#define HBDIA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Rows handled by one 64-lane wavefront; COO group pointers are per group. */
+#define HBDIA_ROW_GROUP 64
+
+typedef struct hbdia_stream_opaque* hbdia_stream_t;
+typedef struct hbdia_event_opaque* hbdia_event_t;
+typedef struct hbdia_graph_opaque* hbdia_graph_t;
+
+enum {
+  HBDIA_SUCCESS = 0,
+  HBDIA_ERROR_INVALID_ARGUMENT = 1,
+  HBDIA_ERROR_DEVICE = 2,
+  HBDIA_ERROR_OUT_OF_MEMORY = 3,
+  HBDIA_ERROR_NOT_SUPPORTED = 4
+};
+
+/* ---- errors, devices ---------------------------------------------------- */
+const char* hbdia_last_error(void);
+int hbdia_get_device_count(int* count);
+int hbdia_set_device(int device);
+int hbdia_get_device(int* device);
+int hbdia_device_synchronize(void);
+int hbdia_mem_info(size_t* free_bytes, size_t* total_bytes);
+int hbdia_device_name(char* buf, size_t len);
+
+/* ---- memory (HBDIA.cpp:768-814, HBDIAVector.cpp:84-153) ------------------ */
+int hbdia_malloc(void** ptr, size_t bytes);
+int hbdia_free(void* ptr);
+int hbdia_malloc_host(void** ptr, size_t bytes);    /* pinned host memory */
+int hbdia_free_host(void* ptr);
+int hbdia_malloc_managed(void** ptr, size_t bytes); /* hipMallocManaged */
+int hbdia_memcpy_h2d(void* dst, const void* src, size_t bytes, hbdia_stream_t stream);
+int hbdia_memcpy_d2h(void* dst, const void* src, size_t bytes, hbdia_stream_t stream);
+int hbdia_memcpy_d2d(void* dst, const void* src, size_t bytes, hbdia_stream_t stream);
+int hbdia_memcpy(void* dst, const void* src, size_t bytes); /* synchronous, any direction */
+int hbdia_memset(void* dst, int value, size_t bytes, hbdia_stream_t stream);
+
+/* ---- streams, events, graphs (HBDIA.cpp:858-876, HBDIASpMV.cpp:221,236-239) */
+int hbdia_stream_create(hbdia_stream_t* stream); /* non-blocking, default priority */
+int hbdia_stream_destroy(hbdia_stream_t stream);
+int hbdia_stream_synchronize(hbdia_stream_t stream);
+int hbdia_event_create(hbdia_event_t* event);
+int hbdia_event_destroy(hbdia_event_t event);
+int hbdia_event_record(hbdia_event_t event, hbdia_stream_t stream);
+int hbdia_event_synchronize(hbdia_event_t event);
+int hbdia_stream_wait_event(hbdia_stream_t stream, hbdia_event_t event);
+int hbdia_event_elapsed_ms(float* ms, hbdia_event_t start, hbdia_event_t stop);
+int hbdia_graph_begin_capture(hbdia_stream_t stream);
+int hbdia_graph_end_capture(hbdia_stream_t stream, hbdia_graph_t* graph);
+int hbdia_graph_launch(hbdia_graph_t graph, hbdia_stream_t stream);
+int hbdia_graph_destroy(hbdia_graph_t graph);
+
+/* ---- kernels ------------------------------------------------------------- */
+/* Blocked-DIA part only; replaces launch_bdia_spmv_kernel<T>
+ * (HBDIASpMV.cu:91-117) with the semantics of bdia_spmv_kernel (:7-74):
+ *   for each row r < num_rows with block b = r / block_width < num_blocks and a
+ *   non-empty offset list, y[r] = sum over the block's offsets in ascending
+ *   order of values[(block_start[b] + o) * block_width + r % block_width] *
+ *   x[r + offsets[block_start[b] + o]], skipping zero values and columns
+ *   outside [0, num_rows), accumulated as FMAs.  Other rows keep y.
+ * values: sum_b (block_start[b+1]-block_start[b]) * block_width elements.
+ * block_start: num_blocks + 1 prefix offsets into `offsets`. */
+int hbdia_bdia_spmv_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                        const double* x, double* y, int64_t num_rows, int32_t num_blocks,
+                        int32_t block_width, hbdia_stream_t stream);
+int hbdia_bdia_spmv_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                        const float* x, float* y, int64_t num_rows, int32_t num_blocks,
+                        int32_t block_width, hbdia_stream_t stream);
+
+/* Blocked-DIA plus the COO fallback in one launch; replaces the BDIA launch,
+ * executeCOOOnGPU and the vector add of hbdiaSpMV (HBDIASpMV.cpp:186-251) when
+ * the fallback is non-empty.  Result: y[r] = bdia(r) + coo(r) for every row,
+ * where bdia(r) is as above (or the old y[r] for rows the BDIA part does not
+ * write) and coo(r) is the sequential FMA sum of the row's COO entries in entry
+ * order (+0 for rows without entries).  coo_rows must be sorted ascending;
+ * coo_group_ptr (ceil(num_rows/64)+1 entries) comes from hbdia_coo_group_ptr.
+ * x must hold every column the COO entries reference. */
+int hbdia_hbdia_spmv_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                         int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                         const int32_t* coo_cols, const double* coo_values,
+                         const int32_t* coo_group_ptr, const double* x, double* y,
+                         int64_t num_rows, hbdia_stream_t stream);
+int hbdia_hbdia_spmv_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                         int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                         const int32_t* coo_cols, const float* coo_values,
+                         const int32_t* coo_group_ptr, const float* x, float* y,
+                         int64_t num_rows, hbdia_stream_t stream);
+
+/* COO fallback alone, into a result buffer: res[r] = sequential FMA sum of the
+ * row's entries for rows that have entries; other rows keep res.  The GPU
+ * counterpart of cpu_coo_spmv_kernel (HBDIASpMV.cpp:35-99). */
+int hbdia_coo_spmv_f64(const int32_t* coo_rows, const int32_t* coo_cols, const double* coo_values,
+                       const int32_t* coo_group_ptr, const double* x, double* res,
+                       int64_t num_rows, hbdia_stream_t stream);
+int hbdia_coo_spmv_f32(const int32_t* coo_rows, const int32_t* coo_cols, const float* coo_values,
+                       const int32_t* coo_group_ptr, const float* x, float* res,
+                       int64_t num_rows, hbdia_stream_t stream);
+
+/* group_ptr[g] = first COO index whose row is >= 64 g, for g in [0, ceil(num_rows/64)];
+ * coo_rows sorted ascending, all in [0, num_rows). */
+int hbdia_coo_group_ptr(const int32_t* coo_rows, int64_t coo_nnz, int64_t num_rows,
+                        int32_t* group_ptr, hbdia_stream_t stream);
+
+/* dst[i] += src[i] for i < n; replaces launch_vector_add_kernel<T> (HBDIASpMV.cu:120-135). */
+int hbdia_vector_add_f64(double* dst, const double* src, int64_t n, hbdia_stream_t stream);
+int hbdia_vector_add_f32(float* dst, const float* src, int64_t n, hbdia_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HBDIA_HIP_H */

@@ -1,0 +1,267 @@
+"""GPU parity of the HIP path against the oracle (CPU restatement of the
+reference) on identical seeded inputs.
+
+Expected agreement: bit-exact.  The blocked-DIA part sums each row over its
+block's offsets in ascending order as FMAs (the reference kernel's order), and
+the COO part sums each row's fallback entries sequentially in entry order (the
+reference CPU path's order), then y = bdia + coo, so every rounding step is
+the same as the oracle's.  The stated tolerance of the path, 1e-12 relative
+per row in FP64, is therefore met with zero error; tests assert equality.
+"""
+from __future__ import annotations
+
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from hbdia_amd import HBDIA, DeviceArray, ExecutionMode, Graph, HBDIAVector, Stream, gen_vector, hbdiaSpMV
+from hbdia_amd import kernels
+from hbdia_amd.device import synchronize
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def oracle_y(A: HBDIA, x: np.ndarray, y0: np.ndarray) -> np.ndarray:
+    hb = A.getHBDIA()
+    hb["num_blocks"] = A.getNumBlocks()
+    hb["block_width"] = A.getBlockWidth()
+    return oracle.hbdia_spmv(hb, x.copy(), y0.copy(), A.getNumRows())
+
+
+def run_spmv(A: HBDIA, x: np.ndarray, y0: np.ndarray, cpu: bool, gpu_flag: bool) -> np.ndarray:
+    X = HBDIAVector(x)
+    Y = HBDIAVector(y0)
+    assert hbdiaSpMV(A, X, Y, cpu, gpu_flag)
+    A.synchronize()
+    return Y.download()
+
+
+def test_config1_reference_example_path(gpu, golden):
+    print("case: config1 8^3 bw32 thr2 execCOOCPU", flush=True)
+    A = HBDIA()
+    A.create3DStencil27Point(8, 8, 8, 0.0)
+    A.convertToHBDIAFormat(32, 2)
+    x = np.arange(1, 513, dtype=np.float64)
+    y = run_spmv(A, x, np.zeros(512), True, False)
+    np.testing.assert_array_equal(y, golden("config1_y.npz")["y"])
+    assert y[0] == pytest.approx(-73.14, rel=1e-13) and y[511] == pytest.approx(11610.51, rel=1e-13)
+
+
+def test_example_binary(gpu):
+    print("case: simple_hbdia_example binary", flush=True)
+    exe = ROOT / "hbdia_amd" / "lib" / "simple_hbdia_example"
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "PASS" in out.stdout
+
+
+@pytest.mark.parametrize("name,n,noise,it,bw,thr,xk", [("noisy16_y.npz", 16, 0.1, 0, 32, 2, "i+1"),
+                                                       ("noisy24_y.npz", 24, 0.02, 1, 16, 3, "sin")])
+@pytest.mark.parametrize("cpu_flag", [False, True])
+def test_golden_noisy(gpu, golden, name, n, noise, it, bw, thr, xk, cpu_flag):
+    print(f"case: golden {name} cpu_coo={cpu_flag}", flush=True)
+    A = HBDIA()
+    A.create3DStencil27Point(n, n, n, noise, it)
+    A.convertToHBDIAFormat(bw, thr)
+    N = n ** 3
+    x = np.arange(1, N + 1, dtype=np.float64) if xk == "i+1" else np.sin(np.arange(N, dtype=np.float64))
+    y = run_spmv(A, x, np.zeros(N), cpu_flag, not cpu_flag)
+    np.testing.assert_array_equal(y, golden(name)["y"])
+
+
+CASES = [
+    (16, 0.1, 0, 32, 2), (16, 0.3, 2, 64, 2), (12, 0.5, 3, 128, 1), (20, 0.05, 4, 48, 2),
+    (10, 1.0, 5, 1, 1), (17, 0.2, 6, 7, 4), (9, 0.0, 0, 64, 2), (13, 2.0, 7, 256, 30), (11, 0.1, 8, 8, 2),
+]
+
+
+@pytest.mark.parametrize("n,noise,it,bw,thr", CASES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("path", ["gpu_coo", "cpu_coo"])
+def test_spmv_bit_exact_vs_oracle(gpu, n, noise, it, bw, thr, dtype, path):
+    print(f"case: n={n} noise={noise} it={it} bw={bw} thr={thr} {np.dtype(dtype).name} {path}", flush=True)
+    A = HBDIA(dtype=dtype)
+    A.create3DStencil27Point(n, n, n, noise, it)
+    A.convertToHBDIAFormat(bw, thr)
+    N = n ** 3
+    x = np.cos(np.arange(N) * 0.731).astype(dtype)
+    y0 = np.full(N, 0.5, dtype=dtype)
+    y = run_spmv(A, x, y0, path == "cpu_coo", path == "gpu_coo")
+    np.testing.assert_array_equal(y, oracle_y(A, x, y0))
+
+
+def test_both_flags_false_with_fallback_raises(gpu):
+    A = HBDIA()
+    A.create3DStencil27Point(8, 8, 8, 0.2, 0)
+    A.convertToHBDIAFormat(32, 2)
+    assert A.info()["num_coo"] > 0
+    with pytest.raises(RuntimeError, match="Invalid COO execution flags"):
+        hbdiaSpMV(A, HBDIAVector(np.ones(512)), HBDIAVector(np.zeros(512)), False, False)
+    A.synchronize()
+
+
+def test_no_fallback_needs_no_flags(gpu):
+    A = HBDIA()
+    A.create3DStencil27Point(8, 8, 8)
+    A.convertToHBDIAFormat(32, 2)
+    y = run_spmv(A, np.ones(512), np.zeros(512), False, False)
+    np.testing.assert_allclose(y, np.bincount(A.getRowIndices(), weights=A.getValues(), minlength=512),
+                               rtol=1e-13)
+
+
+def test_gpu_coo_requires_gpu_mode(gpu):
+    A = HBDIA()
+    A.create3DStencil27Point(8, 8, 8, 0.2, 0)
+    A.convertToHBDIAFormat(32, 2, execMode=ExecutionMode.CPU_COO)
+    with pytest.raises(Exception, match="not on the device"):
+        hbdiaSpMV(A, HBDIAVector(np.ones(512)), HBDIAVector(np.zeros(512)), False, True)
+    y = run_spmv(A, np.ones(512), np.zeros(512), True, False)
+    np.testing.assert_array_equal(y, oracle_y(A, np.ones(512), np.zeros(512)))
+
+
+def test_repeated_calls_are_identical(gpu):
+    A = HBDIA.stencil(20, 27, 0.01, seed=11)
+    A.convertToHBDIAFormat(64, 2)
+    N = 20 ** 3
+    X = HBDIAVector(gen_vector(N, 3))
+    Y = HBDIAVector(np.zeros(N))
+    outs = []
+    for _ in range(4):
+        hbdiaSpMV(A, X, Y, False, True)
+        A.synchronize()
+        outs.append(Y.download())
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("n,bw", [(32, 64), (33, 64), (24, 32)])
+def test_stencil_scatter_workload(gpu, n, bw):
+    print(f"case: stencil+scatter n={n} bw={bw}", flush=True)
+    A = HBDIA.stencil(n, 27, 0.005, seed=0xC2)
+    A.convertToHBDIAFormat(bw, 2)
+    N = n ** 3
+    x = gen_vector(N, 7)
+    y = run_spmv(A, x, np.zeros(N), False, True)
+    np.testing.assert_array_equal(y, oracle_y(A, x, np.zeros(N)))
+    r, c, v = A.getCOO()
+    ref = oracle.coo_spmv_longdouble(r, c, v, x, N)
+    scale = oracle.coo_spmv_longdouble(r, c, np.abs(v), np.abs(x), N)
+    assert np.all(np.abs(y - ref) <= 1e-14 * scale)
+
+
+# ----------------------------------------------------------- raw C-ABI kernels
+def _layout(values, offsets, block_start, N, bw, coo=None):
+    if coo is None:
+        return kernels.DeviceLayout(np.asarray(values), np.asarray(offsets, np.int32),
+                                    np.asarray(block_start, np.int32), N, bw)
+    return kernels.DeviceLayout(np.asarray(values), np.asarray(offsets, np.int32), np.asarray(block_start, np.int32),
+                                N, bw, *coo)
+
+
+@pytest.mark.parametrize("bw", [4, 64])
+def test_raw_kernel_edge_semantics(gpu, bw):
+    """Zero values hide NaN, out-of-range columns are skipped, rows of empty
+    blocks and rows past the last block keep y (no fallback), and with a
+    fallback every row receives y += coo."""
+    rng = np.random.default_rng(bw)
+    nb = 3
+    N = nb * bw + bw // 2          # rows past the last block
+    offs_per = [[-1, 0, 5], [], [-2 * bw, 0, 1]]
+    block_start = np.cumsum([0] + [len(o) for o in offs_per]).astype(np.int32)
+    offsets = np.array(sum(offs_per, []), np.int32)
+    values = rng.standard_normal(len(offsets) * bw)
+    values[::7] = 0.0
+    x = rng.standard_normal(N)
+    x[3] = np.nan
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": nb, "block_width": bw}
+    y0 = rng.standard_normal(N)
+    L = _layout(values, offsets, block_start, N, bw)
+    X, Y = DeviceArray.from_host(x), DeviceArray.from_host(y0)
+    L.spmv(X, Y)
+    synchronize()
+    want = oracle.bdia_spmv(hb, x, y0.copy(), N)
+    np.testing.assert_array_equal(Y.download(), want)
+    # with a fallback: entries in an empty block, past the last block and in normal rows
+    coo_r = np.array(sorted([0, 0, bw + 1, bw + 1, nb * bw + 1, N - 1]), np.int32)
+    coo_c = np.array([N - 1, 2, 0, 1, 4, 0], np.int32)
+    coo_v = np.array([1.5, -2.0, 0.25, 4.0, 3.0, -1.0])
+    L2 = _layout(values, offsets, block_start, N, bw, (coo_r, coo_c, coo_v))
+    Y2 = DeviceArray.from_host(y0)
+    L2.spmv(X, Y2)
+    synchronize()
+    hb2 = dict(hb, coo_rows=coo_r, coo_cols=coo_c, coo_values=coo_v)
+    want2 = oracle.hbdia_spmv(hb2, x, y0.copy(), N)
+    np.testing.assert_array_equal(Y2.download(), want2)
+
+
+def test_raw_coo_kernel_and_group_pointer(gpu):
+    rng = np.random.default_rng(1)
+    N, nnz = 5000, 40000
+    rows = np.sort(rng.integers(0, N, nnz)).astype(np.int32)
+    rows[1000:1300] = rows[1000]            # a long row spanning several 64-entry batches
+    rows = np.sort(rows)
+    cols = rng.integers(0, N, nnz).astype(np.int32)
+    vals = rng.standard_normal(nnz)
+    x = rng.standard_normal(N)
+    R, C, V = DeviceArray.from_host(rows), DeviceArray.from_host(cols), DeviceArray.from_host(vals)
+    G = DeviceArray(kernels.num_groups(N) + 1, np.int32)
+    kernels.coo_group_ptr(R, nnz, N, G)
+    synchronize()
+    np.testing.assert_array_equal(G.download(), np.searchsorted(rows, np.arange(0, kernels.num_groups(N) + 1) * 64))
+    res0 = np.full(N, 9.0)
+    RES = DeviceArray.from_host(res0)
+    kernels.coo_spmv(R, C, V, G, DeviceArray.from_host(x), RES, N)
+    synchronize()
+    want = oracle.cpu_coo_spmv(rows, cols, vals, x, N, res0.copy())
+    np.testing.assert_array_equal(RES.download(), want)
+
+
+def test_vector_add(gpu):
+    a = np.arange(1000.0)
+    b = np.sqrt(np.arange(1000.0))
+    A, B = DeviceArray.from_host(a), DeviceArray.from_host(b)
+    kernels.vector_add(A, B, 1000)
+    synchronize()
+    np.testing.assert_array_equal(A.download(), a + b)
+
+
+def test_unified_memory_modes(gpu):
+    A = HBDIA()
+    A.create3DStencil27Point(10, 10, 10, 0.1, 1)
+    A.convertToHBDIAFormat(32, 2)
+    N = 1000
+    x = np.linspace(-1, 1, N)
+    want = oracle_y(A, x, np.zeros(N))
+    for flags in [(True, False, True, False), (True, True, False, False), (True, False, False, True)]:
+        X = HBDIAVector(x, *flags)
+        Y = HBDIAVector(np.zeros(N), *flags)
+        for cpu_flag in (True, False):
+            Y.upload(np.zeros(N))
+            hbdiaSpMV(A, X, Y, cpu_flag, not cpu_flag)
+            A.synchronize()
+            np.testing.assert_array_equal(Y.download(), want, err_msg=str((flags, cpu_flag)))
+
+
+def test_graph_capture_replays(gpu):
+    A = HBDIA.stencil(24, 27, 0.005, seed=5)
+    A.convertToHBDIAFormat(64, 2)
+    N = 24 ** 3
+    hb = A.getHBDIA()
+    s = Stream()
+    L = kernels.DeviceLayout(hb["values"], hb["offsets"], hb["block_start"], N, 64, hb["coo_rows"], hb["coo_cols"],
+                             hb["coo_values"], stream=s)
+    X = DeviceArray.from_host(gen_vector(N, 1))
+    Y = DeviceArray(N, np.float64, zero=True)
+    L.spmv(X, Y, s)
+    s.synchronize()
+    eager = Y.download()
+    g = Graph.capture(s, lambda: L.spmv(X, Y, s))
+    for rep in range(3):
+        Y.upload(np.full(N, float(rep)))
+        g.launch(s)
+        s.synchronize()
+        np.testing.assert_array_equal(Y.download(), eager)

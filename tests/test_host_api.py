@@ -1,0 +1,173 @@
+"""CPU tests of the product's host side against the oracle: the 27-point
+generator, the COO constructors and the parallel COO -> HBDIA conversion
+(without device upload), the seeded benchmark generators, error behaviour,
+and the exported C ABI.  No GPU calls."""
+from __future__ import annotations
+
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from hbdia_amd import HBDIA, ExecutionMode, _native
+
+ROOT = Path(__file__).resolve().parents[1]
+KEYS = ["values", "offsets", "block_start", "coo_rows", "coo_cols", "coo_values", "row_ptr", "hist_blocks",
+        "hist_nnz"]
+
+
+def assert_same_layout(A: HBDIA, ob: dict):
+    hb = A.getHBDIA()
+    for k in KEYS:
+        np.testing.assert_array_equal(hb[k], ob[k], err_msg=k)
+    assert A.getNumberDiagonals() == ob["num_diagonals"]
+    assert A.getNumBlocks() == ob["num_blocks"]
+
+
+@pytest.mark.parametrize("n,noise,it", [(8, 0.0, 0), (11, 0.07, 5), (16, 0.3, 2), (5, 2.0, 9)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_generator_matches_oracle(n, noise, it, dtype):
+    r, c, v, N = oracle.stencil27(n, n, n, noise, it, dtype=dtype)
+    A = HBDIA(dtype=dtype)
+    A.create3DStencil27Point(n, n, n, noise, it)
+    R, C, V = A.getCOO()
+    np.testing.assert_array_equal(R, r)
+    np.testing.assert_array_equal(C, c)
+    np.testing.assert_array_equal(V, v)
+    assert A.getNumRows() == N == A.getNumCols()
+
+
+@pytest.mark.parametrize("bw,thr,maxcoo", [(32, 2, 10 ** 9), (64, 2, 10 ** 9), (1, 1, 10 ** 9), (7, 3, 10 ** 9),
+                                           (128, 8, 10 ** 9), (16, 4, 50), (48, 100, 10 ** 9)])
+def test_conversion_matches_oracle_stencil(bw, thr, maxcoo):
+    r, c, v, N = oracle.stencil27(13, 13, 13, 0.05, 1)
+    A = HBDIA(r, c, v, N, N)
+    A.convertToHBDIAFormatOnHost(bw, thr, maxcoo)
+    assert_same_layout(A, oracle.convert(r, c, v, N, N, bw, thr, maxcoo))
+
+
+def test_conversion_unsorted_input_with_duplicates():
+    rng = np.random.default_rng(5)
+    n, m, nnz = 300, 280, 5000
+    rows = rng.integers(0, n, nnz).astype(np.int32)
+    cols = np.clip(rows + rng.integers(-6, 7, nnz), 0, m - 1).astype(np.int32)
+    rows[:100] = rows[100:200]  # force duplicate (row, col) pairs
+    cols[:100] = cols[100:200]
+    vals = rng.standard_normal(nnz)
+    for bw, thr in [(16, 3), (64, 1), (5, 2)]:
+        A = HBDIA(rows, cols, vals, n, m)
+        A.convertToHBDIAFormatOnHost(bw, thr)
+        assert_same_layout(A, oracle.convert(rows, cols, vals, n, m, bw, thr))
+
+
+def test_conversion_rows_beyond_last_block_and_wide_matrix():
+    # numCols < numRows: blocks cover min(N, M) rows only; the rest go to COO
+    rows = np.array([0, 1, 5, 9, 9, 3], np.int32)
+    cols = np.array([0, 1, 2, 3, 0, 3], np.int32)
+    vals = np.arange(1.0, 7.0)
+    A = HBDIA(rows, cols, vals, 10, 4)
+    A.convertToHBDIAFormatOnHost(2, 1)
+    assert_same_layout(A, oracle.convert(rows, cols, vals, 10, 4, 2, 1))
+
+
+def test_three_argument_constructor_infers_shape():
+    A = HBDIA([0, 2, 1], [3, 0, 1], [1.0, 2.0, 3.0])
+    assert A.getNumRows() == 3 and A.getNumCols() == 4 and A.getNumNonZeros() == 3
+
+
+def test_constructor_errors():
+    with pytest.raises(ValueError):
+        HBDIA([0, 1], [0], [1.0, 2.0], 2, 2)
+    with pytest.raises(IndexError):
+        HBDIA([0, 5], [0, 1], [1.0, 2.0], 3, 3)
+    with pytest.raises(IndexError):
+        HBDIA([0, 1], [0, -1], [1.0, 2.0], 3, 3)
+    with pytest.raises(ValueError):
+        HBDIA([0], [0], [1.0], 0, 3)
+
+
+def test_remove_duplicates_sums_in_input_order():
+    A = HBDIA([1, 0, 1, 1], [2, 0, 2, 0], [0.1, 5.0, 0.2, 3.0], 2, 3)
+    A.removeCOODuplicates()
+    r, c, v = A.getCOO()
+    assert r.tolist() == [0, 1, 1] and c.tolist() == [0, 0, 2]
+    assert v.tolist() == [5.0, 3.0, (0.0 + 0.1) + 0.2]
+
+
+def test_stencil_scatter_generator_properties():
+    n, frac = 12, 0.01
+    A = HBDIA.stencil(n, 27, frac, seed=123)
+    r, c, v = A.getCOO()
+    N = n ** 3
+    assert np.all(np.diff(r) >= 0)
+    assert len(v) == (3 * n - 2) ** 3 + int(frac * 27 * N)
+    assert len(set(zip(r.tolist(), c.tolist()))) == len(v)
+    diag = r == c
+    assert diag.sum() == N
+    off = ~diag
+    assert np.all((v[off] >= -1.0) & (v[off] < 0.0))
+    absum = np.bincount(r[off], weights=np.abs(v[off]), minlength=N)
+    np.testing.assert_allclose(v[diag], 1.0 + absum, rtol=1e-14)
+    # scatter entries are off-stencil
+    ri, rj, rk = r % n, (r // n) % n, r // (n * n)
+    ci, cj, ck = c % n, (c // n) % n, c // (n * n)
+    near = (np.abs(ri - ci) <= 1) & (np.abs(rj - cj) <= 1) & (np.abs(rk - ck) <= 1)
+    assert (~near).sum() == int(frac * 27 * N)
+    # same seed, same matrix; other seed, other matrix
+    r2, c2, v2 = HBDIA.stencil(n, 27, frac, seed=123).getCOO()
+    assert np.array_equal(c, c2) and np.array_equal(v, v2)
+    assert not np.array_equal(c, HBDIA.stencil(n, 27, frac, seed=124).getCOO()[1])
+
+
+def test_seven_point_and_banded_generators():
+    A = HBDIA.stencil(6, 7, 0.0, seed=1)
+    r, c, v = A.getCOO()
+    n = 6
+    assert len(v) == 7 * n ** 3 - 6 * n ** 2  # each of the 6 faces loses one neighbour per boundary row
+    B = HBDIA.banded(1000, 5, 64, 0.01, seed=3)
+    r, c, v = B.getCOO()
+    offs = np.unique((c - r)[np.abs(c - r) <= 64])
+    assert 0 in offs
+    assert np.all(np.diff(r) >= 0)
+
+
+def test_generator_thread_count_independent():
+    from hbdia_amd import set_num_threads
+    set_num_threads(1)
+    a = HBDIA.stencil(10, 27, 0.02, seed=9).getCOO()
+    set_num_threads(8)
+    b = HBDIA.stencil(10, 27, 0.02, seed=9).getCOO()
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def _declared(header: Path) -> list[str]:
+    text = header.read_text()
+    return re.findall(r"^\s*(?:int|const char\*)\s+(hbdia_\w+)\s*\(", text, flags=re.M)
+
+
+@pytest.mark.parametrize("header,lib,bound", [("hbdia_hip.h", "libhbdia_hip.so", _native.HIP_FUNCTIONS),
+                                              ("hbdia_capi.h", "libhbdia.so", _native.CAPI_FUNCTIONS)])
+def test_c_abi_exports_every_declared_symbol(header, lib, bound):
+    names = _declared(ROOT / "include" / header)
+    assert len(names) > 10
+    out = subprocess.run(["nm", "-D", "--defined-only", str(ROOT / "hbdia_amd" / "lib" / lib)],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    assert set(names) == set(bound), set(names) ^ set(bound)
+
+
+def test_kernel_calls_without_gpu_report_errors_not_crashes():
+    if _native.gpu_available():
+        pytest.skip("a GPU is visible")
+    import ctypes
+    p = ctypes.c_void_p()
+    assert _native.hip.hbdia_malloc(ctypes.byref(p), 64) != 0
+    assert _native.hip.hbdia_last_error()
+    # invalid arguments are rejected before any device call
+    assert _native.hip.hbdia_bdia_spmv_f64(None, None, None, None, None, 10, 1, 0, None) == 1
