@@ -4,15 +4,16 @@
 // Kernel design (HBM-bound; no MFMA, SpMV is ~0.2 flop/byte):
 //   * one 64-lane wavefront per group of 64 consecutive rows; four waves per
 //     256-thread workgroup;
-//   * workgroups are remapped so that each of the 8 XCDs (which receive
-//     workgroups round-robin) works on one contiguous eighth of the rows, so
-//     that the x window and its +-offset halo are reused from that XCD's L2;
+//   * workgroups run in row order, so that all eight XCDs work on
+//     neighbouring rows at once (measured faster on MI355X than giving each
+//     XCD a contiguous eighth of the rows);
 //   * when the block width is a multiple of 64 the block, its offset list and
 //     its start are wave-uniform (scalar loads); values of one (block, offset)
 //     pair are one contiguous 512-byte FP64 line per wave, loaded with the
 //     nontemporal hint so that the value stream does not evict x;
-//   * offsets are consumed in chunks of 8 so that 8 value loads and 8 x loads
-//     are in flight per wave before the first FMA;
+//   * offsets are consumed in branch-free chunks of 9 so that 9 value loads
+//     and 9 x loads are in flight per wave before the first FMA; value and x
+//     addresses are 32-bit byte offsets from wave-uniform bases;
 //   * the COO fallback is fused: the entries of the 64-row group are loaded
 //     cooperatively (one per lane), x is gathered in parallel, and each lane
 //     then sums its own row's entries sequentially in entry order out of LDS.
@@ -48,18 +49,7 @@ inline hipStream_t hs(hbdia_stream_t s) { return reinterpret_cast<hipStream_t>(s
 constexpr int kWave = 64;
 constexpr int kWavesPerWG = 4;
 constexpr int kWG = kWave * kWavesPerWG;
-constexpr int kChunk = 8;
-constexpr unsigned kNumXcd = 8;
-
-// Workgroup i is dispatched to XCD i % 8.  Give XCD x a contiguous run of
-// logical workgroups: XCDs 0..r-1 get q+1, the rest q (q = n / 8, r = n % 8).
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
-  const uint32_t q = n / kNumXcd, r = n % kNumXcd;
-  const uint32_t xcd = bid % kNumXcd, idx = bid / kNumXcd;
-  const uint32_t start = xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q;
-  return start + idx;
-}
-
+constexpr int kChunk = 9;
 template <typename T> __device__ __forceinline__ T fma_t(T a, T b, T c);
 template <> __device__ __forceinline__ double fma_t<double>(double a, double b, double c) {
   return __builtin_fma(a, b, c);
@@ -114,45 +104,97 @@ __device__ __forceinline__ T coo_group_sum(const int* __restrict__ cr, const int
   return cs;
 }
 
-// Blocked-DIA sum for one row over `cnt` offsets, chunked for memory-level
-// parallelism.  `vp` points at the row's lane in the block's first offset
-// line; consecutive offsets are `bw` elements apart.
-template <typename T, bool UNIFORM>
-__device__ __forceinline__ T bdia_row_sum(const T* __restrict__ vp, const int* __restrict__ op,
-                                          int cnt, int64_t bw, const T* __restrict__ x,
-                                          int64_t row, int64_t nrows, bool active) {
+// Element `i` of a wave-uniform base as base + 32-bit byte offset, so that the
+// load uses the scalar-base addressing form (one VGPR per address).
+template <typename T>
+__device__ __forceinline__ const T* at_u32(const T* base, uint32_t i) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<uint32_t>(sizeof(T)));
+}
+
+// Loads x[c] for c already clamped into [0, nrows).  X32: nrows * sizeof(T) < 2^32.
+template <typename T, bool X32>
+__device__ __forceinline__ T load_x(const T* __restrict__ x, int64_t c) {
+  if constexpr (X32) return *at_u32(x, static_cast<uint32_t>(c));
+  else return x[c];
+}
+
+// Blocked-DIA sum of one row whose 64-row wave lies in one block (block width
+// a multiple of 64).  `vb` is the wave-uniform address of the block's first
+// offset line at this wave's first row; offset o of lane l is vb[o*bw + l].
+// Offsets are consumed in chunks of kChunk with no control flow inside a
+// chunk: the tail of the last chunk re-reads the last valid line (a cache hit)
+// and x addresses are clamped into [0, nrows), so every load of a chunk can
+// be in flight at once.  The FMA chain runs over ascending offsets and skips
+// zero values and columns outside [0, nrows), exactly as the reference kernel.
+template <typename T, bool X32>
+__device__ __forceinline__ T bdia_sum_uniform(const T* __restrict__ vb, const int* __restrict__ op, int cnt,
+                                              int bw, int lane, const T* __restrict__ x, int64_t row,
+                                              int64_t nrows, bool active) {
   T sum = T(0);
+  const int64_t last = nrows - 1;
   for (int o0 = 0; o0 < cnt; o0 += kChunk) {
-    T v[kChunk], xv[kChunk];
     int off[kChunk];
+    T v[kChunk], xl[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) off[k] = op[min(o0 + k, cnt - 1)];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) {
-      v[k] = T(0);
-      xv[k] = T(0);
-      off[k] = 0;
-      if (o0 + k < cnt) {
-        off[k] = op[o0 + k];
-        v[k] = __builtin_nontemporal_load(vp + static_cast<int64_t>(o0 + k) * bw);
-        const int64_t c = row + off[k];
-        if (active && c >= 0 && c < nrows) xv[k] = x[c];
-      }
+      int64_t c = row + off[k];
+      c = c < 0 ? 0 : (c > last ? last : c);
+      xl[k] = load_x<T, X32>(x, c);
     }
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) {
       const int64_t c = row + off[k];
       const bool ok = (o0 + k < cnt) && active && v[k] != T(0) && c >= 0 && c < nrows;
-      sum = ok ? fma_t<T>(v[k], xv[k], sum) : sum;
+      sum = ok ? fma_t<T>(v[k], xl[k], sum) : sum;
     }
   }
-  (void)UNIFORM;
+  return sum;
+}
+
+// The same for a row whose block is per lane (block width not a multiple of
+// 64): `vp` is the lane's address in its block's first offset line, `op` the
+// lane's offset list; lanes leave the loop at their own count.
+template <typename T, bool X32>
+__device__ __forceinline__ T bdia_sum_lane(const T* __restrict__ vp, const int* __restrict__ op, int cnt,
+                                           int64_t bw, const T* __restrict__ x, int64_t row, int64_t nrows) {
+  T sum = T(0);
+  const int64_t last = nrows - 1;
+  for (int o0 = 0; o0 < cnt; o0 += kChunk) {
+    int off[kChunk];
+    T v[kChunk], xl[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) off[k] = op[min(o0 + k, cnt - 1)];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      v[k] = __builtin_nontemporal_load(vp + static_cast<int64_t>(min(o0 + k, cnt - 1)) * bw);
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      int64_t c = row + off[k];
+      c = c < 0 ? 0 : (c > last ? last : c);
+      xl[k] = load_x<T, X32>(x, c);
+    }
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const int64_t c = row + off[k];
+      const bool ok = (o0 + k < cnt) && v[k] != T(0) && c >= 0 && c < nrows;
+      sum = ok ? fma_t<T>(v[k], xl[k], sum) : sum;
+    }
+  }
   return sum;
 }
 
 // Fused HBDIA SpMV.  UNIFORM: block_width % 64 == 0, so a wave's 64 rows lie
 // in one block.  HAS_COO: the matrix has a non-empty COO fallback, and then
 // every row receives y = (bdia or old y) + coo, as the reference's
-// vector_add_kernel does for all rows.
-template <typename T, bool UNIFORM, bool HAS_COO>
+// vector_add_kernel does for all rows.  X32: x fits 32-bit byte offsets.
+// Workgroups are taken in row order: measured on MI355X, row order beats an
+// XCD-contiguous remap (neighbouring rows run on all eight XCDs at once).
+template <typename T, bool UNIFORM, bool HAS_COO, bool X32>
 __global__ __launch_bounds__(kWG) void k_hbdia(
     const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const T* __restrict__ x, T* __restrict__ y, int64_t nrows, int nb, int bw,
@@ -164,7 +206,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia(
   __shared__ int s_hi[kWavesPerWG][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t grp = static_cast<int64_t>(xcd_remap(blockIdx.x, gridDim.x)) * kWavesPerWG + wv;
+  const int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
   const int64_t row0 = grp * kWave;
   if (row0 >= nrows) return;
   const int64_t row = row0 + lane;
@@ -179,9 +221,21 @@ __global__ __launch_bounds__(kWG) void k_hbdia(
     cnt = blk[b + 1] - start;
   }
   T sum = T(0);
-  if (cnt > 0) {
-    const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
-    sum = bdia_row_sum<T, UNIFORM>(vp, offs + start, cnt, bw, x, row, nrows, active);
+  if constexpr (UNIFORM) {
+    // a block whose values exceed 32-bit byte offsets takes the per-lane path
+    const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(T)) < (1ll << 32);
+    if (cnt > 0 && small) {
+      const T* vb = vals + static_cast<int64_t>(start) * bw + (row0 - b * bw);
+      sum = bdia_sum_uniform<T, X32>(vb, offs + start, cnt, bw, lane, x, row, nrows, active);
+    } else if (cnt > 0 && active) {
+      const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
+      sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
+    }
+  } else {
+    if (cnt > 0) {
+      const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
+      sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
+    }
   }
   if constexpr (HAS_COO) {
     const int p0 = gptr[grp], p1 = gptr[grp + 1];
@@ -264,15 +318,21 @@ int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   if (wgs > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows");
   const dim3 grid(static_cast<unsigned>(wgs)), block(kWG);
   const bool uniform = (bw % kWave) == 0;
+  // 32-bit byte offsets for x when x fits, and for a block's values always
+  // (checked by the caller through the largest block).
+  const bool x32 = static_cast<uint64_t>(nrows) * sizeof(T) < (1ull << 32);
   hipStream_t s = hs(stream);
-  if (uniform && with_coo)
-    hipLaunchKernelGGL((k_hbdia<T, true, true>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
-  else if (uniform)
-    hipLaunchKernelGGL((k_hbdia<T, true, false>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
-  else if (with_coo)
-    hipLaunchKernelGGL((k_hbdia<T, false, true>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
-  else
-    hipLaunchKernelGGL((k_hbdia<T, false, false>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+#define HBDIA_LAUNCH(U, C, X)                                                                          \
+  hipLaunchKernelGGL((k_hbdia<T, U, C, X>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, \
+                     cv, gptr)
+  if (uniform) {
+    if (with_coo) { if (x32) HBDIA_LAUNCH(true, true, true); else HBDIA_LAUNCH(true, true, false); }
+    else { if (x32) HBDIA_LAUNCH(true, false, true); else HBDIA_LAUNCH(true, false, false); }
+  } else {
+    if (with_coo) { if (x32) HBDIA_LAUNCH(false, true, true); else HBDIA_LAUNCH(false, true, false); }
+    else { if (x32) HBDIA_LAUNCH(false, false, true); else HBDIA_LAUNCH(false, false, false); }
+  }
+#undef HBDIA_LAUNCH
   return check_hip(hipGetLastError(), name);
 }
 
