@@ -45,7 +45,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", choices=["c2", "ref128"], default="c2")
+    p.add_argument("--workload", choices=["c2", "ref128", "precision", "multirhs"], default="c2")
     p.add_argument("--grid", type=int, default=None, help="override the grid edge (testing only)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full-size parity check")
     p.add_argument("--cpu-reps", type=int, default=5)
@@ -57,6 +57,10 @@ def main() -> int:
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.workload == "precision":
+        return bench_precision(args)
+    if args.workload == "multirhs":
+        return bench_multirhs(args)
 
     import hbdia_amd
     from hbdia_amd import HBDIA, Event, HBDIAVector, gen_vector, hbdiaSpMV
@@ -203,6 +207,137 @@ def main() -> int:
         "parity": parity,
         "setup_s": round(time.time() - t0, 1),
     }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _time_launches(fn, sync, stream, steps: int, warmup: int):
+    """Average wall time and stream-event time (ms) of `fn` over `steps` calls."""
+    import hbdia_amd
+    from hbdia_amd import Event
+    from hbdia_amd._native import check_hip, hip
+    for _ in range(warmup):
+        fn()
+    sync()
+    ev0, ev1 = Event(), Event()
+    ts = time.perf_counter()
+    check_hip(hip.hbdia_event_record(ev0.handle, stream))
+    for _ in range(steps):
+        fn()
+    check_hip(hip.hbdia_event_record(ev1.handle, stream))
+    sync()
+    hbdia_amd.synchronize()
+    te = time.perf_counter()
+    return (te - ts) * 1e3 / steps, ev0.elapsed_ms(ev1) / steps
+
+
+def bench_precision(args) -> int:
+    """FP64 / FP32 / BF16 value storage on the 192^3 stencil + scatter matrix (x, y FP64).
+
+    Reports, for each storage form, the time per SpMV, algorithmic GB/s (s_v = 8, 4, 2),
+    the rows recomputed in FP64 and the max / RMS relative error per row against the FP64
+    result, which is the oracle's bit for bit (checked)."""
+    import oracle
+    from hbdia_amd import HBDIA, HBDIAMixed, HBDIAVector, gen_vector, hbdiaSpMV
+    n = args.grid or 192
+    seed = 0xC3
+    t0 = time.time()
+    A = HBDIA.stencil(n, 27, 0.005, seed=seed)
+    A.convertToHBDIAFormat(64, 2)
+    info = A.info()
+    N = n ** 3
+    x = gen_vector(N, seed + 1)
+    log(f"{n}^3 matrix ready in {time.time() - t0:.1f}s")
+    X, Y = HBDIAVector(x), HBDIAVector(np.zeros(N))
+    stream = A.device_ptrs()["bdia_stream"]
+    nnz, ncoo = info["nnz"], info["num_coo"]
+    flop = 2.0 * nnz
+
+    def bytes_for(sv):
+        return (nnz - ncoo) * sv + ncoo * (sv + 8) + 2 * N * 8
+
+    hbdiaSpMV(A, X, Y, False, True)
+    A.synchronize()
+    y64 = Y.download()
+    hb = A.getHBDIA()
+    hb["num_blocks"], hb["block_width"] = A.getNumBlocks(), 64
+    ycpu = oracle.hbdia_spmv(hb, x, np.zeros(N), N)
+    if not np.array_equal(ycpu, y64):
+        print(json.dumps({"error": "FP64 parity failure"}))
+        return 4
+    del hb
+    forms = {}
+    wall, kern = _time_launches(lambda: hbdiaSpMV(A, X, Y, False, True), A.synchronize, stream, args.steps, args.warmup)
+    forms["fp64"] = {"ms": round(kern, 5), "gbs": round(bytes_for(8) / (kern * 1e-3) / 1e9, 1),
+                     "gflops": round(flop / (kern * 1e-3) / 1e9, 1), "max_rel_err": 0.0, "rms_rel_err": 0.0,
+                     "fp64_rows": N}
+    denom = np.maximum(np.abs(y64), 1e-300)
+    for prec, sv, ratio in (("fp32", 4, 1e4), ("bf16", 2, 1e4), ("bf16", 2, 64.0), ("bf16", 2, 8.0)):
+        M = HBDIAMixed(A, prec, ratio)
+        M.spmv(X, Y)
+        A.synchronize()
+        y = Y.download()
+        rel = np.abs(y - y64) / denom
+        nfix = M.fp64_rows()
+        wall, kern = _time_launches(lambda: M.spmv(X, Y), A.synchronize, stream, args.steps, args.warmup)
+        key = f"{prec}_ratio{ratio:g}"
+        forms[key] = {"ms": round(kern, 5), "gbs": round(bytes_for(sv) / (kern * 1e-3) / 1e9, 1),
+                      "gflops": round(flop / (kern * 1e-3) / 1e9, 1), "cancellation_ratio": ratio,
+                      "max_rel_err": float(rel.max()), "rms_rel_err": float(np.sqrt(np.mean(rel ** 2))),
+                      "p99_rel_err": float(np.quantile(rel, 0.99)), "median_rel_err": float(np.median(rel)),
+                      "fp64_rows": nfix, "speedup_vs_fp64": round(forms["fp64"]["ms"] / kern, 3)}
+        log(f"{key}: {forms[key]}")
+        M.close()
+    best = min(forms, key=lambda k: forms[k]["ms"])
+    line = {"metric": "SpMV GFLOP/s by value storage precision (x, y FP64) + max/RMS rel. error vs FP64 oracle",
+            "value": forms[best]["gflops"], "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": forms[best]["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": best, "data": "synthetic (seeded generator)",
+            "config": {"workload": f"stencil27_scatter_{n}^3_precision", "grid": n, "rows": N, "scatter": 0.005,
+                       "block_width": 64, "threshold": 2, "cancellation_ratio": 1e4, "accumulate": "fp32"},
+            "forms": forms}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_multirhs(args) -> int:
+    """k right-hand sides in one launch vs k single-vector launches, 128^3 27-point stencil."""
+    from hbdia_amd import HBDIA, DeviceArray, HBDIAVector, gen_vector, hbdiaSpMV, hbdiaSpMVMulti
+    n = args.grid or 128
+    A = HBDIA.stencil(n, 27, 0.0, seed=0xC4)
+    A.convertToHBDIAFormat(64, 2)
+    info = A.info()
+    N = n ** 3
+    nnz = info["nnz"]
+    stream = A.device_ptrs()["bdia_stream"]
+    X1, Y1 = HBDIAVector(gen_vector(N, 1)), HBDIAVector(np.zeros(N))
+    _, t1 = _time_launches(lambda: hbdiaSpMV(A, X1, Y1, False, True), A.synchronize, stream, args.steps, args.warmup)
+    rows = {}
+    for k in (1, 4, 8, 16, 32):
+        Xh = np.concatenate([gen_vector(N, 10 + j) for j in range(k)])
+        X, Y = DeviceArray.from_host(Xh), DeviceArray(k * N, np.float64, zero=True)
+        hbdiaSpMVMulti(A, X, Y, k)
+        A.synchronize()
+        # column k-1 must equal the single-vector result
+        Xs = HBDIAVector(Xh[(k - 1) * N:])
+        Ys = HBDIAVector(np.zeros(N))
+        hbdiaSpMV(A, Xs, Ys, False, True)
+        A.synchronize()
+        if not np.array_equal(Y.download()[(k - 1) * N:], Ys.download()):
+            print(json.dumps({"error": f"multi-RHS column mismatch at k={k}"}))
+            return 4
+        _, tk = _time_launches(lambda: hbdiaSpMVMulti(A, X, Y, k), A.synchronize, stream, args.steps, args.warmup)
+        b = nnz * 8 + k * N * 16
+        rows[k] = {"ms": round(tk, 5), "gflops": round(2.0 * nnz * k / (tk * 1e-3) / 1e9, 1),
+                   "gbs": round(b / (tk * 1e-3) / 1e9, 1), "speedup_vs_k_single": round(k * t1 / tk, 3)}
+        log(f"k={k}: {rows[k]}")
+    line = {"metric": "multi-RHS SpMV GFLOP/s (FP64), k right-hand sides in one launch", "value": rows[32]["gflops"],
+            "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": rows[32]["ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded generator)",
+            "config": {"workload": f"stencil27_{n}^3_multirhs", "grid": n, "rows": N, "block_width": 64,
+                       "single_vector_ms": round(t1, 5)},
+            "by_k": rows}
     print(json.dumps(line), flush=True)
     return 0
 

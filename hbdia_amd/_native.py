@@ -70,6 +70,14 @@ _HIP_SIGS = {
     "hbdia_bdia_spmv_f32": (_i, [_p, _p, _p, _p, _p, c_int64, c_int32, c_int32, _p]),
     "hbdia_hbdia_spmv_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int64, _p]),
     "hbdia_hbdia_spmv_f32": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int64, _p]),
+    "hbdia_hbdia_spmm_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, c_int64, _p, c_int64, c_int32,
+                                  c_int64, _p]),
+    "hbdia_hbdia_spmm_f32": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, c_int64, _p, c_int64, c_int32,
+                                  c_int64, _p]),
+    "hbdia_convert_values": (_i, [_p, _p, c_int64, c_int, _p]),
+    "hbdia_mixed_workspace_bytes": (c_size_t, [c_int64]),
+    "hbdia_hbdia_spmv_mixed": (_i, [c_int, _p, _p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, _p, c_int64,
+                                    c_double, _p, _p]),
     "hbdia_coo_spmv_f64": (_i, [_p, _p, _p, _p, _p, _p, c_int64, _p]),
     "hbdia_coo_spmv_f32": (_i, [_p, _p, _p, _p, _p, _p, c_int64, _p]),
     "hbdia_coo_group_ptr": (_i, [_p, c_int64, c_int64, _p, _p]),
@@ -101,6 +109,12 @@ _CAPI_SIGS = {
     "hbdia_vector_download_coo": (_i, [_p, _p, c_int64]),
     "hbdia_vector_device_ptrs": (_i, [_p, POINTER(c_void_p), POINTER(c_void_p)]),
     "hbdia_spmv": (_i, [_p, _p, _p, c_int, c_int]),
+    "hbdia_spmv_multi": (_i, [_p, _p, c_int64, _p, c_int64, c_int, c_int]),
+    "hbdia_mixed_create": (_i, [POINTER(c_void_p), _p, c_int, c_double]),
+    "hbdia_mixed_destroy": (_i, [_p]),
+    "hbdia_mixed_spmv": (_i, [_p, _p, _p]),
+    "hbdia_mixed_spmv_raw": (_i, [_p, _p, _p]),
+    "hbdia_mixed_fp64_rows": (_i, [_p, POINTER(c_int64)]),
     "hbdia_gen_stencil": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_banded": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_vector": (_i, [c_int, c_int64, c_uint64, c_double, c_double, _p]),

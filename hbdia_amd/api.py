@@ -277,6 +277,67 @@ def hbdiaSpMV(matrix: HBDIA, inputVector: HBDIAVector, outputVector: HBDIAVector
     return True
 
 
+def hbdiaSpMVMulti(matrix: HBDIA, X, Y, nrhs: int, ldx: int | None = None, ldy: int | None = None,
+                   execCOOGPU: bool = True) -> bool:
+    """Y[:, j] = A X[:, j] for j < nrhs in one launch (X, Y: DeviceArray, column-major).
+
+    Asynchronous like hbdiaSpMV; every column equals hbdiaSpMV on that column bit for bit."""
+    n = matrix.getNumRows()
+    ldx = n if ldx is None else int(ldx)
+    ldy = n if ldy is None else int(ldy)
+    if X.size < ldx * (nrhs - 1) + n or Y.size < ldy * (nrhs - 1) + n:
+        raise ValueError("X or Y too small for nrhs columns")
+    try:
+        check_capi(capi.hbdia_spmv_multi(matrix._h, X.ptr, ldx, Y.ptr, ldy, int(nrhs), int(bool(execCOOGPU))))
+    except HBDIAError as e:
+        if "Invalid COO execution flags" in str(e):
+            raise RuntimeError("Invalid COO execution flags") from None
+        raise
+    return True
+
+
+class HBDIAMixed:
+    """FP32 / BF16 value storage for an FP64 matrix (include/Operations/HBDIAMixed.hpp).
+
+    spmv(x, y) computes y = A x in FP64 vectors with FP32 accumulation; rows whose
+    cancellation ratio exceeds `cancellation_ratio` are recomputed in FP64."""
+
+    PRECISIONS = {"fp32": 1, "bf16": 2}
+
+    def __init__(self, matrix: HBDIA, precision: str = "bf16", cancellation_ratio: float = 1e4):
+        if matrix.dtype != np.float64:
+            raise TypeError("reduced-precision storage needs an FP64 matrix")
+        self.matrix = matrix  # keeps the FP64 matrix alive
+        self.precision = precision
+        h = c_void_p(None)
+        check_capi(capi.hbdia_mixed_create(byref(h), matrix._h, self.PRECISIONS[precision], float(cancellation_ratio)))
+        self._h = h.value
+
+    def spmv(self, x: HBDIAVector, y: HBDIAVector) -> None:
+        check_capi(capi.hbdia_mixed_spmv(self._h, x._h, y._h))
+
+    def spmv_raw(self, x_ptr, y_ptr) -> None:
+        check_capi(capi.hbdia_mixed_spmv_raw(self._h, x_ptr, y_ptr))
+
+    def fp64_rows(self) -> int:
+        n = ctypes.c_int64(0)
+        check_capi(capi.hbdia_mixed_fp64_rows(self._h, byref(n)))
+        return int(n.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            check_capi(capi.hbdia_mixed_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                capi.hbdia_mixed_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
 def gen_vector(n: int, seed: int, lo: float = -1.0, hi: float = 1.0, dtype=np.float64) -> np.ndarray:
     out = np.empty(int(n), np.dtype(dtype))
     check_capi(capi.hbdia_gen_vector(_code(dtype), int(n), int(seed), float(lo), float(hi), out.ctypes.data))

@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
 
 #include "hbdia_hip.h"
@@ -253,6 +254,379 @@ __global__ __launch_bounds__(kWG) void k_hbdia(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Multiple right-hand sides: Y[:, j] = A X[:, j] for j < nrhs, with X and Y
+// column-major (column j at X + j*ldx, Y + j*ldy).  Values are loaded once per
+// (block, offset) and applied to KB columns; each column's arithmetic is the
+// single-vector kernel's (same FMA order, same skips), so every column equals
+// a single-vector SpMV bit for bit.  Block width must be a multiple of 64.
+
+// Fallback sums of the calling lane's row for KB columns (entry order, FMA).
+template <typename T, int KB>
+__device__ __forceinline__ void coo_group_sum_multi(const int* __restrict__ cr, const int* __restrict__ cc,
+                                                    const T* __restrict__ cv, const T* __restrict__ X,
+                                                    int64_t ldx, int kb, int p0, int p1, int64_t row0, int lane,
+                                                    T* s_v, T* s_x, int* s_lo, int* s_hi, T* cs) {
+#pragma unroll
+  for (int j = 0; j < KB; ++j) cs[j] = T(0);
+  for (int base = p0; base < p1; base += kWave) {
+    const int i = base + lane;
+    const bool ok = i < p1;
+    int r = -1, col = 0;
+    T vv = T(0);
+    if (ok) {
+      r = cr[i] - static_cast<int>(row0);
+      col = cc[i];
+      vv = cv[i];
+      if (r < 0 || r >= kWave) r = -1;
+    }
+    const int rp = __shfl_up(r, 1);
+    const int rn = __shfl_down(r, 1);
+    s_lo[lane] = 0;
+    s_hi[lane] = 0;
+    s_v[lane] = vv;
+    wave_sync();
+    if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
+    if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p1 || rn != r)) s_hi[r] = lane + 1;
+    wave_sync();
+    const int a = s_lo[lane], e = s_hi[lane];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j >= kb) break;
+      s_x[lane] = ok ? X[static_cast<int64_t>(j) * ldx + col] : T(0);
+      wave_sync();
+      T acc = cs[j];
+      for (int k = a; k < e; ++k) acc = fma_t<T>(s_v[k], s_x[k], acc);
+      cs[j] = acc;
+      wave_sync();
+    }
+  }
+}
+
+template <int KB> struct MrhsChunk { static constexpr int value = KB >= 16 ? 1 : (KB >= 8 ? 2 : (KB >= 4 ? 3 : 5)); };
+
+template <typename T, int KB, bool HAS_COO, bool X32>
+__global__ __launch_bounds__(kWG) void k_hbdia_multi(
+    const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy, int nrhs, int64_t nrows, int nb,
+    int bw, const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
+    const int* __restrict__ gptr) {
+  constexpr int CH = MrhsChunk<KB>::value;
+  __shared__ T s_v[kWavesPerWG][kWave];
+  __shared__ T s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
+  const int64_t row0 = grp * kWave;
+  if (row0 >= nrows) return;
+  const int j0 = static_cast<int>(blockIdx.y) * KB;
+  const int kb = min(KB, nrhs - j0);
+  const T* Xj = X + static_cast<int64_t>(j0) * ldx;
+  T* Yj = Y + static_cast<int64_t>(j0) * ldy;
+  const int64_t row = row0 + lane;
+  const bool active = row < nrows;
+  const int64_t last = nrows - 1;
+  const int64_t b = row0 / bw;
+  int start = 0, cnt = 0;
+  if (b < nb) {
+    start = blk[b];
+    cnt = blk[b + 1] - start;
+  }
+  T sum[KB];
+#pragma unroll
+  for (int j = 0; j < KB; ++j) sum[j] = T(0);
+  if (cnt > 0) {
+    const T* vb = vals + static_cast<int64_t>(start) * bw + (row0 - b * bw);
+    const int* op = offs + start;
+    const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(T)) < (1ll << 32);
+    for (int o0 = 0; o0 < cnt; o0 += CH) {
+      int off[CH];
+      T v[CH];
+      int64_t cl[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) off[k] = op[min(o0 + k, cnt - 1)];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int kk = min(o0 + k, cnt - 1);
+        v[k] = small ? __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(kk * bw + lane)))
+                     : __builtin_nontemporal_load(vb + static_cast<int64_t>(kk) * bw + lane);
+        int64_t c = row + off[k];
+        cl[k] = c < 0 ? 0 : (c > last ? last : c);
+      }
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        if (j >= kb) break;
+        const T* xj = Xj + static_cast<int64_t>(j) * ldx;
+        T xl[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) xl[k] = load_x<T, X32>(xj, cl[k]);
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+          const int64_t c = row + off[k];
+          const bool ok = (o0 + k < cnt) && active && v[k] != T(0) && c >= 0 && c < nrows;
+          sum[j] = ok ? fma_t<T>(v[k], xl[k], sum[j]) : sum[j];
+        }
+      }
+    }
+  }
+  if constexpr (HAS_COO) {
+    const int p0 = gptr[grp], p1 = gptr[grp + 1];
+    T cs[KB];
+    if (p0 < p1) {
+      coo_group_sum_multi<T, KB>(cr, cc, cv, Xj, ldx, kb, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv],
+                                 s_hi[wv], cs);
+    } else {
+#pragma unroll
+      for (int j = 0; j < KB; ++j) cs[j] = T(0);
+    }
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        if (j >= kb) break;
+        T* yp = Yj + static_cast<int64_t>(j) * ldy + row;
+        const T base = cnt > 0 ? sum[j] : *yp;
+        __builtin_nontemporal_store(base + cs[j], yp);
+      }
+    }
+  } else {
+    if (active && cnt > 0) {
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        if (j >= kb) break;
+        __builtin_nontemporal_store(sum[j], Yj + static_cast<int64_t>(j) * ldy + row);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reduced-precision storage.  Values (blocked part and fallback) are held as
+// FP32 or BF16 copies of the FP64 matrix; x and y stay FP64.  Each row is
+// accumulated in FP32 (products of the stored value and x rounded to FP32),
+// together with the FP32 sum of |a x|.  A row whose cancellation ratio
+// sum|a x| / |sum a x| exceeds `ratio` (or whose sum is zero or not finite) is
+// appended to a device list and recomputed by k_fix_rows from the FP64 arrays
+// with the FP64 kernel's exact arithmetic.
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+// FP32 -> BF16, round to nearest even (NaN stays NaN).
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return static_cast<uint16_t>((u >> 16) | 0x40u);
+  const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(r >> 16);
+}
+
+template <typename TV> __device__ __forceinline__ float lp_to_f32(TV v);
+template <> __device__ __forceinline__ float lp_to_f32<float>(float v) { return v; }
+template <> __device__ __forceinline__ float lp_to_f32<uint16_t>(uint16_t v) { return bf16_to_f32(v); }
+
+__global__ void k_f64_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = static_cast<float>(src[i]);
+}
+
+__global__ void k_f64_to_bf16(const double* __restrict__ src, uint16_t* __restrict__ dst, int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = f32_to_bf16(static_cast<float>(src[i]));
+}
+
+__global__ void k_zero_i32(int* __restrict__ p) { *p = 0; }
+
+template <typename TV, bool HAS_COO>
+__global__ __launch_bounds__(kWG) void k_hbdia_mixed(
+    const TV* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb, int bw,
+    const int* __restrict__ cr, const int* __restrict__ cc, const TV* __restrict__ cv,
+    const int* __restrict__ gptr, float ratio, float u_store, uint64_t* __restrict__ flag_mask) {
+  __shared__ float s_v[kWavesPerWG][kWave];
+  __shared__ float s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
+  const int64_t row0 = grp * kWave;
+  if (row0 >= nrows) return;
+  const int64_t row = row0 + lane;
+  const bool active = row < nrows;
+  const int64_t last = nrows - 1;
+  const int64_t b = row0 / bw;
+  int start = 0, cnt = 0;
+  if (b < nb) {
+    start = blk[b];
+    cnt = blk[b + 1] - start;
+  }
+  float sum = 0.0f, sabs = 0.0f;
+  if (cnt > 0) {
+    const TV* vb = vals + static_cast<int64_t>(start) * bw + (row0 - b * bw);
+    const int* op = offs + start;
+    for (int o0 = 0; o0 < cnt; o0 += kChunk) {
+      int off[kChunk];
+      float v[kChunk];
+      double xl[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) off[k] = op[min(o0 + k, cnt - 1)];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k)
+        v[k] = lp_to_f32<TV>(__builtin_nontemporal_load(vb + static_cast<int64_t>(min(o0 + k, cnt - 1)) * bw + lane));
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        int64_t c = row + off[k];
+        c = c < 0 ? 0 : (c > last ? last : c);
+        xl[k] = x[c];
+      }
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        const int64_t c = row + off[k];
+        const bool ok = (o0 + k < cnt) && active && v[k] != 0.0f && c >= 0 && c < nrows;
+        const float xf = static_cast<float>(xl[k]);
+        sum = ok ? __builtin_fmaf(v[k], xf, sum) : sum;
+        sabs = ok ? __builtin_fmaf(__builtin_fabsf(v[k]), __builtin_fabsf(xf), sabs) : sabs;
+      }
+    }
+  }
+  float cs = 0.0f, cabs = 0.0f;
+  int nterms = cnt;
+  if constexpr (HAS_COO) {
+    const int p0 = gptr[grp], p1 = gptr[grp + 1];
+    for (int base = p0; base < p1; base += kWave) {
+      const int i = base + lane;
+      const bool ok = i < p1;
+      int r = -1;
+      float vv = 0.0f, xx = 0.0f;
+      if (ok) {
+        r = cr[i] - static_cast<int>(row0);
+        vv = lp_to_f32<TV>(cv[i]);
+        xx = static_cast<float>(x[cc[i]]);
+        if (r < 0 || r >= kWave) r = -1;
+      }
+      const int rp = __shfl_up(r, 1);
+      const int rn = __shfl_down(r, 1);
+      s_lo[wv][lane] = 0;
+      s_hi[wv][lane] = 0;
+      s_v[wv][lane] = vv;
+      s_x[wv][lane] = xx;
+      wave_sync();
+      if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[wv][r] = lane;
+      if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p1 || rn != r)) s_hi[wv][r] = lane + 1;
+      wave_sync();
+      const int a = s_lo[wv][lane], e = s_hi[wv][lane];
+      nterms += e - a;
+      for (int k = a; k < e; ++k) {
+        cs = __builtin_fmaf(s_v[wv][k], s_x[wv][k], cs);
+        cabs = __builtin_fmaf(__builtin_fabsf(s_v[wv][k]), __builtin_fabsf(s_x[wv][k]), cabs);
+      }
+      wave_sync();
+    }
+  }
+  // Cancellation test against the smallest |y| consistent with the FP32
+  // result: err bounds |tot - y| by (u_store + u_x + gamma_n) sum|a x| (FP32
+  // value or BF16 storage, x rounded to FP32, one rounding per FMA, plus the
+  // final addition), so a row is kept only if (|tot| - err) * ratio >= sum|a x|.
+  const float tot = sum + cs;
+  const float mag = sabs + cabs;
+  const float u32 = 5.9604645e-08f;  // 2^-24
+  const float err = (u_store + u32 * static_cast<float>(nterms + 2)) * mag * 1.0001f;
+  const bool fix = active && (!((__builtin_fabsf(tot) - err) * ratio >= mag) || !__builtin_isfinite(tot)) &&
+                   (cnt > 0 || cs != 0.0f || cabs != 0.0f);
+  // every group stores its mask (zero included): no reset between calls
+  const uint64_t m = __ballot(fix);
+  if (lane == 0) flag_mask[grp] = m;
+  if (!active || fix) return;
+  if constexpr (HAS_COO) {
+    const double base = cnt > 0 ? static_cast<double>(sum) : y[row];
+    __builtin_nontemporal_store(base + static_cast<double>(cs), y + row);
+  } else {
+    if (cnt > 0) __builtin_nontemporal_store(static_cast<double>(sum), y + row);
+  }
+}
+
+// FP64 recomputation of one row with the FP64 kernel's arithmetic: blocked
+// part in ascending offset order (FMA, zero and range skips), fallback entries
+// of the row in entry order (FMA), y = (bdia or old y) + coo.
+__device__ __forceinline__ void fix_row(int64_t row, const double* __restrict__ vals, const int* __restrict__ offs,
+                                        const int* __restrict__ blk, const double* __restrict__ x,
+                                        double* __restrict__ y, int64_t nrows, int nb, int bw,
+                                        const int* __restrict__ cr, const int* __restrict__ cc,
+                                        const double* __restrict__ cv, const int* __restrict__ gptr) {
+  const int64_t b = row / bw;
+  int start = 0, cnt = 0;
+  if (b < nb) {
+    start = blk[b];
+    cnt = blk[b + 1] - start;
+  }
+  double sum = 0.0;
+  if (cnt > 0) {
+    const double* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
+    sum = bdia_sum_lane<double, false>(vp, offs + start, cnt, bw, x, row, nrows);
+  }
+  if (gptr) {
+    const int64_t g = row / kWave;
+    int lo = gptr[g], hi = gptr[g + 1];
+    const int end = hi;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cr[mid] < row) lo = mid + 1;
+      else hi = mid;
+    }
+    double cs = 0.0;
+    for (int i = lo; i < end && cr[i] == row; ++i) cs = __builtin_fma(cv[i], x[cc[i]], cs);
+    y[row] = (cnt > 0 ? sum : y[row]) + cs;
+  } else if (cnt > 0) {
+    y[row] = sum;
+  }
+}
+
+// Recomputes every row whose bit is set in flag_mask (one 64-bit mask per
+// 64-row group).  Each wave scans the masks of 64 consecutive groups
+// (grid-stride), appends the flagged rows to a wave-private LDS queue and
+// recomputes them 64 at a time, one row per lane, so lanes stay busy whether
+// the flags are sparse or dense.
+__global__ __launch_bounds__(256) void k_fix_groups(const double* __restrict__ vals, const int* __restrict__ offs,
+                                                    const int* __restrict__ blk, const double* __restrict__ x,
+                                                    double* __restrict__ y, int64_t nrows, int nb, int bw,
+                                                    const int* __restrict__ cr, const int* __restrict__ cc,
+                                                    const double* __restrict__ cv, const int* __restrict__ gptr,
+                                                    const uint64_t* __restrict__ flag_mask) {
+  __shared__ int64_t s_q[4][2 * kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x >> 6;
+  int64_t* q = s_q[wv];
+  const uint64_t below = (1ull << lane) - 1ull;  // lanes under this one
+  const int64_t ngroups = (nrows + kWave - 1) / kWave;
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + wv;
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+  int qn = 0;  // wave-uniform queue length, < 64 between groups
+  for (int64_t gbase = wave * kWave; gbase < ngroups; gbase += nwaves * kWave) {
+    const uint64_t mine = gbase + lane < ngroups ? flag_mask[gbase + lane] : 0ull;
+    uint64_t nz = __ballot(mine != 0ull);
+    while (nz) {
+      const int j = __ffsll(static_cast<unsigned long long>(nz)) - 1;
+      nz &= nz - 1;
+      const uint64_t m = __shfl(mine, j);
+      if ((m >> lane) & 1ull) q[qn + __popcll(m & below)] = (gbase + j) * kWave + lane;
+      qn += __popcll(m);
+      if (qn >= kWave) {  // a full batch: recompute q[0..63], move the rest down
+        wave_sync();
+        fix_row(q[lane], vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+        const int64_t rest = lane + kWave < qn ? q[lane + kWave] : 0;
+        wave_sync();
+        if (lane + kWave < qn) q[lane] = rest;
+        qn -= kWave;
+        wave_sync();
+      }
+    }
+  }
+  wave_sync();
+  if (lane < qn) fix_row(q[lane], vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+}
+
 // COO fallback alone: res[r] = row sum for rows with entries (others keep res).
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_coo(const int* __restrict__ cr, const int* __restrict__ cc,
@@ -333,6 +707,94 @@ int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
     else { if (x32) HBDIA_LAUNCH(false, false, true); else HBDIA_LAUNCH(false, false, false); }
   }
 #undef HBDIA_LAUNCH
+  return check_hip(hipGetLastError(), name);
+}
+
+template <typename T, int KB>
+void launch_multi_kb(const T* vals, const int32_t* offs, const int32_t* blk, int32_t nb, int32_t bw,
+                     const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr, const T* X,
+                     int64_t ldx, T* Y, int64_t ldy, int nrhs, int64_t nrows, bool with_coo, bool x32,
+                     dim3 grid, hipStream_t s) {
+#define HBDIA_LAUNCH_M(C, XX)                                                                               \
+  hipLaunchKernelGGL((k_hbdia_multi<T, KB, C, XX>), grid, dim3(kWG), 0, s, vals, offs, blk, X, ldx, Y, ldy, nrhs, \
+                     nrows, nb, bw, cr, cc, cv, gptr)
+  if (with_coo) { if (x32) HBDIA_LAUNCH_M(true, true); else HBDIA_LAUNCH_M(true, false); }
+  else { if (x32) HBDIA_LAUNCH_M(false, true); else HBDIA_LAUNCH_M(false, false); }
+#undef HBDIA_LAUNCH_M
+}
+
+template <typename T>
+int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t nb, int32_t bw,
+                 const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr, const T* X, int64_t ldx,
+                 T* Y, int64_t ldy, int32_t nrhs, int64_t nrows, hbdia_stream_t stream, const char* name) {
+  if (nrows < 0 || nb < 0 || bw <= 0 || nrhs < 0)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative size or block_width <= 0");
+  if (nrows == 0 || nrhs == 0) return HBDIA_SUCCESS;
+  if (bw % kWave != 0)
+    return fail(HBDIA_ERROR_NOT_SUPPORTED, std::string(name) + ": block_width must be a multiple of 64");
+  if (!X || !Y || !blk || (nb > 0 && !offs))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
+  if (ldx < nrows || ldy < nrows)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": leading dimension smaller than num_rows");
+  const bool with_coo = gptr != nullptr;
+  if (with_coo && (!cr || !cc || !cv))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null COO pointer");
+  const int64_t groups = num_groups(nrows);
+  const int64_t wgs = (groups + kWavesPerWG - 1) / kWavesPerWG;
+  if (wgs > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows");
+  const bool x32 = static_cast<uint64_t>(nrows) * sizeof(T) < (1ull << 32);
+  // 8 columns per pass: measured faster than 16 (whose one-offset chunks lose
+  // memory-level parallelism); more columns take more passes (grid.y).
+  const int kb = nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1));
+  const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>((nrhs + kb - 1) / kb));
+  hipStream_t s = hs(stream);
+  switch (kb) {
+    case 8: launch_multi_kb<T, 8>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, grid, s); break;
+    case 4: launch_multi_kb<T, 4>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, grid, s); break;
+    case 2: launch_multi_kb<T, 2>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, grid, s); break;
+    default: launch_multi_kb<T, 1>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, grid, s); break;
+  }
+  return check_hip(hipGetLastError(), name);
+}
+
+int launch_mixed(int vdtype, const void* vals_lp, const double* vals64, const int32_t* offs, const int32_t* blk,
+                 int32_t nb, int32_t bw, const int32_t* cr, const int32_t* cc, const void* cv_lp, const double* cv64,
+                 const int32_t* gptr, const double* x, double* y, int64_t nrows, double ratio, void* workspace,
+                 hbdia_stream_t stream) {
+  const char* name = "hbdia_hbdia_spmv_mixed";
+  if (nrows < 0 || nb < 0 || bw <= 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": bad size");
+  if (nrows == 0) return HBDIA_SUCCESS;
+  if (bw % kWave != 0) return fail(HBDIA_ERROR_NOT_SUPPORTED, std::string(name) + ": block_width must be a multiple of 64");
+  if (vdtype != HBDIA_VALUES_F32 && vdtype != HBDIA_VALUES_BF16)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": vdtype must be HBDIA_VALUES_F32 or HBDIA_VALUES_BF16");
+  if (!vals_lp || !vals64 || !blk || !x || !y || !workspace || (nb > 0 && !offs))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
+  if (nrows > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows");
+  const bool with_coo = gptr != nullptr;
+  if (with_coo && (!cr || !cc || !cv_lp || !cv64))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null COO pointer");
+  if (!(ratio > 0.0)) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": ratio must be positive");
+  uint64_t* mask = reinterpret_cast<uint64_t*>(workspace);
+  hipStream_t s = hs(stream);
+  const int64_t groups = num_groups(nrows);
+  const dim3 grid(static_cast<unsigned>((groups + kWavesPerWG - 1) / kWavesPerWG)), block(kWG);
+  const float r = static_cast<float>(ratio);
+  const float u_store = vdtype == HBDIA_VALUES_F32 ? 5.9604645e-08f : 3.9062500e-03f + 5.9604645e-08f;
+  if (vdtype == HBDIA_VALUES_F32) {
+    const float* v = static_cast<const float*>(vals_lp);
+    const float* c = static_cast<const float*>(cv_lp);
+    if (with_coo) hipLaunchKernelGGL((k_hbdia_mixed<float, true>), grid, block, 0, s, v, offs, blk, x, y, nrows, nb, bw, cr, cc, c, gptr, r, u_store, mask);
+    else hipLaunchKernelGGL((k_hbdia_mixed<float, false>), grid, block, 0, s, v, offs, blk, x, y, nrows, nb, bw, cr, cc, c, gptr, r, u_store, mask);
+  } else {
+    const uint16_t* v = static_cast<const uint16_t*>(vals_lp);
+    const uint16_t* c = static_cast<const uint16_t*>(cv_lp);
+    if (with_coo) hipLaunchKernelGGL((k_hbdia_mixed<uint16_t, true>), grid, block, 0, s, v, offs, blk, x, y, nrows, nb, bw, cr, cc, c, gptr, r, u_store, mask);
+    else hipLaunchKernelGGL((k_hbdia_mixed<uint16_t, false>), grid, block, 0, s, v, offs, blk, x, y, nrows, nb, bw, cr, cc, c, gptr, r, u_store, mask);
+  }
+  const int64_t scan_waves = (groups + kWave - 1) / kWave;
+  const int64_t fix_blocks = std::min<int64_t>((scan_waves + 3) / 4, 2048);
+  hipLaunchKernelGGL(k_fix_groups, dim3(static_cast<unsigned>(fix_blocks)), dim3(256), 0, s, vals64, offs, blk, x, y,
+                     nrows, nb, bw, cr, cc, cv64, gptr, mask);
   return check_hip(hipGetLastError(), name);
 }
 
@@ -536,6 +998,47 @@ int hbdia_hbdia_spmv_f32(const float* values, const int32_t* offsets, const int3
   return launch_hbdia<float>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols,
                              coo_values, coo_group_ptr, x, y, num_rows, stream, true,
                              "hbdia_hbdia_spmv_f32");
+}
+int hbdia_hbdia_spmm_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                         int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                         const int32_t* coo_cols, const double* coo_values, const int32_t* coo_group_ptr,
+                         const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t nrhs, int64_t num_rows,
+                         hbdia_stream_t stream) {
+  return launch_multi<double>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols, coo_values,
+                              coo_group_ptr, X, ldx, Y, ldy, nrhs, num_rows, stream, "hbdia_hbdia_spmm_f64");
+}
+int hbdia_hbdia_spmm_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                         int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                         const int32_t* coo_cols, const float* coo_values, const int32_t* coo_group_ptr,
+                         const float* X, int64_t ldx, float* Y, int64_t ldy, int32_t nrhs, int64_t num_rows,
+                         hbdia_stream_t stream) {
+  return launch_multi<float>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols, coo_values,
+                             coo_group_ptr, X, ldx, Y, ldy, nrhs, num_rows, stream, "hbdia_hbdia_spmm_f32");
+}
+int hbdia_convert_values(const double* src, void* dst, int64_t n, int vdtype, hbdia_stream_t stream) {
+  if (n < 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_convert_values: negative size");
+  if (n == 0) return HBDIA_SUCCESS;
+  if (!src || !dst) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_convert_values: null pointer");
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (vdtype == HBDIA_VALUES_F32)
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, hs(stream), src, static_cast<float*>(dst), n);
+  else if (vdtype == HBDIA_VALUES_BF16)
+    hipLaunchKernelGGL(k_f64_to_bf16, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, hs(stream), src, static_cast<uint16_t*>(dst), n);
+  else
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_convert_values: vdtype must be HBDIA_VALUES_F32 or HBDIA_VALUES_BF16");
+  return check_hip(hipGetLastError(), "hbdia_convert_values");
+}
+size_t hbdia_mixed_workspace_bytes(int64_t num_rows) {
+  return static_cast<size_t>(num_groups(num_rows > 0 ? num_rows : 0) + 1) * sizeof(uint64_t);
+}
+int hbdia_hbdia_spmv_mixed(int vdtype, const void* values_lp, const double* values_f64, const int32_t* offsets,
+                           const int32_t* block_start, int32_t num_blocks, int32_t block_width,
+                           const int32_t* coo_rows, const int32_t* coo_cols, const void* coo_values_lp,
+                           const double* coo_values_f64, const int32_t* coo_group_ptr, const double* x, double* y,
+                           int64_t num_rows, double ratio, void* workspace, hbdia_stream_t stream) {
+  return launch_mixed(vdtype, values_lp, values_f64, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols,
+                      coo_values_lp, coo_values_f64, coo_group_ptr, x, y, num_rows, ratio, workspace, stream);
 }
 int hbdia_coo_spmv_f64(const int32_t* coo_rows, const int32_t* coo_cols, const double* coo_values,
                        const int32_t* coo_group_ptr, const double* x, double* res,

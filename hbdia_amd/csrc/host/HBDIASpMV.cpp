@@ -34,6 +34,13 @@ struct Dev<double> {
     return hbdia_vector_add_f64(d, s_, n, s);
   }
   static double fma(double a, double b, double c) { return std::fma(a, b, c); }
+  static int multi(const HBDIA<double>& m, bool coo, const double* X, int64_t ldx, double* Y, int64_t ldy, int k,
+                   hbdia_stream_t s) {
+    return hbdia_hbdia_spmm_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(), m.getBlockStartIndicesDevice(),
+                                m.getNumBlocks(), m.getBlockWidth(), coo ? m.getCpuRowIndicesDevice() : nullptr,
+                                coo ? m.getCpuColIndicesDevice() : nullptr, coo ? m.getCpuValuesDevice() : nullptr,
+                                coo ? m.getCooGroupPtrDevice() : nullptr, X, ldx, Y, ldy, k, m.getNumRows(), s);
+  }
 };
 
 template <>
@@ -57,6 +64,13 @@ struct Dev<float> {
     return hbdia_vector_add_f32(d, s_, n, s);
   }
   static float fma(float a, float b, float c) { return std::fmaf(a, b, c); }
+  static int multi(const HBDIA<float>& m, bool coo, const float* X, int64_t ldx, float* Y, int64_t ldy, int k,
+                   hbdia_stream_t s) {
+    return hbdia_hbdia_spmm_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(), m.getBlockStartIndicesDevice(),
+                                m.getNumBlocks(), m.getBlockWidth(), coo ? m.getCpuRowIndicesDevice() : nullptr,
+                                coo ? m.getCpuColIndicesDevice() : nullptr, coo ? m.getCpuValuesDevice() : nullptr,
+                                coo ? m.getCooGroupPtrDevice() : nullptr, X, ldx, Y, ldy, k, m.getNumRows(), s);
+  }
 };
 
 template <typename T>
@@ -176,6 +190,21 @@ bool hbdiaSpMV(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector, HBDIAVector<
   return true;
 }
 
+template <typename T>
+bool hbdiaSpMVMulti(HBDIA<T>& matrix, const T* X, int64_t ldx, T* Y, int64_t ldy, int nrhs, bool execCOOGPU) {
+  if (!matrix.isHBDIAFormat() || !matrix.areStreamsInitialized())
+    throw std::runtime_error("hbdiaSpMVMulti: matrix is not converted and on the device");
+  const bool hasCOOWork = !matrix.getCpuValues().empty();
+  if (hasCOOWork && !execCOOGPU) throw std::runtime_error("Invalid COO execution flags");
+  if (hasCOOWork && !matrix.hasDeviceCOO())
+    throw std::runtime_error("hbdiaSpMVMulti: the fallback is not on the device (convert with ExecutionMode::GPU_COO)");
+  CHECK_HBDIA(Dev<T>::multi(matrix, hasCOOWork, X, ldx, Y, ldy, nrhs, matrix.getBDIAStream()));
+  CHECK_HBDIA(hbdia_event_record(matrix.getBDIAEvent(), matrix.getBDIAStream()));
+  return true;
+}
+
+template bool hbdiaSpMVMulti<float>(HBDIA<float>&, const float*, int64_t, float*, int64_t, int, bool);
+template bool hbdiaSpMVMulti<double>(HBDIA<double>&, const double*, int64_t, double*, int64_t, int, bool);
 template bool hbdiaSpMV<float>(HBDIA<float>&, const HBDIAVector<float>&, HBDIAVector<float>&, bool, bool);
 template bool hbdiaSpMV<double>(HBDIA<double>&, const HBDIAVector<double>&, HBDIAVector<double>&, bool, bool);
 template void executeCOOOnCPU<float>(HBDIA<float>&, const HBDIAVector<float>&, HBDIAVector<float>&, int,

@@ -11,6 +11,7 @@
 
 #include "Format/HBDIA.hpp"
 #include "Format/HBDIAVector.hpp"
+#include "Operations/HBDIAMixed.hpp"
 #include "Operations/HBDIASpMV.hpp"
 #include "Workloads.hpp"
 
@@ -18,6 +19,10 @@ struct hbdia_matrix_opaque {
   int dtype = HBDIA_DTYPE_F64;
   std::unique_ptr<HBDIA<double>> d;
   std::unique_ptr<HBDIA<float>> f;
+};
+
+struct hbdia_mixed_opaque {
+  std::unique_ptr<HBDIAMixed> impl;
 };
 
 struct hbdia_vector_opaque {
@@ -337,6 +342,51 @@ int hbdia_spmv(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_co
     need(m->dtype == x->dtype && m->dtype == y->dtype, "dtype mismatch between matrix and vectors");
     if (m->dtype == HBDIA_DTYPE_F64) hbdiaSpMV<double>(*m->d, *x->d, *y->d, exec_coo_cpu != 0, exec_coo_gpu != 0);
     else hbdiaSpMV<float>(*m->f, *x->f, *y->f, exec_coo_cpu != 0, exec_coo_gpu != 0);
+  });
+}
+
+int hbdia_spmv_multi(hbdia_matrix_t m, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrhs, int exec_coo_gpu) {
+  return guard("hbdia_spmv_multi", [&] {
+    need(m != nullptr && X != nullptr && Y != nullptr, "null argument");
+    if (m->dtype == HBDIA_DTYPE_F64)
+      hbdiaSpMVMulti<double>(*m->d, static_cast<const double*>(X), ldx, static_cast<double*>(Y), ldy, nrhs,
+                             exec_coo_gpu != 0);
+    else
+      hbdiaSpMVMulti<float>(*m->f, static_cast<const float*>(X), ldx, static_cast<float*>(Y), ldy, nrhs,
+                            exec_coo_gpu != 0);
+  });
+}
+
+int hbdia_mixed_create(hbdia_mixed_t* h, hbdia_matrix_t m, int precision, double ratio) {
+  return guard("hbdia_mixed_create", [&] {
+    need(h != nullptr && m != nullptr, "null argument");
+    need(m->dtype == HBDIA_DTYPE_F64, "reduced-precision storage needs an FP64 matrix");
+    need(precision == 1 || precision == 2, "precision must be 1 (FP32) or 2 (BF16)");
+    auto o = std::make_unique<hbdia_mixed_opaque>();
+    o->impl = std::make_unique<HBDIAMixed>(*m->d, static_cast<StoragePrecision>(precision), ratio);
+    *h = o.release();
+  });
+}
+int hbdia_mixed_destroy(hbdia_mixed_t h) {
+  return guard("hbdia_mixed_destroy", [&] { delete h; });
+}
+int hbdia_mixed_spmv(hbdia_mixed_t h, hbdia_vector_t x, hbdia_vector_t y) {
+  return guard("hbdia_mixed_spmv", [&] {
+    need(h && x && y, "null argument");
+    need(x->dtype == HBDIA_DTYPE_F64 && y->dtype == HBDIA_DTYPE_F64, "vectors must be FP64");
+    h->impl->spmv(*x->d, *y->d);
+  });
+}
+int hbdia_mixed_spmv_raw(hbdia_mixed_t h, const void* x, void* y) {
+  return guard("hbdia_mixed_spmv_raw", [&] {
+    need(h && x && y, "null argument");
+    h->impl->spmv(static_cast<const double*>(x), static_cast<double*>(y));
+  });
+}
+int hbdia_mixed_fp64_rows(hbdia_mixed_t h, int64_t* rows) {
+  return guard("hbdia_mixed_fp64_rows", [&] {
+    need(h && rows, "null argument");
+    *rows = h->impl->lastFp64Rows();
   });
 }
 

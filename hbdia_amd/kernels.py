@@ -48,6 +48,16 @@ def hbdia_spmv(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArr
                  coo_cols.ptr, coo_values.ptr, coo_group_ptr.ptr, x.ptr, y.ptr, int(num_rows), _h(stream)))
 
 
+def hbdia_spmm(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArray, num_blocks: int,
+               block_width: int, coo, X: DeviceArray, ldx: int, Y: DeviceArray, ldy: int, nrhs: int, num_rows: int,
+               stream: Stream | None = None) -> None:
+    """Many right-hand sides; coo = (rows, cols, values, group_ptr) DeviceArrays or None."""
+    fn = getattr(hip, "hbdia_hbdia_spmm_" + _sfx(values.dtype))
+    cr, cc, cv, gp = (a.ptr for a in coo) if coo is not None else (None, None, None, None)
+    check_hip(fn(values.ptr, offsets.ptr, block_start.ptr, int(num_blocks), int(block_width), cr, cc, cv, gp, X.ptr,
+                 int(ldx), Y.ptr, int(ldy), int(nrhs), int(num_rows), _h(stream)))
+
+
 def coo_spmv(coo_rows: DeviceArray, coo_cols: DeviceArray, coo_values: DeviceArray, coo_group_ptr: DeviceArray,
              x: DeviceArray, res: DeviceArray, num_rows: int, stream: Stream | None = None) -> None:
     fn = getattr(hip, "hbdia_coo_spmv_" + _sfx(coo_values.dtype))
@@ -99,6 +109,11 @@ class DeviceLayout:
             else:
                 from .device import synchronize
                 synchronize()
+
+    def spmm(self, X: DeviceArray, Y: DeviceArray, nrhs: int, stream: Stream | None = None) -> None:
+        coo = (self.coo_rows, self.coo_cols, self.coo_values, self.group_ptr) if self.coo_nnz else None
+        hbdia_spmm(self.values, self.offsets, self.block_start, self.num_blocks, self.block_width, coo, X,
+                   self.num_rows, Y, self.num_rows, nrhs, self.num_rows, stream)
 
     def spmv(self, x: DeviceArray, y: DeviceArray, stream: Stream | None = None) -> None:
         """y = A x with the fused kernel (blocked-DIA only when there is no fallback)."""

@@ -22,6 +22,15 @@ template <typename T>
 bool hbdiaSpMV(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector, HBDIAVector<T>& outputVector,
                bool execCooCPU, bool execCooGPU);
 
+// Multiple right-hand sides (no reference counterpart): for j < nrhs,
+// Y[:, j] = A X[:, j] with device arrays X and Y in column-major order
+// (leading dimensions ldx, ldy >= numRows).  One launch on the BDIA stream;
+// with execCOOGPU the fallback is fused as in hbdiaSpMV's GPU branch.  Each
+// column equals hbdiaSpMV on that column bit for bit.  Needs a block width that
+// is a multiple of 64.  Asynchronous.
+template <typename T>
+bool hbdiaSpMVMulti(HBDIA<T>& matrix, const T* X, int64_t ldx, T* Y, int64_t ldy, int nrhs, bool execCOOGPU);
+
 // COO fallback on the host (reference HBDIASpMV.cpp:101-131).
 template <typename T>
 void executeCOOOnCPU(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector,

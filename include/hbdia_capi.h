@@ -16,6 +16,8 @@
  *   hbdia_vector_create                -> HBDIAVector<T>(localData[, four memory flags]),
  *                                         include/Format/HBDIAVector.hpp:31-32
  *   hbdia_spmv                         -> hbdiaSpMV<T>, include/Operations/HBDIASpMV.hpp:8-10
+ *   hbdia_spmv_multi                   -> no reference counterpart: many right-hand sides in one launch
+ *   hbdia_mixed_*                      -> no reference counterpart: FP32 / BF16 value storage
  *   hbdia_gen_*                        -> no reference counterpart: seeded benchmark inputs
  */
 #ifndef HBDIA_CAPI_H
@@ -96,6 +98,17 @@ int hbdia_vector_download_coo(hbdia_vector_t v, void* out, int64_t n);
 int hbdia_vector_device_ptrs(hbdia_vector_t v, void** local, void** coo);
 
 int hbdia_spmv(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_coo_cpu, int exec_coo_gpu);
+/* hbdiaSpMVMulti: X, Y device arrays (column-major, leading dimensions ldx, ldy). */
+int hbdia_spmv_multi(hbdia_matrix_t m, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrhs, int exec_coo_gpu);
+
+/* Reduced-precision storage (HBDIAMixed, include/Operations/HBDIAMixed.hpp):
+ * precision 1 = FP32, 2 = BF16; an FP64 matrix only.  x, y: FP64 vectors. */
+typedef struct hbdia_mixed_opaque* hbdia_mixed_t;
+int hbdia_mixed_create(hbdia_mixed_t* h, hbdia_matrix_t m, int precision, double cancellation_ratio);
+int hbdia_mixed_destroy(hbdia_mixed_t h);
+int hbdia_mixed_spmv(hbdia_mixed_t h, hbdia_vector_t x, hbdia_vector_t y);
+int hbdia_mixed_spmv_raw(hbdia_mixed_t h, const void* x, void* y);
+int hbdia_mixed_fp64_rows(hbdia_mixed_t h, int64_t* rows);
 
 /* Seeded benchmark inputs (generators in Workloads.hpp). */
 int hbdia_gen_stencil(hbdia_matrix_t* m, int dtype, int n, int points, double scatter, uint64_t seed);

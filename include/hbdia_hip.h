@@ -115,6 +115,50 @@ int hbdia_hbdia_spmv_f32(const float* values, const int32_t* offsets, const int3
                          const int32_t* coo_group_ptr, const float* x, float* y,
                          int64_t num_rows, hbdia_stream_t stream);
 
+/* Multiple right-hand sides (no reference counterpart: the reference's
+ * hbdiaSpMV takes one vector; this is the batched form of the same product).
+ * For j < nrhs: Y[:, j] = A X[:, j], column j at X + j*ldx and Y + j*ldy
+ * (ldx, ldy >= num_rows).  With coo_group_ptr == NULL the result follows
+ * hbdia_bdia_spmv_* (no fallback); otherwise hbdia_hbdia_spmv_*.  Every column
+ * equals the single-vector result bit for bit.  block_width % 64 == 0. */
+int hbdia_hbdia_spmm_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                         int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                         const int32_t* coo_cols, const double* coo_values, const int32_t* coo_group_ptr,
+                         const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t nrhs,
+                         int64_t num_rows, hbdia_stream_t stream);
+int hbdia_hbdia_spmm_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                         int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                         const int32_t* coo_cols, const float* coo_values, const int32_t* coo_group_ptr,
+                         const float* X, int64_t ldx, float* Y, int64_t ldy, int32_t nrhs,
+                         int64_t num_rows, hbdia_stream_t stream);
+
+/* Reduced-precision storage (no reference counterpart: the reference stores
+ * and computes in one type).  values_lp / coo_values_lp are FP32 or BF16 copies
+ * (vdtype) of the FP64 arrays, made with hbdia_convert_values (FP64 -> FP32
+ * round to nearest even, then for BF16 FP32 -> BF16 round to nearest even).
+ * x and y are FP64.  Each row is accumulated in FP32 (products of the stored
+ * value and x rounded to FP32) together with m = sum |a x| and the bound
+ * e = (u_store + (n+2) 2^-24) m on its error (u_store = 2^-24 for FP32,
+ * 2^-8 + 2^-24 for BF16; n terms).  A row is recomputed from the FP64 arrays,
+ * with exactly the arithmetic of hbdia_hbdia_spmv_f64 / hbdia_bdia_spmv_f64,
+ * when its cancellation ratio against the smallest consistent |y| exceeds
+ * `ratio`: m > ratio (|sum| - e), or the sum is not finite.  Kept rows then
+ * have a relative error of at most e / (|sum| - e) < u_eff * ratio.  coo_group_ptr == NULL: no
+ * fallback.  workspace: hbdia_mixed_workspace_bytes(num_rows) bytes of device
+ * memory (8-byte aligned); after the call (stream-ordered) it holds one 64-bit
+ * mask per 64-row group whose bit r is set when row 64 g + r was recomputed in
+ * FP64.  Every call rewrites every mask, so nothing needs clearing; calls that
+ * share a workspace must not overlap.  block_width % 64 == 0. */
+#define HBDIA_VALUES_F32 1
+#define HBDIA_VALUES_BF16 2
+int hbdia_convert_values(const double* src, void* dst, int64_t n, int vdtype, hbdia_stream_t stream);
+size_t hbdia_mixed_workspace_bytes(int64_t num_rows);
+int hbdia_hbdia_spmv_mixed(int vdtype, const void* values_lp, const double* values_f64, const int32_t* offsets,
+                           const int32_t* block_start, int32_t num_blocks, int32_t block_width,
+                           const int32_t* coo_rows, const int32_t* coo_cols, const void* coo_values_lp,
+                           const double* coo_values_f64, const int32_t* coo_group_ptr, const double* x,
+                           double* y, int64_t num_rows, double ratio, void* workspace, hbdia_stream_t stream);
+
 /* COO fallback alone, into a result buffer: res[r] = sequential FMA sum of the
  * row's entries for rows that have entries; other rows keep res.  The GPU
  * counterpart of cpu_coo_spmv_kernel (HBDIASpMV.cpp:35-99). */

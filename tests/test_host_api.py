@@ -146,7 +146,7 @@ def test_generator_thread_count_independent():
 
 def _declared(header: Path) -> list[str]:
     text = header.read_text()
-    return re.findall(r"^\s*(?:int|const char\*)\s+(hbdia_\w+)\s*\(", text, flags=re.M)
+    return re.findall(r"^\s*(?:int|size_t|const char\*)\s+(hbdia_\w+)\s*\(", text, flags=re.M)
 
 
 @pytest.mark.parametrize("header,lib,bound", [("hbdia_hip.h", "libhbdia_hip.so", _native.HIP_FUNCTIONS),
