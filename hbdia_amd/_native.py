@@ -43,6 +43,7 @@ _HIP_SIGS = {
     "hbdia_device_synchronize": (_i, []),
     "hbdia_mem_info": (_i, [POINTER(c_size_t), POINTER(c_size_t)]),
     "hbdia_device_name": (_i, [ctypes.c_char_p, c_size_t]),
+    "hbdia_device_compute_units": (_i, [POINTER(c_int)]),
     "hbdia_malloc": (_i, [POINTER(c_void_p), c_size_t]),
     "hbdia_free": (_i, [_p]),
     "hbdia_malloc_host": (_i, [POINTER(c_void_p), c_size_t]),
@@ -76,6 +77,10 @@ _HIP_SIGS = {
                                   c_int64, _p]),
     "hbdia_convert_values": (_i, [_p, _p, c_int64, c_int, _p]),
     "hbdia_mixed_workspace_bytes": (c_size_t, [c_int64]),
+    "hbdia_batch_make_tiles": (_i, [_p, c_int32, _p, _p, _p, c_int32, _p, c_int64]),
+    "hbdia_batch_spmv_f64": (_i, [_p, _p, _p, _p, _p]),
+    "hbdia_batch_spmv_f32": (_i, [_p, _p, _p, _p, _p]),
+    "hbdia_batch_occupancy": (_i, [_p, c_int32, c_int32, POINTER(c_int)]),
     "hbdia_hbdia_spmv_mixed": (_i, [c_int, _p, _p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, _p, c_int64,
                                     c_double, _p, _p]),
     "hbdia_coo_spmv_f64": (_i, [_p, _p, _p, _p, _p, _p, c_int64, _p]),
@@ -118,6 +123,24 @@ _CAPI_SIGS = {
     "hbdia_gen_stencil": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_banded": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_vector": (_i, [c_int, c_int64, c_uint64, c_double, c_double, _p]),
+    "hbdia_gen_banded_batch": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_double,
+                                    c_uint64]),
+    "hbdia_banded_batch_shape": (_i, [c_int, c_int, c_int, c_int, c_int, c_uint64, POINTER(c_int32),
+                                      POINTER(c_int32), POINTER(c_uint64)]),
+    "hbdia_batch_create": (_i, [POINTER(c_void_p), POINTER(c_void_p), c_int]),
+    "hbdia_batch_autotune": (_i, [POINTER(c_void_p), POINTER(c_void_p), c_int, POINTER(c_int), c_int, c_int, c_int,
+                                  POINTER(c_double), c_int, POINTER(c_int)]),
+    "hbdia_batch_destroy": (_i, [_p]),
+    "hbdia_batch_info": (_i, [_p, POINTER(c_int64)]),
+    "hbdia_batch_row_offsets": (_i, [_p, POINTER(c_int64)]),
+    "hbdia_batch_set_schedule": (_i, [_p, c_int, c_int, c_int]),
+    "hbdia_batch_resident_workgroups": (_i, [_p, c_int, POINTER(c_int)]),
+    "hbdia_batch_spmv": (_i, [_p, _p, _p]),
+    "hbdia_batch_spmv_per_matrix": (_i, [_p, _p, _p]),
+    "hbdia_batch_capture_graph": (_i, [_p, _p, _p]),
+    "hbdia_batch_launch_graph": (_i, [_p]),
+    "hbdia_batch_synchronize": (_i, [_p]),
+    "hbdia_batch_stream": (_i, [_p, POINTER(c_void_p)]),
 }
 
 

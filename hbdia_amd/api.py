@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import ctypes
 import enum
-from ctypes import byref, c_void_p
+from ctypes import byref, c_double, c_void_p
 
 import numpy as np
 
@@ -102,6 +102,26 @@ class HBDIA:
         check_capi(capi.hbdia_gen_banded(byref(h), _code(dtype), int(n), int(noffsets), int(max_offset),
                                          float(offband), int(seed)))
         return cls(dtype=dtype, _handle=h.value)
+
+    @classmethod
+    def banded_batch(cls, count: int = 2048, min_rows: int = 1000, max_rows: int = 100000, min_offsets: int = 3,
+                     max_offsets: int = 9, max_offset: int = 64, offband: float = 0.01, seed: int = 0xC4B,
+                     dtype=np.float64) -> list["HBDIA"]:
+        """The seeded set of independent banded matrices (BandedBatchSpec in include/Workloads.hpp)."""
+        hs = (c_void_p * int(count))()
+        check_capi(capi.hbdia_gen_banded_batch(hs, _code(dtype), int(count), int(min_rows), int(max_rows),
+                                               int(min_offsets), int(max_offsets), int(max_offset), float(offband),
+                                               int(seed)))
+        return [cls(dtype=dtype, _handle=h) for h in hs]
+
+    @staticmethod
+    def banded_batch_shape(i: int, min_rows: int = 1000, max_rows: int = 100000, min_offsets: int = 3,
+                           max_offsets: int = 9, seed: int = 0xC4B) -> tuple[int, int, int]:
+        """(rows, offsets, seed) of matrix i of banded_batch."""
+        r, k, sd = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_uint64(0)
+        check_capi(capi.hbdia_banded_batch_shape(int(i), int(min_rows), int(max_rows), int(min_offsets),
+                                                 int(max_offsets), int(seed), byref(r), byref(k), byref(sd)))
+        return int(r.value), int(k.value), int(sd.value)
 
     def create3DStencil27Point(self, nx: int, ny: int, nz: int, noise: float = 0.0, iteration: int = 0) -> None:
         check_capi(capi.hbdia_matrix_stencil27(self._h, nx, ny, nz, float(noise), iteration))
@@ -333,6 +353,107 @@ class HBDIAMixed:
         try:
             if getattr(self, "_h", None):
                 capi.hbdia_mixed_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class HBDIABatch:
+    """Many independent matrices multiplied in one persistent launch
+    (include/Operations/HBDIABatch.hpp).
+
+    x and y are device pools (DeviceArray or raw pointers) of `rows` values;
+    matrix m owns rows [row_offsets[m], row_offsets[m+1]).  Each matrix's part of
+    y equals hbdiaSpMV(matrix, x_m, y_m, False, True) bit for bit."""
+
+    INTERLEAVED, CONTIGUOUS = 0, 1
+    _INFO = ["matrices", "rows", "tiles", "bdia_nnz", "coo_nnz", "algorithmic_bytes", "stored_bytes", "flops",
+             "kind", "tiles_per_step", "workgroups", "dtype"]
+
+    def __init__(self, matrices: list[HBDIA] | None = None, _handle=None, table=None):
+        if _handle is None:
+            if not matrices:
+                raise ValueError("need at least one matrix")
+            arr = (c_void_p * len(matrices))(*[m._h for m in matrices])
+            h = c_void_p(None)
+            check_capi(capi.hbdia_batch_create(byref(h), arr, len(matrices)))
+            _handle = h.value
+        self._h = _handle
+        self.table = table or []
+        info = self.info()
+        self.dtype = np.dtype(np.float64 if info["dtype"] == F64 else np.float32)
+        offs = np.zeros(info["matrices"] + 1, np.int64)
+        check_capi(capi.hbdia_batch_row_offsets(self._h, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        self.row_offsets = offs
+
+    @classmethod
+    def autotune(cls, matrices: list[HBDIA], block_widths=(64, 128, 256), threshold: int = 2,
+                 reps: int = 10) -> "HBDIABatch":
+        """Times every block width and tile schedule; the matrices are left converted at the chosen width.
+        The timed candidates are in .table as dicts."""
+        arr = (c_void_p * len(matrices))(*[m._h for m in matrices])
+        bws = (ctypes.c_int * len(block_widths))(*[int(b) for b in block_widths])
+        cap = 64 * len(block_widths)
+        tab = (c_double * (5 * cap))()
+        nrows = ctypes.c_int(0)
+        h = c_void_p(None)
+        check_capi(capi.hbdia_batch_autotune(byref(h), arr, len(matrices), bws, len(block_widths), int(threshold),
+                                             int(reps), tab, cap, byref(nrows)))
+        rows = [dict(block_width=int(tab[5 * i]), kind=int(tab[5 * i + 1]), tiles_per_step=int(tab[5 * i + 2]),
+                     workgroups=int(tab[5 * i + 3]), ms=float(tab[5 * i + 4])) for i in range(min(cap, nrows.value))]
+        return cls(_handle=h.value, table=rows)
+
+    def info(self) -> dict:
+        a = np.zeros(len(self._INFO), np.int64)
+        check_capi(capi.hbdia_batch_info(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return dict(zip(self._INFO, (int(v) for v in a)))
+
+    @property
+    def rows(self) -> int:
+        return int(self.row_offsets[-1])
+
+    def set_schedule(self, kind: int = 0, tiles_per_step: int = 1, workgroups: int = 0) -> None:
+        check_capi(capi.hbdia_batch_set_schedule(self._h, int(kind), int(tiles_per_step), int(workgroups)))
+
+    def resident_workgroups(self, kind: int = 0) -> int:
+        n = ctypes.c_int(0)
+        check_capi(capi.hbdia_batch_resident_workgroups(self._h, int(kind), byref(n)))
+        return int(n.value)
+
+    @staticmethod
+    def _ptr(a):
+        return a.ptr if hasattr(a, "ptr") else a
+
+    def spmv(self, x, y) -> None:
+        check_capi(capi.hbdia_batch_spmv(self._h, self._ptr(x), self._ptr(y)))
+
+    def spmv_per_matrix(self, x, y) -> None:
+        check_capi(capi.hbdia_batch_spmv_per_matrix(self._h, self._ptr(x), self._ptr(y)))
+
+    def capture_graph(self, x, y) -> None:
+        check_capi(capi.hbdia_batch_capture_graph(self._h, self._ptr(x), self._ptr(y)))
+
+    def launch_graph(self) -> None:
+        check_capi(capi.hbdia_batch_launch_graph(self._h))
+
+    def synchronize(self) -> None:
+        check_capi(capi.hbdia_batch_synchronize(self._h))
+
+    @property
+    def stream_handle(self) -> int:
+        s = c_void_p(None)
+        check_capi(capi.hbdia_batch_stream(self._h, byref(s)))
+        return s.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            check_capi(capi.hbdia_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                capi.hbdia_batch_destroy(self._h)
                 self._h = None
         except Exception:
             pass

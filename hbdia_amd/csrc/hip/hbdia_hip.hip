@@ -59,6 +59,25 @@ template <> __device__ __forceinline__ float fma_t<float>(float a, float b, floa
   return __builtin_fmaf(a, b, c);
 }
 
+// ok ? fma(a, b, c) : c, computed as fma(a & m, b & m, c) with m = all ones or
+// zero.  For !ok that is fma(+0, +0, c) = c + (+0), which is c for every c
+// except -0; the FMA chains here start at +0 and round to nearest, so they
+// never hold -0.  Masking the operands instead of selecting the result keeps
+// every load of a chunk unconditional: with a select the compiler sinks the x
+// load behind the test of the loaded value, a second memory latency per chunk.
+__device__ __forceinline__ double masked_fma(bool ok, double a, double b, double c) {
+  const uint64_t m = ok ? ~0ull : 0ull;
+  const double am = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, a) & m);
+  const double bm = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, b) & m);
+  return __builtin_fma(am, bm, c);
+}
+__device__ __forceinline__ float masked_fma(bool ok, float a, float b, float c) {
+  const uint32_t m = ok ? ~0u : 0u;
+  const float am = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, a) & m);
+  const float bm = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, b) & m);
+  return __builtin_fmaf(am, bm, c);
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -69,11 +88,10 @@ __device__ __forceinline__ void wave_sync() {
 // row, in entry order, as sequential FMAs (the reference CPU COO order).
 // Entries [p0, p1) are row-sorted and all lie in rows [row0, row0 + 64).
 template <typename T>
-__device__ __forceinline__ T coo_group_sum(const int* __restrict__ cr, const int* __restrict__ cc,
-                                           const T* __restrict__ cv, const T* __restrict__ x,
-                                           int p0, int p1, int64_t row0, int lane, T* s_v,
-                                           T* s_x, int* s_lo, int* s_hi, bool* any) {
-  T cs = T(0);
+__device__ __forceinline__ T coo_group_sum_from(const int* __restrict__ cr, const int* __restrict__ cc,
+                                                const T* __restrict__ cv, const T* __restrict__ x,
+                                                int p0, int p1, int64_t row0, int lane, T* s_v,
+                                                T* s_x, int* s_lo, int* s_hi, T cs, bool* any) {
   bool has = false;
   for (int base = p0; base < p1; base += kWave) {
     const int i = base + lane;
@@ -105,6 +123,14 @@ __device__ __forceinline__ T coo_group_sum(const int* __restrict__ cr, const int
   return cs;
 }
 
+template <typename T>
+__device__ __forceinline__ T coo_group_sum(const int* __restrict__ cr, const int* __restrict__ cc,
+                                           const T* __restrict__ cv, const T* __restrict__ x,
+                                           int p0, int p1, int64_t row0, int lane, T* s_v,
+                                           T* s_x, int* s_lo, int* s_hi, bool* any) {
+  return coo_group_sum_from<T>(cr, cc, cv, x, p0, p1, row0, lane, s_v, s_x, s_lo, s_hi, T(0), any);
+}
+
 // Element `i` of a wave-uniform base as base + 32-bit byte offset, so that the
 // load uses the scalar-base addressing form (one VGPR per address).
 template <typename T>
@@ -125,7 +151,10 @@ __device__ __forceinline__ T load_x(const T* __restrict__ x, int64_t c) {
 // Offsets are consumed in chunks of kChunk with no control flow inside a
 // chunk: the tail of the last chunk re-reads the last valid line (a cache hit)
 // and x addresses are clamped into [0, nrows), so every load of a chunk can
-// be in flight at once.  The FMA chain runs over ascending offsets and skips
+// be in flight at once.  A chunk's offsets are read as kChunk consecutive
+// entries (one wide scalar load; entries past the block's count belong to the
+// next block or to the padding behind the array and only feed clamped, unused
+// x loads).  The FMA chain runs over ascending offsets and skips
 // zero values and columns outside [0, nrows), exactly as the reference kernel.
 template <typename T, bool X32>
 __device__ __forceinline__ T bdia_sum_uniform(const T* __restrict__ vb, const int* __restrict__ op, int cnt,
@@ -137,7 +166,7 @@ __device__ __forceinline__ T bdia_sum_uniform(const T* __restrict__ vb, const in
     int off[kChunk];
     T v[kChunk], xl[kChunk];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) off[k] = op[min(o0 + k, cnt - 1)];
+    for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
       v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
@@ -150,8 +179,8 @@ __device__ __forceinline__ T bdia_sum_uniform(const T* __restrict__ vb, const in
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) {
       const int64_t c = row + off[k];
-      const bool ok = (o0 + k < cnt) && active && v[k] != T(0) && c >= 0 && c < nrows;
-      sum = ok ? fma_t<T>(v[k], xl[k], sum) : sum;
+      const bool ok = (o0 + k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
+      sum = masked_fma(ok, v[k], xl[k], sum);
     }
   }
   return sum;
@@ -182,8 +211,8 @@ __device__ __forceinline__ T bdia_sum_lane(const T* __restrict__ vp, const int* 
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) {
       const int64_t c = row + off[k];
-      const bool ok = (o0 + k < cnt) && v[k] != T(0) && c >= 0 && c < nrows;
-      sum = ok ? fma_t<T>(v[k], xl[k], sum) : sum;
+      const bool ok = (o0 + k < cnt) & (v[k] != T(0)) & (c >= 0) & (c < nrows);
+      sum = masked_fma(ok, v[k], xl[k], sum);
     }
   }
   return sum;
@@ -346,7 +375,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
       T v[CH];
       int64_t cl[CH];
 #pragma unroll
-      for (int k = 0; k < CH; ++k) off[k] = op[min(o0 + k, cnt - 1)];
+      for (int k = 0; k < CH; ++k) off[k] = op[o0 + k];
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
         const int kk = min(o0 + k, cnt - 1);
@@ -365,8 +394,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
 #pragma unroll
         for (int k = 0; k < CH; ++k) {
           const int64_t c = row + off[k];
-          const bool ok = (o0 + k < cnt) && active && v[k] != T(0) && c >= 0 && c < nrows;
-          sum[j] = ok ? fma_t<T>(v[k], xl[k], sum[j]) : sum[j];
+          const bool ok = (o0 + k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
+          sum[j] = masked_fma(ok, v[k], xl[k], sum[j]);
         }
       }
     }
@@ -400,6 +429,217 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// Many matrices in one persistent launch.  A tile is 64 consecutive rows of one
+// matrix; it is processed with exactly the arithmetic of k_hbdia<T, true, ...>
+// on that matrix alone (blocked part: ascending-offset FMA chain with the zero
+// and column-range skips; fallback fused when the matrix has one), so every
+// matrix's rows equal the single-matrix results bit for bit.
+//
+// A tile is a short chain of dependent loads (record -> offsets -> values and
+// x -> fallback x gather), so a wave overlaps tiles: the next tile's record and
+// first offsets are loaded while the current tile's vector loads are in
+// flight, and the current tile's fallback entries are loaded before its
+// blocked part is summed.
+
+constexpr int kTileGroup = 16;  // tile records fetched together by one wave
+constexpr int kTileDwords = 20;  // 80-byte tile record
+static_assert(sizeof(hbdia_batch_tile_t) == kTileDwords * 4, "tile record layout");
+static_assert(HBDIA_BATCH_INLINE_OFFSETS >= kChunk, "a tile record carries the first chunk's offsets");
+
+template <typename T, bool HAS_COO, bool X32, bool CONTIG>
+__global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__ tiles,
+                                                     const int32_t* __restrict__ wave_tiles,
+                                                     const T* __restrict__ vals, const int* __restrict__ offs,
+                                                     const int* __restrict__ cr, const int* __restrict__ cc,
+                                                     const T* __restrict__ cv, int64_t num_tiles, int tiles_per_step,
+                                                     const T* __restrict__ xp, T* __restrict__ yp) {
+  __shared__ T s_v[kWavesPerWG][kWave];
+  __shared__ T s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
+  const int64_t step = static_cast<int64_t>(gridDim.x) * kWavesPerWG * tiles_per_step;
+  int64_t first = 0, limit = num_tiles;
+  if constexpr (CONTIG) {
+    first = wave_tiles[wave];
+    limit = wave_tiles[wave + 1];
+  }
+  T* sv = s_v[wv];
+  T* sx = s_x[wv];
+  int* slo = s_lo[wv];
+  int* shi = s_hi[wv];
+  // The wave's tiles form a sequence s = 0, 1, ...: contiguous, tile first + s;
+  // interleaved, runs of tiles_per_step tiles `step` apart.  Valid tiles are a
+  // prefix of the sequence.  Lane 4j + q holds dwords [5q, 5q + 5) of the
+  // record of sequence entry s0 + j.
+  const int jl = lane >> 2, ql = lane & 3;
+  for (int64_t s0 = 0;; s0 += kTileGroup) {
+    const int64_t sj = s0 + jl;
+    int64_t tj;
+    if constexpr (CONTIG) tj = first + sj;
+    else tj = wave * tiles_per_step + (sj / tiles_per_step) * step + sj % tiles_per_step;
+    const bool vj = tj < limit;
+    const int32_t* rp = tiles + (vj ? tj : 0) * kTileDwords + ql * 5;
+    int rv[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) rv[r] = rp[r];
+    const int ng = __popcll(__ballot(vj && ql == 0));
+    if (ng == 0) break;
+    // fields of tile jj of the group, read from the lanes holding its record;
+    // the first tile's are read before the loop, each next one at the end of
+    // an iteration, so the loop never waits on the record load
+    struct Fields {
+      int value_line, rows, row0, num_rows, bw, cnt, offs_at, p0, p1, fused;
+      int off[kChunk];
+    };
+    auto read_fields = [&](int jj) {
+      Fields f;
+#define TILE_FIELD(i) __builtin_amdgcn_readlane(rv[(i) % 5], 4 * jj + (i) / 5)
+      f.value_line = TILE_FIELD(0);
+      f.rows = TILE_FIELD(1);
+      f.row0 = TILE_FIELD(2);
+      f.num_rows = TILE_FIELD(3);
+      f.bw = TILE_FIELD(4);
+      f.cnt = TILE_FIELD(5);
+      f.offs_at = TILE_FIELD(6);
+      f.p0 = TILE_FIELD(7);
+      f.p1 = TILE_FIELD(8);
+      f.fused = TILE_FIELD(9);
+      f.off[0] = TILE_FIELD(10);
+      f.off[1] = TILE_FIELD(11);
+      f.off[2] = TILE_FIELD(12);
+      f.off[3] = TILE_FIELD(13);
+      f.off[4] = TILE_FIELD(14);
+      f.off[5] = TILE_FIELD(15);
+      f.off[6] = TILE_FIELD(16);
+      f.off[7] = TILE_FIELD(17);
+      f.off[8] = TILE_FIELD(18);
+#undef TILE_FIELD
+      return f;
+    };
+    Fields nextf = read_fields(0);
+    for (int jj = 0; jj < ng; ++jj) {
+      const Fields cf = nextf;
+      const int value_line = cf.value_line, rows = cf.rows, row0 = cf.row0, num_rows = cf.num_rows, bw = cf.bw,
+                cnt = cf.cnt, offs_at = cf.offs_at, p0 = cf.p0, p1 = cf.p1, fused = cf.fused;
+      const int* off = cf.off;
+      if (jj + 1 < ng) nextf = read_fields(jj + 1);
+      const int64_t nrows = num_rows;
+      const int64_t row = static_cast<int64_t>(row0) + lane;  // row within the matrix
+      const bool active = row < nrows;
+      const T* x = xp + rows;
+      T* y = yp + rows;
+      const int64_t last = nrows - 1;
+
+      // ---- issue the tile's loads: fallback entries, first chunk of values and x
+      int r_ = 0, c_ = 0;
+      T ev = T(0);
+      if constexpr (HAS_COO) {
+        const int i = p0 + lane;  // inside the padded pool: loads need no guard
+        r_ = cr[i];
+        c_ = cc[i];
+        ev = cv[i];
+      }
+      const T* vb = vals + static_cast<int64_t>(value_line) * kWave;
+      const int lastk = max(cnt - 1, 0);
+      T v[kChunk], xl[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k)
+        v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(k, lastk) * bw + lane)));
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        int64_t c = row + off[k];
+        c = c < 0 ? 0 : (c > last ? last : c);
+        xl[k] = load_x<T, X32>(x, c);
+      }
+      // every load above is issued before anything waits for one of them
+      __builtin_amdgcn_sched_barrier(0);
+      T ex = T(0);
+      int er = -1;
+      if constexpr (HAS_COO) {
+        const int okm = -static_cast<int>(p0 + lane < p1);  // all ones or zero
+        ex = load_x<T, X32>(x, c_ & okm);
+        er = ((r_ - row0) & okm) | ~okm;  // -1 past the tile's entries
+        if (er < 0 || er >= kWave) er = -1;  // unsorted input: drop rather than write outside LDS
+      }
+
+      // ---- blocked part: the FMA chain over the first chunk, then any further chunks
+      T sum = T(0);
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        const int64_t c = row + off[k];
+        const bool ok = (k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
+        sum = masked_fma(ok, v[k], xl[k], sum);
+      }
+      if (cnt > kChunk) {
+        const int* op = offs + offs_at;
+        for (int o0 = kChunk; o0 < cnt; o0 += kChunk) {
+          int of2[kChunk];
+          T v2[kChunk], x2[kChunk];
+#pragma unroll
+          for (int k = 0; k < kChunk; ++k) of2[k] = op[o0 + k];  // padded pool
+#pragma unroll
+          for (int k = 0; k < kChunk; ++k)
+            v2[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
+#pragma unroll
+          for (int k = 0; k < kChunk; ++k) {
+            int64_t c = row + of2[k];
+            c = c < 0 ? 0 : (c > last ? last : c);
+            x2[k] = load_x<T, X32>(x, c);
+          }
+#pragma unroll
+          for (int k = 0; k < kChunk; ++k) {
+            const int64_t c = row + of2[k];
+            const bool ok = (o0 + k < cnt) & active & (v2[k] != T(0)) & (c >= 0) & (c < nrows);
+            sum = masked_fma(ok, v2[k], x2[k], sum);
+          }
+        }
+      }
+
+      // ---- fallback: per-row sums in entry order, fused into y
+      if constexpr (HAS_COO) {
+        // stored on every path: a fallback load left unconsumed would make the
+        // next tile wait for it (and for this tile's store) before reusing its register
+        slo[lane] = 0;
+        shi[lane] = 0;
+        sv[lane] = ev;
+        sx[lane] = ex;
+      }
+      if (HAS_COO && fused) {
+        T cs = T(0);
+        if (p0 < p1) {
+          const int erp = __shfl_up(er, 1);
+          const int ern = __shfl_down(er, 1);
+          wave_sync();
+          if (er >= 0 && (lane == 0 || erp != er)) slo[er] = lane;
+          if (er >= 0 && (lane == kWave - 1 || p0 + lane + 1 >= p1 || ern != er)) shi[er] = lane + 1;
+          wave_sync();
+          const int a = slo[lane], e = shi[lane];
+          for (int k = a; k < e; ++k) cs = fma_t<T>(sv[k], sx[k], cs);
+          wave_sync();
+          if (p1 - p0 > kWave) {  // more than 64 entries (rare): the generic loop
+            bool any = false;
+            cs = coo_group_sum_from<T>(cr, cc, cv, x, p0 + kWave, p1, row0, lane, sv, sx, slo, shi, cs, &any);
+          }
+        }
+        if (active) {
+          const T base = cnt > 0 ? sum : y[row];
+          __builtin_nontemporal_store(base + cs, y + row);
+        }
+      } else {
+        if (active && cnt > 0) __builtin_nontemporal_store(sum, y + row);
+      }
+    }
+    if (ng < kTileGroup) break;
+  }
+}
+
+
+
 
 // ---------------------------------------------------------------------------
 // Reduced-precision storage.  Values (blocked part and fallback) are held as
@@ -471,7 +711,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
       float v[kChunk];
       double xl[kChunk];
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) off[k] = op[min(o0 + k, cnt - 1)];
+      for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];
 #pragma unroll
       for (int k = 0; k < kChunk; ++k)
         v[k] = lp_to_f32<TV>(__builtin_nontemporal_load(vb + static_cast<int64_t>(min(o0 + k, cnt - 1)) * bw + lane));
@@ -484,7 +724,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
 #pragma unroll
       for (int k = 0; k < kChunk; ++k) {
         const int64_t c = row + off[k];
-        const bool ok = (o0 + k < cnt) && active && v[k] != 0.0f && c >= 0 && c < nrows;
+        const bool ok = (o0 + k < cnt) & active & (v[k] != 0.0f) & (c >= 0) & (c < nrows);
         const float xf = static_cast<float>(xl[k]);
         sum = ok ? __builtin_fmaf(v[k], xf, sum) : sum;
         sabs = ok ? __builtin_fmaf(__builtin_fabsf(v[k]), __builtin_fabsf(xf), sabs) : sabs;
@@ -799,6 +1039,68 @@ int launch_mixed(int vdtype, const void* vals_lp, const double* vals64, const in
 }
 
 template <typename T>
+int launch_batch(const hbdia_batch_t* bt, const hbdia_batch_schedule_t* sc, const T* x, T* y, hbdia_stream_t stream,
+                 const char* name) {
+  if (!bt || !sc) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null batch or schedule");
+  if (bt->num_tiles < 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative sizes");
+  if (sc->num_workgroups < 1 || sc->tiles_per_step < 1 ||
+      (sc->kind != HBDIA_BATCH_INTERLEAVED && sc->kind != HBDIA_BATCH_CONTIGUOUS))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": invalid schedule");
+  if (bt->num_tiles == 0) return HBDIA_SUCCESS;
+  if (!x || !y || !bt->tiles || !bt->values || !bt->offsets)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
+  if (bt->has_coo && (!bt->coo_rows || !bt->coo_cols || !bt->coo_values))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null fallback pool");
+  if (sc->kind == HBDIA_BATCH_CONTIGUOUS && !bt->wave_tiles)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": contiguous schedule without wave_tiles");
+  const dim3 grid(static_cast<unsigned>(sc->num_workgroups)), block(kWG);
+  hipStream_t s = hs(stream);
+  const bool coo = bt->has_coo != 0, x32 = bt->x32 != 0, contig = sc->kind == HBDIA_BATCH_CONTIGUOUS;
+#define HBDIA_BATCH_LAUNCH(C, X, G) \
+  hipLaunchKernelGGL((k_hbdia_batch<T, C, X, G>), grid, block, 0, s,                                          \
+                     reinterpret_cast<const int32_t*>(bt->tiles), bt->wave_tiles,                           \
+                     static_cast<const T*>(bt->values), bt->offsets, bt->coo_rows, bt->coo_cols,            \
+                     static_cast<const T*>(bt->coo_values), bt->num_tiles, sc->tiles_per_step, x, y)
+  if (coo) {
+    if (x32) {
+      if (contig) HBDIA_BATCH_LAUNCH(true, true, true); else HBDIA_BATCH_LAUNCH(true, true, false);
+    } else {
+      if (contig) HBDIA_BATCH_LAUNCH(true, false, true); else HBDIA_BATCH_LAUNCH(true, false, false);
+    }
+  } else {
+    if (x32) {
+      if (contig) HBDIA_BATCH_LAUNCH(false, true, true); else HBDIA_BATCH_LAUNCH(false, true, false);
+    } else {
+      if (contig) HBDIA_BATCH_LAUNCH(false, false, true); else HBDIA_BATCH_LAUNCH(false, false, false);
+    }
+  }
+#undef HBDIA_BATCH_LAUNCH
+  return check_hip(hipGetLastError(), name);
+}
+
+template <typename T>
+int batch_occupancy(bool coo, bool x32, bool contig, int* out) {
+  hipError_t e;
+#define HBDIA_BATCH_OCC(C, X, G) \
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_hbdia_batch<T, C, X, G>, kWG, 0)
+  if (coo) {
+    if (x32) {
+      if (contig) HBDIA_BATCH_OCC(true, true, true); else HBDIA_BATCH_OCC(true, true, false);
+    } else {
+      if (contig) HBDIA_BATCH_OCC(true, false, true); else HBDIA_BATCH_OCC(true, false, false);
+    }
+  } else {
+    if (x32) {
+      if (contig) HBDIA_BATCH_OCC(false, true, true); else HBDIA_BATCH_OCC(false, true, false);
+    } else {
+      if (contig) HBDIA_BATCH_OCC(false, false, true); else HBDIA_BATCH_OCC(false, false, false);
+    }
+  }
+#undef HBDIA_BATCH_OCC
+  return check_hip(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+}
+
+template <typename T>
 int launch_coo(const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr, const T* x,
                T* res, int64_t nrows, hbdia_stream_t stream, const char* name) {
   if (nrows < 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative size");
@@ -853,6 +1155,15 @@ int hbdia_device_name(char* buf, size_t len) {
   if (rc) return rc;
   std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
   return HBDIA_SUCCESS;
+}
+
+int hbdia_device_compute_units(int* count) {
+  if (!count) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_device_compute_units: null");
+  int dev = 0;
+  int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  return check_hip(hipDeviceGetAttribute(count, hipDeviceAttributeMultiprocessorCount, dev),
+                   "hipDeviceGetAttribute");
 }
 
 int hbdia_malloc(void** ptr, size_t bytes) {
@@ -1014,6 +1325,82 @@ int hbdia_hbdia_spmm_f32(const float* values, const int32_t* offsets, const int3
                          hbdia_stream_t stream) {
   return launch_multi<float>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols, coo_values,
                              coo_group_ptr, X, ldx, Y, ldy, nrhs, num_rows, stream, "hbdia_hbdia_spmm_f32");
+}
+
+int hbdia_batch_make_tiles(const hbdia_batch_desc_t* desc, int32_t num_matrices, const int32_t* offsets,
+                           const int32_t* block_start, const int32_t* group_ptr, int32_t value_size,
+                           hbdia_batch_tile_t* tiles, int64_t num_tiles) {
+  if (num_matrices < 0 || (num_matrices > 0 && (!desc || !offsets || !block_start || !tiles)) ||
+      (value_size != 4 && value_size != 8))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_batch_make_tiles: invalid argument");
+  constexpr int64_t kMax32 = 0x7fffffff;
+  for (int32_t m = 0; m < num_matrices; ++m) {
+    const hbdia_batch_desc_t& d = desc[m];
+    if (d.num_rows < 0 || d.block_width <= 0 || d.block_width % kWave != 0)
+      return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_batch_make_tiles: block width not a multiple of 64");
+    if (d.coo_nnz > 0 && !group_ptr)
+      return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_batch_make_tiles: fallback without group pointers");
+    if (d.values % kWave != 0 || d.rows + d.num_rows > kMax32)
+      return fail(HBDIA_ERROR_NOT_SUPPORTED, "hbdia_batch_make_tiles: pools beyond 32-bit tile fields");
+    const int64_t groups = num_groups(d.num_rows);
+    if (d.first_tile < 0 || d.first_tile + groups > num_tiles)
+      return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_batch_make_tiles: tiles outside the tile array");
+    const int32_t* bs = block_start + d.block_start;
+    for (int64_t g = 0; g < groups; ++g) {
+      hbdia_batch_tile_t& tl = tiles[d.first_tile + g];
+      const int64_t row0 = g * kWave;
+      const int64_t b = row0 / d.block_width;
+      int32_t start = 0, cnt = 0;
+      if (b < d.num_blocks) {
+        start = bs[b];
+        cnt = bs[b + 1] - start;
+      }
+      if (static_cast<int64_t>(cnt) * d.block_width * value_size >= (int64_t{1} << 32))
+        return fail(HBDIA_ERROR_NOT_SUPPORTED, "hbdia_batch_make_tiles: a block's values exceed 2^32 bytes");
+      const int64_t v = cnt > 0 ? d.values + static_cast<int64_t>(start) * d.block_width + (row0 - b * d.block_width)
+                                : 0;
+      if (v / kWave > kMax32)
+        return fail(HBDIA_ERROR_NOT_SUPPORTED, "hbdia_batch_make_tiles: values pool beyond 2^37 entries");
+      tl.value_line = static_cast<int32_t>(v / kWave);
+      tl.rows = static_cast<int32_t>(d.rows);
+      tl.row0 = static_cast<int32_t>(row0);
+      tl.num_rows = d.num_rows;
+      tl.stride = d.block_width;
+      tl.count = cnt;
+      tl.offsets = d.offsets + start;
+      for (int k = 0; k < HBDIA_BATCH_INLINE_OFFSETS; ++k)
+        tl.first_offsets[k] = k < cnt ? offsets[d.offsets + start + k] : 0;
+      if (d.coo_nnz > 0) {
+        const int32_t* gp = group_ptr + d.group_ptr;
+        tl.coo_begin = d.coo + gp[g];
+        tl.coo_end = d.coo + gp[g + 1];
+        tl.fused = 1;
+      } else {
+        tl.coo_begin = tl.coo_end = 0;
+        tl.fused = 0;
+      }
+    }
+  }
+  return HBDIA_SUCCESS;
+}
+
+int hbdia_batch_spmv_f64(const hbdia_batch_t* batch, const hbdia_batch_schedule_t* schedule, const double* x,
+                         double* y, hbdia_stream_t stream) {
+  return launch_batch<double>(batch, schedule, x, y, stream, "hbdia_batch_spmv_f64");
+}
+
+int hbdia_batch_spmv_f32(const hbdia_batch_t* batch, const hbdia_batch_schedule_t* schedule, const float* x,
+                         float* y, hbdia_stream_t stream) {
+  return launch_batch<float>(batch, schedule, x, y, stream, "hbdia_batch_spmv_f32");
+}
+
+int hbdia_batch_occupancy(const hbdia_batch_t* batch, int32_t kind, int32_t value_size, int* workgroups_per_cu) {
+  if (!batch || !workgroups_per_cu || (value_size != 4 && value_size != 8) ||
+      (kind != HBDIA_BATCH_INTERLEAVED && kind != HBDIA_BATCH_CONTIGUOUS))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_batch_occupancy: invalid argument");
+  const bool coo = batch->has_coo != 0, x32 = batch->x32 != 0, contig = kind == HBDIA_BATCH_CONTIGUOUS;
+  return value_size == 8 ? batch_occupancy<double>(coo, x32, contig, workgroups_per_cu)
+                         : batch_occupancy<float>(coo, x32, contig, workgroups_per_cu);
 }
 int hbdia_convert_values(const double* src, void* dst, int64_t n, int vdtype, hbdia_stream_t stream) {
   if (n < 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_convert_values: negative size");

@@ -416,9 +416,8 @@ bool HBDIA<T>::convertToHBDIAFormatOnHost(int bw, int thr, int maxCoo, bool /*CO
     const int nth = omp_get_max_threads();
     std::vector<std::vector<int64_t>> cnt(nth, std::vector<int64_t>(static_cast<size_t>(nb) + 1, 0));
     const int64_t chunk = (nnz + nth - 1) / nth;
-#pragma omp parallel num_threads(nth)
-    {
-      const int t = omp_get_thread_num();
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int t = 0; t < nth; ++t) {
       const int64_t lo = std::min(nnz, t * chunk), hi = std::min(nnz, lo + chunk);
       for (int64_t i = lo; i < hi; ++i) cnt[t][keyOf(i)]++;
     }
@@ -433,9 +432,8 @@ bool HBDIA<T>::convertToHBDIAFormatOnHost(int bw, int thr, int maxCoo, bool /*CO
     }
     bstart[nb + 1] = run;
     order.resize(nnz);
-#pragma omp parallel num_threads(nth)
-    {
-      const int t = omp_get_thread_num();
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int t = 0; t < nth; ++t) {
       const int64_t lo = std::min(nnz, t * chunk), hi = std::min(nnz, lo + chunk);
       for (int64_t i = lo; i < hi; ++i) order[cnt[t][keyOf(i)]++] = i;
     }
@@ -547,9 +545,8 @@ bool HBDIA<T>::convertToHBDIAFormatOnHost(int bw, int thr, int maxCoo, bool /*CO
   {
     const int64_t chunk = (nnz + nth - 1) / nth;
     std::vector<int64_t> cnt(nth + 1, 0);
-#pragma omp parallel num_threads(nth)
-    {
-      const int t = omp_get_thread_num();
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int t = 0; t < nth; ++t) {
       const int64_t lo = std::min(nnz, t * chunk), hi = std::min(nnz, lo + chunk);
       int64_t c = 0;
       for (int64_t i = lo; i < hi; ++i) c += toCoo[i];
@@ -559,9 +556,8 @@ bool HBDIA<T>::convertToHBDIAFormatOnHost(int bw, int thr, int maxCoo, bool /*CO
     cpuRowIndices.assign(cnt[nth], 0);
     cpuColIndices.assign(cnt[nth], 0);
     cpuValues.assign(cnt[nth], T(0));
-#pragma omp parallel num_threads(nth)
-    {
-      const int t = omp_get_thread_num();
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int t = 0; t < nth; ++t) {
       const int64_t lo = std::min(nnz, t * chunk), hi = std::min(nnz, lo + chunk);
       int64_t w = cnt[t];
       for (int64_t i = lo; i < hi; ++i)
@@ -692,10 +688,13 @@ void HBDIA<T>::prepareForGPU() {
     flattenedOffsets.insert(flattenedOffsets.end(), offsetsPerBlock[b].begin(), offsetsPerBlock[b].end());
   }
   blockStartIndices.push_back(static_cast<int>(flattenedOffsets.size()));
+  // the kernels read a chunk's offsets as one wide load: keep the tail readable
+  std::vector<int> paddedOffsets(flattenedOffsets);
+  paddedOffsets.resize(flattenedOffsets.size() + HBDIA_OFFSETS_PAD, 0);
   initializeStreams();
 
   dev_upload(&hbdiaData_d_, hbdiaData.data(), hbdiaData.size());
-  dev_upload(&flattenedOffsets_d_, flattenedOffsets.data(), flattenedOffsets.size());
+  dev_upload(&flattenedOffsets_d_, paddedOffsets.data(), paddedOffsets.size());
   dev_upload(&blockStartIndices_d_, blockStartIndices.data(), blockStartIndices.size());
   dev_upload(&blockSizes_d_, blockSizes.data(), blockSizes.size());
 

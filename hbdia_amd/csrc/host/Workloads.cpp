@@ -11,7 +11,7 @@
 namespace hbdia {
 namespace {
 
-constexpr uint64_t kStreamRow = 1, kStreamCol = 2, kStreamVal = 3, kStreamVec = 4, kStreamOff = 5;
+constexpr uint64_t kStreamRow = 1, kStreamCol = 2, kStreamVal = 3, kStreamVec = 4, kStreamOff = 5, kStreamBatch = 6;
 
 inline uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
@@ -201,7 +201,9 @@ void generateBanded(int n_, int noffsets, int maxOffset, double offband, uint64_
   auto inBand = [&](int64_t r, int64_t col) {
     return std::binary_search(offs.begin(), offs.end(), static_cast<int>(col - r));
   };
-  const int64_t extra = static_cast<int64_t>(offband * static_cast<double>(n));
+  int64_t bandNnz = 0;
+  for (int o : offs) bandNnz += std::max<int64_t>(0, n - (o < 0 ? -o : o));
+  const int64_t extra = static_cast<int64_t>(offband * static_cast<double>(bandNnz));
   std::vector<int> extraPerRow(n, 0);
   for (int64_t e = 0; e < extra; ++e)
     extraPerRow[bounded(hash4(seed, kStreamRow, static_cast<uint64_t>(e), 0), n)]++;
@@ -259,6 +261,30 @@ void generateBanded(int n_, int noffsets, int maxOffset, double offband, uint64_
   }
 }
 
+BandedShape bandedBatchShape(const BandedBatchSpec& spec, int i) {
+  if (spec.minRows < 1 || spec.maxRows < spec.minRows || spec.minOffsets < 1 || spec.maxOffsets < spec.minOffsets)
+    throw std::invalid_argument("bandedBatchShape: invalid spec");
+  const uint64_t h = hash4(spec.seed, kStreamBatch, static_cast<uint64_t>(i), 0);
+  const double lo = std::log(static_cast<double>(spec.minRows)), hi = std::log(static_cast<double>(spec.maxRows));
+  int n = static_cast<int>(std::floor(std::exp(lo + (hi - lo) * u01(h))));
+  n = std::min(std::max(n, spec.minRows), spec.maxRows);
+  const int span = spec.maxOffsets - spec.minOffsets + 1;
+  const int k = spec.minOffsets + static_cast<int>(bounded(hash4(spec.seed, kStreamBatch, static_cast<uint64_t>(i), 1),
+                                                           static_cast<uint64_t>(span)));
+  return {n, k, hash4(spec.seed, kStreamBatch, static_cast<uint64_t>(i), 2)};
+}
+
+template <typename T>
+void generateBandedBatch(const BandedBatchSpec& spec, std::vector<CooMatrix<T>>& out) {
+  if (spec.count < 0) throw std::invalid_argument("generateBandedBatch: negative count");
+  out.assign(static_cast<size_t>(spec.count), CooMatrix<T>{});
+  std::vector<BandedShape> shapes(static_cast<size_t>(spec.count));
+  for (int i = 0; i < spec.count; ++i) shapes[i] = bandedBatchShape(spec, i);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < spec.count; ++i)
+    generateBanded<T>(shapes[i].rows, shapes[i].noffsets, spec.maxOffset, spec.offband, shapes[i].seed, out[i]);
+}
+
 template <typename T>
 void generateUniformVector(int64_t n, uint64_t seed, double lo, double hi, T* out) {
 #pragma omp parallel for schedule(static)
@@ -270,6 +296,8 @@ template void generateStencilScatter<double>(int, int, double, uint64_t, CooMatr
 template void generateStencilScatter<float>(int, int, double, uint64_t, CooMatrix<float>&);
 template void generateBanded<double>(int, int, int, double, uint64_t, CooMatrix<double>&);
 template void generateBanded<float>(int, int, int, double, uint64_t, CooMatrix<float>&);
+template void generateBandedBatch<double>(const BandedBatchSpec&, std::vector<CooMatrix<double>>&);
+template void generateBandedBatch<float>(const BandedBatchSpec&, std::vector<CooMatrix<float>>&);
 template void generateUniformVector<double>(int64_t, uint64_t, double, double, double*);
 template void generateUniformVector<float>(int64_t, uint64_t, double, double, float*);
 

@@ -11,6 +11,7 @@
 
 #include "Format/HBDIA.hpp"
 #include "Format/HBDIAVector.hpp"
+#include "Operations/HBDIABatch.hpp"
 #include "Operations/HBDIAMixed.hpp"
 #include "Operations/HBDIASpMV.hpp"
 #include "Workloads.hpp"
@@ -23,6 +24,12 @@ struct hbdia_matrix_opaque {
 
 struct hbdia_mixed_opaque {
   std::unique_ptr<HBDIAMixed> impl;
+};
+
+struct hbdia_batch_opaque {
+  int dtype = HBDIA_DTYPE_F64;
+  std::unique_ptr<HBDIABatch<double>> d;
+  std::unique_ptr<HBDIABatch<float>> f;
 };
 
 struct hbdia_vector_opaque {
@@ -439,3 +446,239 @@ int hbdia_gen_vector(int dtype, int64_t n, uint64_t seed, double lo, double hi, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// many-matrix batches
+
+namespace {
+
+template <typename T>
+std::unique_ptr<HBDIA<T>>& member(hbdia_matrix_opaque* m);
+template <>
+std::unique_ptr<HBDIA<double>>& member<double>(hbdia_matrix_opaque* m) { return m->d; }
+template <>
+std::unique_ptr<HBDIA<float>>& member<float>(hbdia_matrix_opaque* m) { return m->f; }
+
+int batch_dtype(const hbdia_matrix_t* ms, int count) {
+  need(ms != nullptr && count > 0, "need at least one matrix");
+  for (int i = 0; i < count; ++i) need(ms[i] != nullptr, "null matrix handle");
+  const int dt = ms[0]->dtype;
+  for (int i = 1; i < count; ++i) need(ms[i]->dtype == dt, "matrices of one dtype only");
+  return dt;
+}
+
+template <typename F>
+auto with_batch(hbdia_batch_handle_t b, F&& fn) {
+  need(b != nullptr, "null batch handle");
+  if (b->dtype == HBDIA_DTYPE_F64) return fn(*b->d);
+  return fn(*b->f);
+}
+
+}  // namespace
+
+int hbdia_batch_create(hbdia_batch_handle_t* b, const hbdia_matrix_t* matrices, int count) {
+  return guard("hbdia_batch_create", [&] {
+    need(b != nullptr, "null handle pointer");
+    const int dt = batch_dtype(matrices, count);
+    auto h = std::make_unique<hbdia_batch_opaque>();
+    h->dtype = dt;
+    if (dt == HBDIA_DTYPE_F64) {
+      std::vector<const HBDIA<double>*> v;
+      for (int i = 0; i < count; ++i) v.push_back(matrices[i]->d.get());
+      h->d = std::make_unique<HBDIABatch<double>>(v);
+    } else {
+      std::vector<const HBDIA<float>*> v;
+      for (int i = 0; i < count; ++i) v.push_back(matrices[i]->f.get());
+      h->f = std::make_unique<HBDIABatch<float>>(v);
+    }
+    *b = h.release();
+  });
+}
+
+int hbdia_batch_autotune(hbdia_batch_handle_t* b, const hbdia_matrix_t* matrices, int count,
+                         const int* block_widths, int num_widths, int threshold, int reps, double* table,
+                         int table_capacity, int* table_rows) {
+  return guard("hbdia_batch_autotune", [&] {
+    need(b != nullptr, "null handle pointer");
+    need(block_widths != nullptr && num_widths > 0, "need block widths");
+    const int dt = batch_dtype(matrices, count);
+    const std::vector<int> bws(block_widths, block_widths + num_widths);
+    std::vector<BatchTuneResult> rows;
+    auto h = std::make_unique<hbdia_batch_opaque>();
+    h->dtype = dt;
+    if (dt == HBDIA_DTYPE_F64) {
+      std::vector<HBDIA<double>*> v;
+      for (int i = 0; i < count; ++i) v.push_back(matrices[i]->d.get());
+      h->d = autotuneBatch<double>(v, bws, threshold, reps, &rows);
+    } else {
+      std::vector<HBDIA<float>*> v;
+      for (int i = 0; i < count; ++i) v.push_back(matrices[i]->f.get());
+      h->f = autotuneBatch<float>(v, bws, threshold, reps, &rows);
+    }
+    if (table)
+      for (int i = 0; i < std::min<int>(table_capacity, static_cast<int>(rows.size())); ++i) {
+        table[5 * i + 0] = rows[i].blockWidth;
+        table[5 * i + 1] = rows[i].schedule.kind;
+        table[5 * i + 2] = rows[i].schedule.tilesPerStep;
+        table[5 * i + 3] = rows[i].schedule.workgroups;
+        table[5 * i + 4] = rows[i].ms;
+      }
+    if (table_rows) *table_rows = static_cast<int>(rows.size());
+    *b = h.release();
+  });
+}
+
+int hbdia_batch_destroy(hbdia_batch_handle_t b) {
+  return guard("hbdia_batch_destroy", [&] { delete b; });
+}
+
+int hbdia_batch_info(hbdia_batch_handle_t b, int64_t* info) {
+  return guard("hbdia_batch_info", [&] {
+    need(info != nullptr, "null info");
+    with_batch(b, [&](auto& B) {
+      info[HBDIA_BATCH_INFO_MATRICES] = B.numMatrices();
+      info[HBDIA_BATCH_INFO_ROWS] = B.totalRows();
+      info[HBDIA_BATCH_INFO_TILES] = B.numTiles();
+      info[HBDIA_BATCH_INFO_BDIA_NNZ] = B.bdiaNnz();
+      info[HBDIA_BATCH_INFO_COO_NNZ] = B.cooNnz();
+      info[HBDIA_BATCH_INFO_ALG_BYTES] = B.algorithmicBytes();
+      info[HBDIA_BATCH_INFO_STORED_BYTES] = B.storedBytes();
+      info[HBDIA_BATCH_INFO_FLOPS] = B.flops();
+      info[HBDIA_BATCH_INFO_KIND] = B.schedule().kind;
+      info[HBDIA_BATCH_INFO_TILES_PER_STEP] = B.schedule().tilesPerStep;
+      info[HBDIA_BATCH_INFO_WORKGROUPS] = B.schedule().workgroups;
+      info[HBDIA_BATCH_INFO_DTYPE] = b->dtype;
+      return 0;
+    });
+  });
+}
+
+int hbdia_batch_row_offsets(hbdia_batch_handle_t b, int64_t* offsets) {
+  return guard("hbdia_batch_row_offsets", [&] {
+    need(offsets != nullptr, "null output");
+    with_batch(b, [&](auto& B) {
+      for (int m = 0; m < B.numMatrices(); ++m) offsets[m] = B.rowOffset(m);
+      offsets[B.numMatrices()] = B.totalRows();
+      return 0;
+    });
+  });
+}
+
+int hbdia_batch_set_schedule(hbdia_batch_handle_t b, int kind, int tiles_per_step, int workgroups) {
+  return guard("hbdia_batch_set_schedule", [&] {
+    with_batch(b, [&](auto& B) {
+      B.setSchedule(BatchSchedule{kind, tiles_per_step, workgroups});
+      return 0;
+    });
+  });
+}
+
+int hbdia_batch_resident_workgroups(hbdia_batch_handle_t b, int kind, int* workgroups) {
+  return guard("hbdia_batch_resident_workgroups", [&] {
+    need(workgroups != nullptr, "null output");
+    with_batch(b, [&](auto& B) {
+      *workgroups = B.residentWorkgroups(kind);
+      return 0;
+    });
+  });
+}
+
+int hbdia_batch_spmv(hbdia_batch_handle_t b, const void* x, void* y) {
+  return guard("hbdia_batch_spmv", [&] {
+    need(b != nullptr, "null batch handle");
+    if (b->dtype == HBDIA_DTYPE_F64) b->d->spmv(static_cast<const double*>(x), static_cast<double*>(y));
+    else b->f->spmv(static_cast<const float*>(x), static_cast<float*>(y));
+  });
+}
+
+int hbdia_batch_spmv_per_matrix(hbdia_batch_handle_t b, const void* x, void* y) {
+  return guard("hbdia_batch_spmv_per_matrix", [&] {
+    need(b != nullptr, "null batch handle");
+    if (b->dtype == HBDIA_DTYPE_F64) b->d->spmvPerMatrix(static_cast<const double*>(x), static_cast<double*>(y));
+    else b->f->spmvPerMatrix(static_cast<const float*>(x), static_cast<float*>(y));
+  });
+}
+
+int hbdia_batch_capture_graph(hbdia_batch_handle_t b, const void* x, void* y) {
+  return guard("hbdia_batch_capture_graph", [&] {
+    need(b != nullptr, "null batch handle");
+    if (b->dtype == HBDIA_DTYPE_F64) b->d->captureGraph(static_cast<const double*>(x), static_cast<double*>(y));
+    else b->f->captureGraph(static_cast<const float*>(x), static_cast<float*>(y));
+  });
+}
+
+int hbdia_batch_launch_graph(hbdia_batch_handle_t b) {
+  return guard("hbdia_batch_launch_graph", [&] {
+    with_batch(b, [&](auto& B) {
+      B.launchGraph();
+      return 0;
+    });
+  });
+}
+
+int hbdia_batch_synchronize(hbdia_batch_handle_t b) {
+  return guard("hbdia_batch_synchronize", [&] {
+    with_batch(b, [&](auto& B) {
+      B.synchronize();
+      return 0;
+    });
+  });
+}
+
+int hbdia_batch_stream(hbdia_batch_handle_t b, void** stream) {
+  return guard("hbdia_batch_stream", [&] {
+    need(stream != nullptr, "null output");
+    with_batch(b, [&](auto& B) {
+      *stream = static_cast<void*>(B.stream());
+      return 0;
+    });
+  });
+}
+
+int hbdia_gen_banded_batch(hbdia_matrix_t* out, int dtype, int count, int min_rows, int max_rows,
+                           int min_offsets, int max_offsets, int max_offset, double offband, uint64_t seed) {
+  return guard("hbdia_gen_banded_batch", [&] {
+    need(out != nullptr && count >= 0, "bad output or count");
+    need(dtype == HBDIA_DTYPE_F64 || dtype == HBDIA_DTYPE_F32, "dtype must be F64 or F32");
+    hbdia::BandedBatchSpec spec;
+    spec.count = count;
+    spec.minRows = min_rows;
+    spec.maxRows = max_rows;
+    spec.minOffsets = min_offsets;
+    spec.maxOffsets = max_offsets;
+    spec.maxOffset = max_offset;
+    spec.offband = offband;
+    spec.seed = seed;
+    std::vector<std::unique_ptr<hbdia_matrix_opaque>> hs(static_cast<size_t>(count));
+    auto fill = [&](auto tag) {
+      using T = decltype(tag);
+      std::vector<hbdia::CooMatrix<T>> gens;
+      hbdia::generateBandedBatch<T>(spec, gens);
+      for (int i = 0; i < count; ++i) {
+        hs[i] = std::make_unique<hbdia_matrix_opaque>();
+        hs[i]->dtype = dtype;
+        member<T>(hs[i].get()) = from_gen(gens[i]);
+      }
+    };
+    if (dtype == HBDIA_DTYPE_F64) fill(double{});
+    else fill(float{});
+    for (int i = 0; i < count; ++i) out[i] = hs[i].release();
+  });
+}
+
+int hbdia_banded_batch_shape(int i, int min_rows, int max_rows, int min_offsets, int max_offsets, uint64_t seed,
+                             int32_t* rows, int32_t* noffsets, uint64_t* matrix_seed) {
+  return guard("hbdia_banded_batch_shape", [&] {
+    need(rows && noffsets && matrix_seed, "null output");
+    hbdia::BandedBatchSpec spec;
+    spec.minRows = min_rows;
+    spec.maxRows = max_rows;
+    spec.minOffsets = min_offsets;
+    spec.maxOffsets = max_offsets;
+    spec.seed = seed;
+    const hbdia::BandedShape s = hbdia::bandedBatchShape(spec, i);
+    *rows = s.rows;
+    *noffsets = s.noffsets;
+    *matrix_seed = s.seed;
+  });
+}

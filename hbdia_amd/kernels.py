@@ -12,6 +12,7 @@ from ._native import check_hip, hip
 from .device import DeviceArray, Stream
 
 ROW_GROUP = 64
+OFFSETS_PAD = 32  # HBDIA_OFFSETS_PAD of include/hbdia_hip.h
 
 
 def _sfx(dtype) -> str:
@@ -92,7 +93,9 @@ class DeviceLayout:
         self.num_blocks = int(len(block_start) - 1)
         self.dtype = np.dtype(values.dtype)
         self.values = DeviceArray.from_host(values if values.size else np.zeros(1, self.dtype))
-        self.offsets = DeviceArray.from_host((offsets if offsets.size else np.zeros(1, np.int32)).astype(np.int32))
+        # blocked kernels read HBDIA_OFFSETS_PAD entries past the last offset
+        self.offsets = DeviceArray.from_host(np.concatenate([np.asarray(offsets, np.int32),
+                                                             np.zeros(OFFSETS_PAD, np.int32)]))
         self.block_start = DeviceArray.from_host(np.asarray(block_start, dtype=np.int32))
         self.coo_nnz = 0 if coo_rows is None else int(len(coo_rows))
         if self.coo_nnz:

@@ -16,8 +16,9 @@
 // banded (many independent matrices):
 //   n rows, `noffsets` distinct diagonal offsets drawn from [-maxOffset,
 //   maxOffset] always including 0, values U[-1,1) off the main diagonal,
-//   diagonal = 1 + sum |off| of the row; plus floor(offband * n) entries at
-//   uniform columns off the band.
+//   diagonal = 1 + sum |off| of the row; plus floor(offband * band entries)
+//   entries at uniform rows and uniform columns off the band (values U[-1,1),
+//   counted in the diagonal's sum too).
 #ifndef HBDIA_WORKLOADS_HPP
 #define HBDIA_WORKLOADS_HPP
 
@@ -38,6 +39,34 @@ void generateStencilScatter(int n, int points, double scatter, uint64_t seed, Co
 
 template <typename T>
 void generateBanded(int n, int noffsets, int maxOffset, double offband, uint64_t seed, CooMatrix<T>& out);
+
+// A set of independent banded matrices (the many-matrix batch): matrix i has
+// n_i = floor(exp(ln minRows + U (ln maxRows - ln minRows))) rows (log-uniform),
+// noffsets_i uniform in [minOffsets, maxOffsets], and its own seed, all drawn
+// from (seed, i); each is generateBanded(n_i, noffsets_i, maxOffset, offband,
+// seed_i).
+struct BandedBatchSpec {
+  int count = 2048;
+  int minRows = 1000;
+  int maxRows = 100000;
+  int minOffsets = 3;
+  int maxOffsets = 9;
+  int maxOffset = 64;
+  double offband = 0.01;
+  uint64_t seed = 0xC4B;
+};
+
+struct BandedShape {
+  int rows;
+  int noffsets;
+  uint64_t seed;
+};
+
+BandedShape bandedBatchShape(const BandedBatchSpec& spec, int i);
+
+// Generates every matrix of the set (in parallel over matrices).
+template <typename T>
+void generateBandedBatch(const BandedBatchSpec& spec, std::vector<CooMatrix<T>>& out);
 
 // v[i] = lo + (hi - lo) * U[0,1)
 template <typename T>

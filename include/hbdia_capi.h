@@ -18,6 +18,7 @@
  *   hbdia_spmv                         -> hbdiaSpMV<T>, include/Operations/HBDIASpMV.hpp:8-10
  *   hbdia_spmv_multi                   -> no reference counterpart: many right-hand sides in one launch
  *   hbdia_mixed_*                      -> no reference counterpart: FP32 / BF16 value storage
+ *   hbdia_batch_*                      -> no reference counterpart: many matrices in one launch
  *   hbdia_gen_*                        -> no reference counterpart: seeded benchmark inputs
  */
 #ifndef HBDIA_CAPI_H
@@ -110,11 +111,61 @@ int hbdia_mixed_spmv(hbdia_mixed_t h, hbdia_vector_t x, hbdia_vector_t y);
 int hbdia_mixed_spmv_raw(hbdia_mixed_t h, const void* x, void* y);
 int hbdia_mixed_fp64_rows(hbdia_mixed_t h, int64_t* rows);
 
+/* Many matrices in one persistent launch (HBDIABatch and autotuneBatch,
+ * include/Operations/HBDIABatch.hpp).  The matrices share one dtype, are
+ * square and converted with block widths that are multiples of 64; the batch
+ * copies their arrays, so they may be destroyed afterwards.  x and y are
+ * device pools of info[HBDIA_BATCH_INFO_ROWS] values; matrix m's rows start
+ * at row offset m.  Launches are asynchronous on the batch's stream. */
+typedef struct hbdia_batch_opaque* hbdia_batch_handle_t;
+enum {
+  HBDIA_BATCH_INFO_MATRICES = 0,
+  HBDIA_BATCH_INFO_ROWS = 1,
+  HBDIA_BATCH_INFO_TILES = 2,
+  HBDIA_BATCH_INFO_BDIA_NNZ = 3,
+  HBDIA_BATCH_INFO_COO_NNZ = 4,
+  HBDIA_BATCH_INFO_ALG_BYTES = 5,
+  HBDIA_BATCH_INFO_STORED_BYTES = 6,
+  HBDIA_BATCH_INFO_FLOPS = 7,
+  HBDIA_BATCH_INFO_KIND = 8,
+  HBDIA_BATCH_INFO_TILES_PER_STEP = 9,
+  HBDIA_BATCH_INFO_WORKGROUPS = 10,
+  HBDIA_BATCH_INFO_DTYPE = 11,
+  HBDIA_BATCH_INFO_COUNT = 12
+};
+int hbdia_batch_create(hbdia_batch_handle_t* b, const hbdia_matrix_t* matrices, int count);
+/* Tries every block width in block_widths (host conversion with `threshold`,
+ * left at the chosen width) and every tile schedule, `reps` timed launches
+ * each; returns the fastest batch.  table (may be NULL) receives up to
+ * table_capacity rows of 5 doubles (block_width, kind, tiles_per_step,
+ * workgroups, ms per launch); *table_rows the number of candidates timed. */
+int hbdia_batch_autotune(hbdia_batch_handle_t* b, const hbdia_matrix_t* matrices, int count,
+                         const int* block_widths, int num_widths, int threshold, int reps, double* table,
+                         int table_capacity, int* table_rows);
+int hbdia_batch_destroy(hbdia_batch_handle_t b);
+int hbdia_batch_info(hbdia_batch_handle_t b, int64_t* info); /* HBDIA_BATCH_INFO_COUNT entries */
+int hbdia_batch_row_offsets(hbdia_batch_handle_t b, int64_t* offsets); /* matrices + 1 entries */
+int hbdia_batch_set_schedule(hbdia_batch_handle_t b, int kind, int tiles_per_step, int workgroups);
+int hbdia_batch_resident_workgroups(hbdia_batch_handle_t b, int kind, int* workgroups);
+int hbdia_batch_spmv(hbdia_batch_handle_t b, const void* x, void* y);
+int hbdia_batch_spmv_per_matrix(hbdia_batch_handle_t b, const void* x, void* y);
+int hbdia_batch_capture_graph(hbdia_batch_handle_t b, const void* x, void* y);
+int hbdia_batch_launch_graph(hbdia_batch_handle_t b);
+int hbdia_batch_synchronize(hbdia_batch_handle_t b);
+int hbdia_batch_stream(hbdia_batch_handle_t b, void** stream); /* its hbdia_stream_t */
+
 /* Seeded benchmark inputs (generators in Workloads.hpp). */
 int hbdia_gen_stencil(hbdia_matrix_t* m, int dtype, int n, int points, double scatter, uint64_t seed);
 int hbdia_gen_banded(hbdia_matrix_t* m, int dtype, int n, int noffsets, int max_offset, double offband,
                      uint64_t seed);
 int hbdia_gen_vector(int dtype, int64_t n, uint64_t seed, double lo, double hi, void* out);
+/* The banded matrix set of BandedBatchSpec: out receives `count` new handles;
+ * matrix i is hbdia_gen_banded(rows_i, noffsets_i, max_offset, offband,
+ * seed_i) with the shape hbdia_banded_batch_shape reports. */
+int hbdia_gen_banded_batch(hbdia_matrix_t* out, int dtype, int count, int min_rows, int max_rows,
+                           int min_offsets, int max_offsets, int max_offset, double offband, uint64_t seed);
+int hbdia_banded_batch_shape(int i, int min_rows, int max_rows, int min_offsets, int max_offsets, uint64_t seed,
+                             int32_t* rows, int32_t* noffsets, uint64_t* matrix_seed);
 
 #ifdef __cplusplus
 }

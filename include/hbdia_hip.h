@@ -30,6 +30,11 @@ extern "C" {
 
 /* Rows handled by one 64-lane wavefront; COO group pointers are per group. */
 #define HBDIA_ROW_GROUP 64
+/* Every `offsets` array passed to a blocked kernel must stay readable for
+ * HBDIA_OFFSETS_PAD int32 past its last offset (block_start[num_blocks]); the
+ * kernels read a chunk's offsets as one wide load and ignore what lies past a
+ * block's count.  HBDIA<T>::prepareForGPU allocates this padding. */
+#define HBDIA_OFFSETS_PAD 32
 
 typedef struct hbdia_stream_opaque* hbdia_stream_t;
 typedef struct hbdia_event_opaque* hbdia_event_t;
@@ -51,6 +56,7 @@ int hbdia_get_device(int* device);
 int hbdia_device_synchronize(void);
 int hbdia_mem_info(size_t* free_bytes, size_t* total_bytes);
 int hbdia_device_name(char* buf, size_t len);
+int hbdia_device_compute_units(int* count); /* compute units of the current device */
 
 /* ---- memory (HBDIA.cpp:768-814, HBDIAVector.cpp:84-153) ------------------ */
 int hbdia_malloc(void** ptr, size_t bytes);
@@ -158,6 +164,96 @@ int hbdia_hbdia_spmv_mixed(int vdtype, const void* values_lp, const double* valu
                            const int32_t* coo_rows, const int32_t* coo_cols, const void* coo_values_lp,
                            const double* coo_values_f64, const int32_t* coo_group_ptr, const double* x,
                            double* y, int64_t num_rows, double ratio, void* workspace, hbdia_stream_t stream);
+
+/* Many independent matrices in one persistent launch (no reference
+ * counterpart: the reference multiplies one matrix per call).  The matrices'
+ * HBDIA arrays are concatenated into pools (values, offsets, fallback rows /
+ * columns / values); each matrix keeps the single-matrix layout inside its part
+ * (fallback rows and columns relative to the matrix), and its rows occupy a
+ * contiguous range of the x and y pools.  The work is cut into tiles of 64
+ * consecutive rows of one matrix, each described by a tile record (built on
+ * the host by hbdia_batch_make_tiles), and every matrix's rows come out bit for
+ * bit as hbdia_hbdia_spmv_* (matrix with a fallback) or hbdia_bdia_spmv_*
+ * (without) would give them for that matrix alone.  The grid is persistent:
+ * num_workgroups workgroups of 4 waves, each wave taking tiles
+ *   HBDIA_BATCH_INTERLEAVED: runs of tiles_per_step consecutive tiles, wave w
+ *     taking runs w, w + W, w + 2W, ... (W = 4 num_workgroups);
+ *   HBDIA_BATCH_CONTIGUOUS: tiles [wave_tiles[w], wave_tiles[w+1]), a split
+ *     that the host balances by work (wave_tiles has W + 1 entries).
+ * A tile record carries its block's first offsets, so a wave needs one
+ * scalar load per tile, issued for the next tile while the current tile's
+ * loads are in flight.  Pools need padding past their last entry: offsets
+ * HBDIA_OFFSETS_PAD entries, fallback rows / columns / values
+ * HBDIA_BATCH_COO_PAD entries (any values; columns must be 0), values 64. */
+#define HBDIA_BATCH_INTERLEAVED 0
+#define HBDIA_BATCH_CONTIGUOUS 1
+#define HBDIA_BATCH_COO_PAD 64
+
+/* One matrix of a batch, as placed in the pools (host-side description). */
+typedef struct hbdia_batch_desc {
+  int64_t values;      /* first value of the matrix in the values pool */
+  int64_t rows;        /* first row of the matrix in the x and y pools */
+  int64_t first_tile;  /* index of its first tile */
+  int32_t offsets;     /* first of its flattened offsets in the offsets pool */
+  int32_t block_start; /* first of its num_blocks + 1 block starts (relative to its offsets) */
+  int32_t coo;         /* first of its fallback entries in the fallback pools */
+  int32_t coo_nnz;     /* its fallback entries (0: blocked part only) */
+  int32_t group_ptr;   /* first of its ceil(num_rows/64) + 1 group pointers (relative to its fallback) */
+  int32_t num_rows;
+  int32_t num_blocks;
+  int32_t block_width; /* multiple of 64 */
+} hbdia_batch_desc_t;
+
+/* One tile: 64 rows of one matrix (80 bytes, one scalar load). */
+#define HBDIA_BATCH_INLINE_OFFSETS 10
+typedef struct hbdia_batch_tile {
+  int32_t value_line; /* values-pool index / 64 of the tile's first row in its block's first offset line */
+  int32_t rows;       /* first row of the tile's matrix in the x and y pools */
+  int32_t row0;       /* first row of the tile within its matrix (a multiple of 64) */
+  int32_t num_rows;   /* rows of the matrix */
+  int32_t stride;     /* block width: distance between offset lines, in values */
+  int32_t count;      /* offsets of the tile's block (0: empty block) */
+  int32_t offsets;    /* offsets-pool index of the block's first offset */
+  int32_t coo_begin;  /* fallback-pool range [coo_begin, coo_end) of the tile's rows */
+  int32_t coo_end;
+  int32_t fused;      /* 1: every row gets (blocked sum or old y) + fallback; 0: blocked part only */
+  int32_t first_offsets[HBDIA_BATCH_INLINE_OFFSETS]; /* the block's first offsets; 0 past count */
+} hbdia_batch_tile_t;
+
+typedef struct hbdia_batch {
+  const hbdia_batch_tile_t* tiles; /* device, num_tiles records */
+  const int32_t* wave_tiles;       /* device, contiguous schedule only (else NULL) */
+  const void* values;              /* device pools (padded as above) */
+  const int32_t* offsets;
+  const int32_t* coo_rows;
+  const int32_t* coo_cols;
+  const void* coo_values;
+  int64_t num_tiles;
+  int32_t has_coo; /* any tile with fused == 1 */
+  int32_t x32;     /* every matrix: num_rows * sizeof(value) < 2^32 */
+} hbdia_batch_t;
+
+typedef struct hbdia_batch_schedule {
+  int32_t kind;           /* HBDIA_BATCH_INTERLEAVED or HBDIA_BATCH_CONTIGUOUS */
+  int32_t tiles_per_step; /* interleaved: tiles per run, >= 1 */
+  int32_t num_workgroups; /* persistent grid size, >= 1 */
+} hbdia_batch_schedule_t;
+
+/* Host function: fills tiles[desc[m].first_tile + g] for every matrix m and
+ * 64-row group g from the host copies of the offsets, block-start and
+ * group-pointer pools.  value_size: 4 or 8.  Fails when a block's values
+ * exceed 2^32 bytes or a pool index exceeds 2^31. */
+int hbdia_batch_make_tiles(const hbdia_batch_desc_t* desc, int32_t num_matrices, const int32_t* offsets,
+                           const int32_t* block_start, const int32_t* group_ptr, int32_t value_size,
+                           hbdia_batch_tile_t* tiles, int64_t num_tiles);
+int hbdia_batch_spmv_f64(const hbdia_batch_t* batch, const hbdia_batch_schedule_t* schedule, const double* x,
+                         double* y, hbdia_stream_t stream);
+int hbdia_batch_spmv_f32(const hbdia_batch_t* batch, const hbdia_batch_schedule_t* schedule, const float* x,
+                         float* y, hbdia_stream_t stream);
+/* Workgroups of the batch kernel that one compute unit holds at once (the
+ * kernel variant that `batch`, `kind` and value_size (4 or 8) select); a grid
+ * of at most this times the compute units is resident as a whole. */
+int hbdia_batch_occupancy(const hbdia_batch_t* batch, int32_t kind, int32_t value_size, int* workgroups_per_cu);
 
 /* COO fallback alone, into a result buffer: res[r] = sequential FMA sum of the
  * row's entries for rows that have entries; other rows keep res.  The GPU

@@ -1,0 +1,161 @@
+"""GPU tests of the many-matrix batch (one persistent launch over pooled
+HBDIA arrays).
+
+Each matrix's slice of y must equal, bit for bit, the single-matrix
+hbdiaSpMV(A, x, y, False, True) on that matrix (same kernel arithmetic) and the
+oracle's hbdia_spmv; this must hold for every tile schedule, for graph replays
+and for the batch the autotuner returns.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from hbdia_amd import HBDIA, DeviceArray, HBDIABatch, HBDIAVector, gen_vector, hbdiaSpMV
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_y(A: HBDIA, x: np.ndarray, y0: np.ndarray) -> np.ndarray:
+    hb = A.getHBDIA()
+    hb["num_blocks"] = A.getNumBlocks()
+    hb["block_width"] = A.getBlockWidth()
+    return oracle.hbdia_spmv(hb, x.copy(), y0.copy(), A.getNumRows())
+
+
+def single(A: HBDIA, x: np.ndarray, y0: np.ndarray) -> np.ndarray:
+    A.prepareForGPU()
+    X, Y = HBDIAVector(x), HBDIAVector(y0)
+    hbdiaSpMV(A, X, Y, False, True)
+    A.synchronize()
+    return Y.download()
+
+
+def empty_block_matrices(dtype):
+    """Matrices with empty blocks, with and without a fallback part."""
+    # entries only in the first 64 rows: blocks 1.. are empty, no fallback
+    r = np.repeat(np.arange(64, dtype=np.int32), 2)
+    c = np.stack([np.arange(64), (np.arange(64) + 1) % 300]).T.ravel().astype(np.int32)
+    A = HBDIA(r, c, np.linspace(1.0, 2.0, len(r)).astype(dtype), 300, 300)
+    # the same plus lone far entries in empty blocks: those rows get old y + fallback
+    r2 = np.concatenate([r, np.array([70, 200, 299], np.int32)])
+    c2 = np.concatenate([c, np.array([5, 17, 0], np.int32)])
+    B = HBDIA(r2, c2, np.linspace(-1.0, 3.0, len(r2)).astype(dtype), 300, 300)
+    return [A, B]
+
+
+def make_set(dtype, widths):
+    ms = HBDIA.banded_batch(count=24, min_rows=40, max_rows=5000, seed=77, dtype=dtype)
+    ms += empty_block_matrices(dtype)
+    for i, m in enumerate(ms):
+        m.convertToHBDIAFormatOnHost(widths[i % len(widths)], 2)
+    return ms
+
+
+def pools(ms, dtype, seed=3):
+    xs = [gen_vector(m.getNumRows(), seed + i, dtype=dtype) for i, m in enumerate(ms)]
+    y0s = [np.full(m.getNumRows(), 0.5 + i, dtype) for i, m in enumerate(ms)]
+    return xs, y0s
+
+
+def run_batch(batch, xs, y0s, graph=False):
+    X = DeviceArray.from_host(np.concatenate(xs))
+    Y = DeviceArray.from_host(np.concatenate(y0s))
+    if graph:
+        batch.capture_graph(X, Y)
+        batch.launch_graph()
+    else:
+        batch.spmv(X, Y)
+    batch.synchronize()
+    return Y.download()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_equals_single_matrix_and_oracle(gpu, dtype):
+    print(f"case: batch parity {np.dtype(dtype).name}", flush=True)
+    ms = make_set(dtype, [64, 128, 256])
+    xs, y0s = pools(ms, dtype)
+    B = HBDIABatch(ms)
+    info = B.info()
+    assert info["matrices"] == len(ms) and info["rows"] == sum(m.getNumRows() for m in ms)
+    assert info["tiles"] == sum((m.getNumRows() + 63) // 64 for m in ms)
+    y = run_batch(B, xs, y0s)
+    off = B.row_offsets
+    for i, m in enumerate(ms):
+        got = y[off[i]:off[i + 1]]
+        np.testing.assert_array_equal(got, oracle_y(m, xs[i], y0s[i]), err_msg=f"matrix {i} vs oracle")
+        np.testing.assert_array_equal(got, single(m, xs[i], y0s[i]), err_msg=f"matrix {i} vs hbdiaSpMV")
+
+
+@pytest.mark.parametrize("kind,tps,wg", [(0, 1, 0), (0, 3, 0), (0, 8, 5), (0, 1, 1), (1, 1, 0), (1, 1, 3),
+                                         (1, 1, 1)])
+def test_every_schedule_gives_the_same_result(gpu, kind, tps, wg):
+    print(f"case: batch schedule kind={kind} tps={tps} wg={wg}", flush=True)
+    ms = make_set(np.float64, [64, 192])
+    xs, y0s = pools(ms, np.float64, seed=11)
+    B = HBDIABatch(ms)
+    ref = run_batch(B, xs, y0s)
+    B.set_schedule(kind, tps, wg)
+    assert B.info()["kind"] == kind and B.info()["tiles_per_step"] == tps
+    if wg:
+        assert B.info()["workgroups"] == wg
+    np.testing.assert_array_equal(run_batch(B, xs, y0s), ref)
+    # the per-matrix launches agree too
+    X = DeviceArray.from_host(np.concatenate(xs))
+    Y = DeviceArray.from_host(np.concatenate(y0s))
+    B.spmv_per_matrix(X, Y)
+    B.synchronize()
+    np.testing.assert_array_equal(Y.download(), ref)
+
+
+def test_graph_replays_match_eager(gpu):
+    print("case: batch graph replay", flush=True)
+    ms = make_set(np.float64, [64])
+    xs, y0s = pools(ms, np.float64, seed=5)
+    B = HBDIABatch(ms)
+    eager = run_batch(B, xs, y0s)
+    X = DeviceArray.from_host(np.concatenate(xs))
+    Y = DeviceArray.from_host(np.concatenate(y0s))
+    B.capture_graph(X, Y)
+    for rep in range(3):
+        Y.upload(np.concatenate(y0s))
+        B.launch_graph()
+        B.synchronize()
+        np.testing.assert_array_equal(Y.download(), eager, err_msg=f"replay {rep}")
+        B.spmv(X, Y)  # eager work between replays
+        B.synchronize()
+
+
+def test_autotune_picks_a_candidate_and_stays_exact(gpu):
+    print("case: batch autotune", flush=True)
+    ms = HBDIA.banded_batch(count=16, min_rows=500, max_rows=8000, seed=5)
+    B = HBDIABatch.autotune(ms, block_widths=(64, 128), threshold=2, reps=3)
+    assert len(B.table) >= 2 * 5
+    assert {t["block_width"] for t in B.table} == {64, 128}
+    best = min(B.table, key=lambda t: t["ms"])
+    info = B.info()
+    assert (info["kind"], info["tiles_per_step"], info["workgroups"]) == (best["kind"], best["tiles_per_step"],
+                                                                         best["workgroups"])
+    assert all(m.getBlockWidth() == best["block_width"] for m in ms)
+    xs, y0s = pools(ms, np.float64, seed=9)
+    y = run_batch(B, xs, y0s, graph=True)
+    for i, m in enumerate(ms):
+        np.testing.assert_array_equal(y[B.row_offsets[i]:B.row_offsets[i + 1]], oracle_y(m, xs[i], y0s[i]))
+
+
+def test_full_size_set_against_per_matrix_launches(gpu):
+    # 2048 matrices (rows log-uniform in [1e3, 1e4]): one launch equals 2048 single launches
+    print("case: batch 2048 matrices", flush=True)
+    ms = HBDIA.banded_batch(count=2048, max_rows=10000, seed=0xC4B)
+    for m in ms:
+        m.convertToHBDIAFormatOnHost(64, 2)
+    B = HBDIABatch(ms)
+    n = B.rows
+    X = DeviceArray.from_host(gen_vector(n, 1))
+    Y1 = DeviceArray(n, np.float64, zero=True)
+    Y2 = DeviceArray(n, np.float64, zero=True)
+    B.spmv(X, Y1)
+    B.spmv_per_matrix(X, Y2)
+    B.synchronize()
+    np.testing.assert_array_equal(Y1.download(), Y2.download())

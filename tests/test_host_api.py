@@ -134,6 +134,50 @@ def test_seven_point_and_banded_generators():
     assert np.all(np.diff(r) >= 0)
 
 
+def test_banded_generator_offband_count_and_batch_shapes():
+    # floor(offband * band entries) extra entries, each off the band, none duplicated
+    n, k = 3000, 7
+    B = HBDIA.banded(n, k, 64, 0.01, seed=12)
+    r, c, v = B.getCOO()
+    d = c - r
+    offs = np.unique(d[np.abs(d) <= 64])
+    band_cols = {int(o) for o in offs}
+    # offsets of the band are those present on (almost) every row
+    counts = {int(o): int((d == o).sum()) for o in offs}
+    band = sorted(o for o, cnt in counts.items() if cnt >= n - abs(o) - 1)
+    assert len(band) == k and 0 in band
+    band_nnz = sum(n - abs(o) for o in band)
+    extra = len(v) - band_nnz
+    assert extra == int(0.01 * band_nnz)
+    keys = r.astype(np.int64) * n + c
+    assert len(np.unique(keys)) == len(keys)
+    # diagonal dominance: diag = 1 + sum |off-diagonal| of the row
+    diag = v[d == 0]
+    off_abs = np.bincount(r[d != 0], weights=np.abs(v[d != 0]), minlength=n)
+    np.testing.assert_allclose(diag, 1.0 + off_abs, rtol=1e-12)
+    del band_cols
+    # the batch set: matrix i is banded(shape_i); shapes within the spec
+    ms = HBDIA.banded_batch(count=6, min_rows=1000, max_rows=20000, seed=99)
+    for i, m in enumerate(ms):
+        rows, nk, sd = HBDIA.banded_batch_shape(i, 1000, 20000, seed=99)
+        assert 1000 <= rows <= 20000 and 3 <= nk <= 9
+        ref = HBDIA.banded(rows, nk, 64, 0.01, seed=sd)
+        for a, b in zip(m.getCOO(), ref.getCOO()):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_batch_rejects_unconverted_and_narrow_blocks():
+    from hbdia_amd import HBDIABatch, HBDIAError
+    A = HBDIA.banded(500, 3, 8, 0.0, seed=1)
+    with pytest.raises(HBDIAError, match="not converted"):
+        HBDIABatch([A])
+    A.convertToHBDIAFormatOnHost(32, 2)
+    with pytest.raises(HBDIAError, match="multiples of 64"):
+        HBDIABatch([A])
+    with pytest.raises(ValueError):
+        HBDIABatch([])
+
+
 def test_generator_thread_count_independent():
     from hbdia_amd import set_num_threads
     set_num_threads(1)
