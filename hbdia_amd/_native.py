@@ -58,6 +58,7 @@ _HIP_SIGS = {
     "hbdia_stream_destroy": (_i, [_p]),
     "hbdia_stream_synchronize": (_i, [_p]),
     "hbdia_event_create": (_i, [POINTER(c_void_p)]),
+    "hbdia_event_query": (_i, [_p, POINTER(c_int)]),
     "hbdia_event_destroy": (_i, [_p]),
     "hbdia_event_record": (_i, [_p, _p]),
     "hbdia_event_synchronize": (_i, [_p]),
@@ -79,6 +80,8 @@ _HIP_SIGS = {
     "hbdia_mixed_workspace_bytes": (c_size_t, [c_int64]),
     "hbdia_batch_make_tiles": (_i, [_p, c_int32, _p, _p, _p, c_int32, _p, c_int64]),
     "hbdia_batch_spmv_f64": (_i, [_p, _p, _p, _p, _p]),
+    "hbdia_serve_spmv_f64": (_i, [_p, _p, c_int32, c_int64, c_int32, c_int32, _p, _p, _p]),
+    "hbdia_serve_spmv_f32": (_i, [_p, _p, c_int32, c_int64, c_int32, c_int32, _p, _p, _p]),
     "hbdia_batch_spmv_f32": (_i, [_p, _p, _p, _p, _p]),
     "hbdia_batch_occupancy": (_i, [_p, c_int32, c_int32, POINTER(c_int)]),
     "hbdia_hbdia_spmv_mixed": (_i, [c_int, _p, _p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, _p, c_int64,
@@ -123,6 +126,22 @@ _CAPI_SIGS = {
     "hbdia_gen_stencil": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_banded": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_vector": (_i, [c_int, c_int64, c_uint64, c_double, c_double, _p]),
+    "hbdia_server_create": (_i, [POINTER(c_void_p), POINTER(c_void_p), c_int, c_int64, c_int, c_int, c_int, c_int,
+                                 c_int]),
+    "hbdia_server_destroy": (_i, [_p]),
+    "hbdia_server_submit": (_i, [_p, c_int, _p, _p, c_int, POINTER(c_int64)]),
+    "hbdia_server_wait": (_i, [_p, c_int64]),
+    "hbdia_server_drain": (_i, [_p]),
+    "hbdia_server_times": (_i, [_p, c_int64, POINTER(c_int64)]),
+    "hbdia_server_info": (_i, [_p, POINTER(c_int64)]),
+    "hbdia_server_num_rows": (_i, [_p, c_int, POINTER(c_int64)]),
+    "hbdia_server_open_loop": (_i, [_p, c_double, c_int64, c_uint64, POINTER(c_void_p), c_int, POINTER(c_void_p),
+                                    c_int, POINTER(c_double), POINTER(c_double)]),
+    "hbdia_arena_create": (_i, [POINTER(c_void_p), c_int64, c_int64]),
+    "hbdia_arena_destroy": (_i, [_p]),
+    "hbdia_arena_allocate": (_i, [_p, c_int64, POINTER(c_int64)]),
+    "hbdia_arena_release": (_i, [_p, c_int64]),
+    "hbdia_arena_info": (_i, [_p, POINTER(c_int64)]),
     "hbdia_gen_banded_batch": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_double,
                                     c_uint64]),
     "hbdia_banded_batch_shape": (_i, [c_int, c_int, c_int, c_int, c_int, c_uint64, POINTER(c_int32),

@@ -459,6 +459,120 @@ class HBDIABatch:
             pass
 
 
+class HBDIAServer:
+    """Continuous-batching SpMV server over resident matrices (include/Operations/HBDIAServe.hpp).
+
+    submit(m, x, y) queues y = A_m x (y starts at zero); x and y are host arrays (kept
+    alive until the request completes) or, with device=True, DeviceArrays.  batching=False
+    gives the plain form: one request at a time through the single-matrix kernel."""
+
+    _INFO = ["matrices", "resident_bytes", "arena_bytes", "launches", "completed", "largest_launch", "dtype"]
+
+    def __init__(self, matrices: list[HBDIA], hbm_budget: int = 0, max_jobs: int = 256, max_in_flight: int = 2,
+                 workgroups: int = 0, tiles_per_step: int = 4, batching: bool = True):
+        if not matrices:
+            raise ValueError("need at least one matrix")
+        arr = (c_void_p * len(matrices))(*[m._h for m in matrices])
+        h = c_void_p(None)
+        check_capi(capi.hbdia_server_create(byref(h), arr, len(matrices), int(hbm_budget), int(max_jobs),
+                                            int(max_in_flight), int(workgroups), int(tiles_per_step),
+                                            int(bool(batching))))
+        self._h = h.value
+        self.dtype = np.dtype(np.float64 if self.info()["dtype"] == F64 else np.float32)
+        self._keep = {}
+
+    def info(self) -> dict:
+        a = np.zeros(len(self._INFO), np.int64)
+        check_capi(capi.hbdia_server_info(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return dict(zip(self._INFO, (int(v) for v in a)))
+
+    def num_rows(self, m: int) -> int:
+        n = ctypes.c_int64(0)
+        check_capi(capi.hbdia_server_num_rows(self._h, int(m), byref(n)))
+        return int(n.value)
+
+    def submit(self, m: int, x, y, device: bool = False) -> int:
+        if device:
+            xp, yp = x.ptr, y.ptr
+        else:
+            if x.dtype != self.dtype or y.dtype != self.dtype or not x.flags.c_contiguous or not y.flags.c_contiguous:
+                raise TypeError(f"x and y must be contiguous {self.dtype} arrays")
+            xp, yp = x.ctypes.data, y.ctypes.data
+        rid = ctypes.c_int64(0)
+        check_capi(capi.hbdia_server_submit(self._h, int(m), xp, yp, int(bool(device)), byref(rid)))
+        self._keep[rid.value] = (x, y)
+        return int(rid.value)
+
+    def wait(self, rid: int) -> None:
+        check_capi(capi.hbdia_server_wait(self._h, int(rid)))
+        self._keep.pop(rid, None)
+
+    def drain(self) -> None:
+        check_capi(capi.hbdia_server_drain(self._h))
+        self._keep.clear()
+
+    def times(self, rid: int) -> tuple[int, int, int]:
+        a = (ctypes.c_int64 * 3)()
+        check_capi(capi.hbdia_server_times(self._h, int(rid), a))
+        return int(a[0]), int(a[1]), int(a[2])
+
+    def open_loop(self, rate: float, requests: int, seed: int, xs: list, variants: int, ys: list,
+                  device: bool = False) -> tuple[np.ndarray, float]:
+        """Poisson arrivals at `rate` per second (0: all at once); returns latencies (us) and wall seconds."""
+        xp = (c_void_p * len(xs))(*[(a.ptr if device else a.ctypes.data) for a in xs])
+        yp = (c_void_p * len(ys))(*[(a.ptr if device else a.ctypes.data) for a in ys])
+        lat = np.zeros(int(requests), np.float64)
+        wall = c_double(0.0)
+        check_capi(capi.hbdia_server_open_loop(self._h, float(rate), int(requests), int(seed), xp, int(variants), yp,
+                                               int(bool(device)),
+                                               lat.ctypes.data_as(ctypes.POINTER(c_double)), byref(wall)))
+        return lat, float(wall.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            check_capi(capi.hbdia_server_destroy(self._h))
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                capi.hbdia_server_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class VectorArena:
+    """The server's first-fit allocator on its own (host only), for tests."""
+
+    def __init__(self, capacity: int, align: int = 64):
+        h = c_void_p(None)
+        check_capi(capi.hbdia_arena_create(byref(h), int(capacity), int(align)))
+        self._h = h.value
+
+    def allocate(self, n: int) -> int:
+        off = ctypes.c_int64(0)
+        check_capi(capi.hbdia_arena_allocate(self._h, int(n), byref(off)))
+        return int(off.value)
+
+    def release(self, offset: int) -> None:
+        check_capi(capi.hbdia_arena_release(self._h, int(offset)))
+
+    def info(self) -> dict:
+        a = (ctypes.c_int64 * 4)()
+        check_capi(capi.hbdia_arena_info(self._h, a))
+        return dict(capacity=int(a[0]), used=int(a[1]), largest_free=int(a[2]), free_blocks=int(a[3]))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                capi.hbdia_arena_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
 def gen_vector(n: int, seed: int, lo: float = -1.0, hi: float = 1.0, dtype=np.float64) -> np.ndarray:
     out = np.empty(int(n), np.dtype(dtype))
     check_capi(capi.hbdia_gen_vector(_code(dtype), int(n), int(seed), float(lo), float(hi), out.ctypes.data))

@@ -448,13 +448,18 @@ constexpr int kTileDwords = 20;  // 80-byte tile record
 static_assert(sizeof(hbdia_batch_tile_t) == kTileDwords * 4, "tile record layout");
 static_assert(HBDIA_BATCH_INLINE_OFFSETS >= kChunk, "a tile record carries the first chunk's offsets");
 
-template <typename T, bool HAS_COO, bool X32, bool CONTIG>
+// JOBS: the work is a list of jobs (serving), each the tiles of one pooled
+// matrix applied to its own x / y slot; the wave's work indices are mapped to
+// (job, tile) by a binary search over the jobs' work_begin.
+template <typename T, bool HAS_COO, bool X32, bool CONTIG, bool JOBS>
 __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__ tiles,
                                                      const int32_t* __restrict__ wave_tiles,
                                                      const T* __restrict__ vals, const int* __restrict__ offs,
                                                      const int* __restrict__ cr, const int* __restrict__ cc,
                                                      const T* __restrict__ cv, int64_t num_tiles, int tiles_per_step,
+                                                     const hbdia_serve_job_t* __restrict__ jobs, int num_jobs,
                                                      const T* __restrict__ xp, T* __restrict__ yp) {
+  static_assert(!(JOBS && CONTIG), "jobs run on the interleaved schedule");
   __shared__ T s_v[kWavesPerWG][kWave];
   __shared__ T s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][kWave];
@@ -483,10 +488,24 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
     if constexpr (CONTIG) tj = first + sj;
     else tj = wave * tiles_per_step + (sj / tiles_per_step) * step + sj % tiles_per_step;
     const bool vj = tj < limit;
-    const int32_t* rp = tiles + (vj ? tj : 0) * kTileDwords + ql * 5;
+    int64_t rec = vj ? tj : 0;
+    int shift = 0;
+    if constexpr (JOBS) {
+      // the last job whose work starts at or before tj
+      int lo = 0, hi = num_jobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].work_begin <= rec) lo = mid;
+        else hi = mid - 1;
+      }
+      rec = vj ? jobs[lo].tile_begin + (tj - jobs[lo].work_begin) : 0;
+      shift = vj ? jobs[lo].row_shift : 0;
+    }
+    const int32_t* rp = tiles + rec * kTileDwords + ql * 5;
     int rv[5];
 #pragma unroll
     for (int r = 0; r < 5; ++r) rv[r] = rp[r];
+    if constexpr (JOBS) rv[1] += ql == 0 ? shift : 0;  // dword 1 (rows) is held by lane 4j
     const int ng = __popcll(__ballot(vj && ql == 0));
     if (ng == 0) break;
     // fields of tile jj of the group, read from the lanes holding its record;
@@ -1057,10 +1076,10 @@ int launch_batch(const hbdia_batch_t* bt, const hbdia_batch_schedule_t* sc, cons
   hipStream_t s = hs(stream);
   const bool coo = bt->has_coo != 0, x32 = bt->x32 != 0, contig = sc->kind == HBDIA_BATCH_CONTIGUOUS;
 #define HBDIA_BATCH_LAUNCH(C, X, G) \
-  hipLaunchKernelGGL((k_hbdia_batch<T, C, X, G>), grid, block, 0, s,                                          \
+  hipLaunchKernelGGL((k_hbdia_batch<T, C, X, G, false>), grid, block, 0, s,                                   \
                      reinterpret_cast<const int32_t*>(bt->tiles), bt->wave_tiles,                           \
                      static_cast<const T*>(bt->values), bt->offsets, bt->coo_rows, bt->coo_cols,            \
-                     static_cast<const T*>(bt->coo_values), bt->num_tiles, sc->tiles_per_step, x, y)
+                     static_cast<const T*>(bt->coo_values), bt->num_tiles, sc->tiles_per_step, nullptr, 0, x, y)
   if (coo) {
     if (x32) {
       if (contig) HBDIA_BATCH_LAUNCH(true, true, true); else HBDIA_BATCH_LAUNCH(true, true, false);
@@ -1079,10 +1098,37 @@ int launch_batch(const hbdia_batch_t* bt, const hbdia_batch_schedule_t* sc, cons
 }
 
 template <typename T>
+int launch_serve(const hbdia_batch_t* bt, const hbdia_serve_job_t* jobs, int32_t num_jobs, int64_t total_tiles,
+                 int32_t num_workgroups, int32_t tiles_per_step, const T* x, T* y, hbdia_stream_t stream,
+                 const char* name) {
+  if (!bt || num_jobs < 0 || total_tiles < 0 || num_workgroups < 1 || tiles_per_step < 1)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": invalid argument");
+  if (num_jobs == 0 || total_tiles == 0) return HBDIA_SUCCESS;
+  if (!jobs || !x || !y || !bt->tiles || !bt->values || !bt->offsets)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
+  if (bt->has_coo && (!bt->coo_rows || !bt->coo_cols || !bt->coo_values))
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null fallback pool");
+  const dim3 grid(static_cast<unsigned>(num_workgroups)), block(kWG);
+  hipStream_t s = hs(stream);
+#define HBDIA_SERVE_LAUNCH(C, X) \
+  hipLaunchKernelGGL((k_hbdia_batch<T, C, X, false, true>), grid, block, 0, s,                                \
+                     reinterpret_cast<const int32_t*>(bt->tiles), nullptr, static_cast<const T*>(bt->values), \
+                     bt->offsets, bt->coo_rows, bt->coo_cols, static_cast<const T*>(bt->coo_values),          \
+                     total_tiles, tiles_per_step, jobs, num_jobs, x, y)
+  if (bt->has_coo) {
+    if (bt->x32) HBDIA_SERVE_LAUNCH(true, true); else HBDIA_SERVE_LAUNCH(true, false);
+  } else {
+    if (bt->x32) HBDIA_SERVE_LAUNCH(false, true); else HBDIA_SERVE_LAUNCH(false, false);
+  }
+#undef HBDIA_SERVE_LAUNCH
+  return check_hip(hipGetLastError(), name);
+}
+
+template <typename T>
 int batch_occupancy(bool coo, bool x32, bool contig, int* out) {
   hipError_t e;
 #define HBDIA_BATCH_OCC(C, X, G) \
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_hbdia_batch<T, C, X, G>, kWG, 0)
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_hbdia_batch<T, C, X, G, false>, kWG, 0)
   if (coo) {
     if (x32) {
       if (contig) HBDIA_BATCH_OCC(true, true, true); else HBDIA_BATCH_OCC(true, true, false);
@@ -1244,6 +1290,20 @@ int hbdia_event_synchronize(hbdia_event_t event) {
   if (!event) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_event_synchronize: null");
   return check_hip(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)), "hipEventSynchronize");
 }
+
+int hbdia_event_query(hbdia_event_t event, int* done) {
+  if (!event || !done) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_event_query: null");
+  const hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(event));
+  if (e == hipSuccess) {
+    *done = 1;
+    return HBDIA_SUCCESS;
+  }
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return HBDIA_SUCCESS;
+  }
+  return check_hip(e, "hipEventQuery");
+}
 int hbdia_stream_wait_event(hbdia_stream_t stream, hbdia_event_t event) {
   if (!event) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_stream_wait_event: null");
   return check_hip(hipStreamWaitEvent(hs(stream), reinterpret_cast<hipEvent_t>(event), 0),
@@ -1392,6 +1452,20 @@ int hbdia_batch_spmv_f64(const hbdia_batch_t* batch, const hbdia_batch_schedule_
 int hbdia_batch_spmv_f32(const hbdia_batch_t* batch, const hbdia_batch_schedule_t* schedule, const float* x,
                          float* y, hbdia_stream_t stream) {
   return launch_batch<float>(batch, schedule, x, y, stream, "hbdia_batch_spmv_f32");
+}
+
+int hbdia_serve_spmv_f64(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t num_jobs,
+                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, const double* x,
+                         double* y, hbdia_stream_t stream) {
+  return launch_serve<double>(pool, jobs, num_jobs, total_tiles, num_workgroups, tiles_per_step, x, y, stream,
+                              "hbdia_serve_spmv_f64");
+}
+
+int hbdia_serve_spmv_f32(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t num_jobs,
+                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, const float* x,
+                         float* y, hbdia_stream_t stream) {
+  return launch_serve<float>(pool, jobs, num_jobs, total_tiles, num_workgroups, tiles_per_step, x, y, stream,
+                             "hbdia_serve_spmv_f32");
 }
 
 int hbdia_batch_occupancy(const hbdia_batch_t* batch, int32_t kind, int32_t value_size, int* workgroups_per_cu) {

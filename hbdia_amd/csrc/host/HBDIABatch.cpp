@@ -268,23 +268,25 @@ void HBDIABatch<T>::spmv(const T* x, T* y) const {
 }
 
 template <typename T>
-void HBDIABatch<T>::spmvPerMatrix(const T* x, T* y) const {
+void HBDIABatch<T>::spmvMatrix(int m, const T* x, T* y, hbdia_stream_t stream) const {
+  const hbdia_batch_desc_t& d = desc_.at(m);
   const T* vals = static_cast<const T*>(pools_[2]);
   const int32_t* offs = static_cast<const int32_t*>(pools_[3]);
   const int32_t* blk = static_cast<const int32_t*>(pools_[4]);
-  const int32_t* cr = static_cast<const int32_t*>(pools_[5]);
-  const int32_t* cc = static_cast<const int32_t*>(pools_[6]);
-  const T* cv = static_cast<const T*>(pools_[7]);
-  const int32_t* gp = static_cast<const int32_t*>(pools_[8]);
-  for (const hbdia_batch_desc_t& d : desc_) {
-    if (d.coo_nnz > 0)
-      CHECK_HBDIA(BatchOps<T>::fused(vals + d.values, offs + d.offsets, blk + d.block_start, d.num_blocks,
-                                     d.block_width, cr + d.coo, cc + d.coo, cv + d.coo, gp + d.group_ptr,
-                                     x + d.rows, y + d.rows, d.num_rows, stream_));
-    else
-      CHECK_HBDIA(BatchOps<T>::bdia(vals + d.values, offs + d.offsets, blk + d.block_start, x + d.rows, y + d.rows,
-                                    d.num_rows, d.num_blocks, d.block_width, stream_));
-  }
+  if (d.coo_nnz > 0)
+    CHECK_HBDIA(BatchOps<T>::fused(vals + d.values, offs + d.offsets, blk + d.block_start, d.num_blocks, d.block_width,
+                                   static_cast<const int32_t*>(pools_[5]) + d.coo,
+                                   static_cast<const int32_t*>(pools_[6]) + d.coo,
+                                   static_cast<const T*>(pools_[7]) + d.coo,
+                                   static_cast<const int32_t*>(pools_[8]) + d.group_ptr, x, y, d.num_rows, stream));
+  else
+    CHECK_HBDIA(BatchOps<T>::bdia(vals + d.values, offs + d.offsets, blk + d.block_start, x, y, d.num_rows,
+                                  d.num_blocks, d.block_width, stream));
+}
+
+template <typename T>
+void HBDIABatch<T>::spmvPerMatrix(const T* x, T* y) const {
+  for (int m = 0; m < numMatrices(); ++m) spmvMatrix(m, x + desc_[m].rows, y + desc_[m].rows, stream_);
 }
 
 template <typename T>

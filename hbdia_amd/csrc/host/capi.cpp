@@ -13,6 +13,7 @@
 #include "Format/HBDIAVector.hpp"
 #include "Operations/HBDIABatch.hpp"
 #include "Operations/HBDIAMixed.hpp"
+#include "Operations/HBDIAServe.hpp"
 #include "Operations/HBDIASpMV.hpp"
 #include "Workloads.hpp"
 
@@ -30,6 +31,16 @@ struct hbdia_batch_opaque {
   int dtype = HBDIA_DTYPE_F64;
   std::unique_ptr<HBDIABatch<double>> d;
   std::unique_ptr<HBDIABatch<float>> f;
+};
+
+struct hbdia_server_opaque {
+  int dtype = HBDIA_DTYPE_F64;
+  std::unique_ptr<HBDIAServer<double>> d;
+  std::unique_ptr<HBDIAServer<float>> f;
+};
+
+struct hbdia_arena_opaque {
+  hbdia::VectorArena arena{1};
 };
 
 struct hbdia_vector_opaque {
@@ -680,5 +691,182 @@ int hbdia_banded_batch_shape(int i, int min_rows, int max_rows, int min_offsets,
     *rows = s.rows;
     *noffsets = s.noffsets;
     *matrix_seed = s.seed;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// serving
+
+namespace {
+
+template <typename F>
+auto with_server(hbdia_server_t s, F&& fn) {
+  need(s != nullptr, "null server handle");
+  if (s->dtype == HBDIA_DTYPE_F64) return fn(*s->d);
+  return fn(*s->f);
+}
+
+}  // namespace
+
+int hbdia_server_create(hbdia_server_t* s, const hbdia_matrix_t* matrices, int count, int64_t hbm_budget_bytes,
+                        int max_jobs_per_launch, int max_launches_in_flight, int workgroups, int tiles_per_step,
+                        int batching) {
+  return guard("hbdia_server_create", [&] {
+    need(s != nullptr, "null handle pointer");
+    const int dt = batch_dtype(matrices, count);
+    ServeConfig cfg;
+    cfg.hbmBudgetBytes = hbm_budget_bytes;
+    cfg.maxJobsPerLaunch = max_jobs_per_launch;
+    cfg.maxLaunchesInFlight = max_launches_in_flight;
+    cfg.workgroups = workgroups;
+    cfg.tilesPerStep = tiles_per_step;
+    cfg.batching = batching != 0;
+    auto h = std::make_unique<hbdia_server_opaque>();
+    h->dtype = dt;
+    if (dt == HBDIA_DTYPE_F64) {
+      std::vector<const HBDIA<double>*> v;
+      for (int i = 0; i < count; ++i) v.push_back(matrices[i]->d.get());
+      h->d = std::make_unique<HBDIAServer<double>>(v, cfg);
+    } else {
+      std::vector<const HBDIA<float>*> v;
+      for (int i = 0; i < count; ++i) v.push_back(matrices[i]->f.get());
+      h->f = std::make_unique<HBDIAServer<float>>(v, cfg);
+    }
+    *s = h.release();
+  });
+}
+
+int hbdia_server_destroy(hbdia_server_t s) {
+  return guard("hbdia_server_destroy", [&] { delete s; });
+}
+
+int hbdia_server_submit(hbdia_server_t s, int matrix, const void* x, void* y, int device_vectors, int64_t* id) {
+  return guard("hbdia_server_submit", [&] {
+    need(s != nullptr && id != nullptr, "null argument");
+    if (s->dtype == HBDIA_DTYPE_F64)
+      *id = s->d->submit(matrix, static_cast<const double*>(x), static_cast<double*>(y), device_vectors != 0);
+    else
+      *id = s->f->submit(matrix, static_cast<const float*>(x), static_cast<float*>(y), device_vectors != 0);
+  });
+}
+
+int hbdia_server_wait(hbdia_server_t s, int64_t id) {
+  return guard("hbdia_server_wait", [&] {
+    with_server(s, [&](auto& S) {
+      S.wait(id);
+      return 0;
+    });
+  });
+}
+
+int hbdia_server_drain(hbdia_server_t s) {
+  return guard("hbdia_server_drain", [&] {
+    with_server(s, [&](auto& S) {
+      S.drain();
+      return 0;
+    });
+  });
+}
+
+int hbdia_server_times(hbdia_server_t s, int64_t id, int64_t* times) {
+  return guard("hbdia_server_times", [&] {
+    need(times != nullptr, "null output");
+    with_server(s, [&](auto& S) {
+      const RequestTimes t = S.times(id);
+      times[0] = t.submitNs;
+      times[1] = t.admitNs;
+      times[2] = t.completeNs;
+      return 0;
+    });
+  });
+}
+
+int hbdia_server_info(hbdia_server_t s, int64_t* info) {
+  return guard("hbdia_server_info", [&] {
+    need(info != nullptr, "null output");
+    with_server(s, [&](auto& S) {
+      info[HBDIA_SERVER_INFO_MATRICES] = S.numMatrices();
+      info[HBDIA_SERVER_INFO_RESIDENT_BYTES] = S.residentBytes();
+      info[HBDIA_SERVER_INFO_ARENA_BYTES] = S.arenaBytes();
+      info[HBDIA_SERVER_INFO_LAUNCHES] = S.launches();
+      info[HBDIA_SERVER_INFO_COMPLETED] = S.completed();
+      info[HBDIA_SERVER_INFO_LARGEST_LAUNCH] = S.largestLaunch();
+      info[HBDIA_SERVER_INFO_DTYPE] = s->dtype;
+      return 0;
+    });
+  });
+}
+
+int hbdia_server_num_rows(hbdia_server_t s, int matrix, int64_t* rows) {
+  return guard("hbdia_server_num_rows", [&] {
+    need(rows != nullptr, "null output");
+    with_server(s, [&](auto& S) {
+      need(matrix >= 0 && matrix < S.numMatrices(), "no such matrix");
+      *rows = S.numRows(matrix);
+      return 0;
+    });
+  });
+}
+
+int hbdia_server_open_loop(hbdia_server_t s, double rate, int64_t requests, uint64_t seed, const void* const* xs,
+                           int variants, void* const* ys, int device_vectors, double* latencies_us,
+                           double* wall_seconds) {
+  return guard("hbdia_server_open_loop", [&] {
+    need(s != nullptr && xs != nullptr && ys != nullptr && variants > 0, "bad argument");
+    need(requests >= 0 && (requests == 0 || latencies_us != nullptr), "bad request count or output");
+    LoadReport rep;
+    const int M = s->dtype == HBDIA_DTYPE_F64 ? s->d->numMatrices() : s->f->numMatrices();
+    if (s->dtype == HBDIA_DTYPE_F64) {
+      std::vector<const double*> x(static_cast<size_t>(M) * variants);
+      std::vector<double*> y(static_cast<size_t>(M));
+      for (size_t i = 0; i < x.size(); ++i) x[i] = static_cast<const double*>(xs[i]);
+      for (int m = 0; m < M; ++m) y[m] = static_cast<double*>(ys[m]);
+      rep = runOpenLoop<double>(*s->d, rate, requests, seed, x, variants, y, device_vectors != 0);
+    } else {
+      std::vector<const float*> x(static_cast<size_t>(M) * variants);
+      std::vector<float*> y(static_cast<size_t>(M));
+      for (size_t i = 0; i < x.size(); ++i) x[i] = static_cast<const float*>(xs[i]);
+      for (int m = 0; m < M; ++m) y[m] = static_cast<float*>(ys[m]);
+      rep = runOpenLoop<float>(*s->f, rate, requests, seed, x, variants, y, device_vectors != 0);
+    }
+    for (size_t i = 0; i < rep.latencyUs.size(); ++i) latencies_us[i] = rep.latencyUs[i];
+    if (wall_seconds) *wall_seconds = rep.wallSeconds;
+  });
+}
+
+int hbdia_arena_create(hbdia_arena_t* a, int64_t capacity, int64_t align) {
+  return guard("hbdia_arena_create", [&] {
+    need(a != nullptr, "null handle pointer");
+    auto h = std::make_unique<hbdia_arena_opaque>();
+    h->arena = hbdia::VectorArena(capacity, align);
+    *a = h.release();
+  });
+}
+
+int hbdia_arena_destroy(hbdia_arena_t a) {
+  return guard("hbdia_arena_destroy", [&] { delete a; });
+}
+
+int hbdia_arena_allocate(hbdia_arena_t a, int64_t n, int64_t* offset) {
+  return guard("hbdia_arena_allocate", [&] {
+    need(a != nullptr && offset != nullptr, "null argument");
+    *offset = a->arena.allocate(n);
+  });
+}
+
+int hbdia_arena_release(hbdia_arena_t a, int64_t offset) {
+  return guard("hbdia_arena_release", [&] {
+    need(a != nullptr, "null argument");
+    a->arena.release(offset);
+  });
+}
+
+int hbdia_arena_info(hbdia_arena_t a, int64_t* info) {
+  return guard("hbdia_arena_info", [&] {
+    need(a != nullptr && info != nullptr, "null argument");
+    info[0] = a->arena.capacity();
+    info[1] = a->arena.used();
+    info[2] = a->arena.largestFree();
+    info[3] = a->arena.freeBlocks();
   });
 }

@@ -90,6 +90,35 @@ class DeviceArray:
             pass
 
 
+class PinnedArray:
+    """Page-locked host memory (hipHostMalloc) viewed as a numpy array (.array);
+    copies from and to it can overlap kernels."""
+
+    def __init__(self, size: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.size = int(size)
+        p = c_void_p(None)
+        check_hip(hip.hbdia_malloc_host(byref(p), max(self.size, 1) * self.dtype.itemsize))
+        self._p = p.value
+        buf = (ctypes.c_char * (max(self.size, 1) * self.dtype.itemsize)).from_address(self._p)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=self.size)
+
+    @classmethod
+    def from_host(cls, arr: np.ndarray) -> "PinnedArray":
+        out = cls(arr.size, arr.dtype)
+        out.array[:] = arr.ravel()
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "_p", None):
+                self.array = None
+                hip.hbdia_free_host(self._p)
+                self._p = None
+        except Exception:
+            pass
+
+
 class Stream:
     """A non-blocking HIP stream at the default priority."""
 

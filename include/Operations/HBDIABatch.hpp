@@ -65,6 +65,14 @@ class HBDIABatch {
   // The same with one single-matrix launch per matrix (hbdia_hbdia_spmv_* or
   // hbdia_bdia_spmv_* on the matrix's part of the pools).
   void spmvPerMatrix(const T* x, T* y) const;
+  // One single-matrix launch of matrix m on `stream`; x and y point at the
+  // matrix's own first row (not at the pools).
+  void spmvMatrix(int m, const T* x, T* y, hbdia_stream_t stream) const;
+  // Device view of the pools (tile records, values, offsets, fallback) and
+  // the placement of matrix m in them.
+  const hbdia_batch_t& deviceBatch() const { return dev_; }
+  const hbdia_batch_desc_t& desc(int m) const { return desc_.at(m); }
+  int64_t tilesOf(int m) const { return (static_cast<int64_t>(desc_.at(m).num_rows) + 63) / 64; }
   // Records spmv(x, y) into a graph on the batch stream; launchGraph replays it.
   void captureGraph(const T* x, T* y);
   void launchGraph() const;

@@ -19,6 +19,7 @@
  *   hbdia_spmv_multi                   -> no reference counterpart: many right-hand sides in one launch
  *   hbdia_mixed_*                      -> no reference counterpart: FP32 / BF16 value storage
  *   hbdia_batch_*                      -> no reference counterpart: many matrices in one launch
+ *   hbdia_server_*, hbdia_arena_*      -> no reference counterpart: continuous-batching request loop
  *   hbdia_gen_*                        -> no reference counterpart: seeded benchmark inputs
  */
 #ifndef HBDIA_CAPI_H
@@ -153,6 +154,49 @@ int hbdia_batch_capture_graph(hbdia_batch_handle_t b, const void* x, void* y);
 int hbdia_batch_launch_graph(hbdia_batch_handle_t b);
 int hbdia_batch_synchronize(hbdia_batch_handle_t b);
 int hbdia_batch_stream(hbdia_batch_handle_t b, void** stream); /* its hbdia_stream_t */
+
+/* Continuous-batching server over resident matrices (HBDIAServer,
+ * include/Operations/HBDIAServe.hpp).  Matrices: one dtype, square, converted
+ * with block widths that are multiples of 64 (copied; may be destroyed after
+ * creation).  hbm_budget_bytes covers the resident matrices plus the request
+ * arena (0: resident bytes + 1 GiB); batching 0 selects the plain form (one
+ * request at a time through the single-matrix kernel).  x, y of a request:
+ * host memory, or device memory with device_vectors; valid until it completes.
+ * y = A x with y starting at zero. */
+typedef struct hbdia_server_opaque* hbdia_server_t;
+enum {
+  HBDIA_SERVER_INFO_MATRICES = 0,
+  HBDIA_SERVER_INFO_RESIDENT_BYTES = 1,
+  HBDIA_SERVER_INFO_ARENA_BYTES = 2,
+  HBDIA_SERVER_INFO_LAUNCHES = 3,
+  HBDIA_SERVER_INFO_COMPLETED = 4,
+  HBDIA_SERVER_INFO_LARGEST_LAUNCH = 5,
+  HBDIA_SERVER_INFO_DTYPE = 6,
+  HBDIA_SERVER_INFO_COUNT = 7
+};
+int hbdia_server_create(hbdia_server_t* s, const hbdia_matrix_t* matrices, int count, int64_t hbm_budget_bytes,
+                        int max_jobs_per_launch, int max_launches_in_flight, int workgroups, int tiles_per_step,
+                        int batching);
+int hbdia_server_destroy(hbdia_server_t s); /* completes every submitted request first */
+int hbdia_server_submit(hbdia_server_t s, int matrix, const void* x, void* y, int device_vectors, int64_t* id);
+int hbdia_server_wait(hbdia_server_t s, int64_t id);
+int hbdia_server_drain(hbdia_server_t s);
+int hbdia_server_times(hbdia_server_t s, int64_t id, int64_t* times); /* submit, admit, complete (ns) */
+int hbdia_server_info(hbdia_server_t s, int64_t* info);             /* HBDIA_SERVER_INFO_COUNT entries */
+int hbdia_server_num_rows(hbdia_server_t s, int matrix, int64_t* rows);
+/* Open-loop load (runOpenLoop): xs holds matrices * variants vectors (matrix
+ * m, variant v at m * variants + v), ys one per matrix; latencies_us receives
+ * `requests` latencies. */
+int hbdia_server_open_loop(hbdia_server_t s, double rate, int64_t requests, uint64_t seed, const void* const* xs,
+                           int variants, void* const* ys, int device_vectors, double* latencies_us,
+                           double* wall_seconds);
+/* The server's first-fit arena allocator on its own (host only). */
+typedef struct hbdia_arena_opaque* hbdia_arena_t;
+int hbdia_arena_create(hbdia_arena_t* a, int64_t capacity, int64_t align);
+int hbdia_arena_destroy(hbdia_arena_t a);
+int hbdia_arena_allocate(hbdia_arena_t a, int64_t n, int64_t* offset); /* *offset = -1 when it does not fit */
+int hbdia_arena_release(hbdia_arena_t a, int64_t offset);
+int hbdia_arena_info(hbdia_arena_t a, int64_t* info); /* capacity, used, largest free block, free blocks */
 
 /* Seeded benchmark inputs (generators in Workloads.hpp). */
 int hbdia_gen_stencil(hbdia_matrix_t* m, int dtype, int n, int points, double scatter, uint64_t seed);

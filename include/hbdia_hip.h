@@ -78,6 +78,7 @@ int hbdia_event_create(hbdia_event_t* event);
 int hbdia_event_destroy(hbdia_event_t event);
 int hbdia_event_record(hbdia_event_t event, hbdia_stream_t stream);
 int hbdia_event_synchronize(hbdia_event_t event);
+int hbdia_event_query(hbdia_event_t event, int* done); /* *done = 1 once the event has completed */
 int hbdia_stream_wait_event(hbdia_stream_t stream, hbdia_event_t event);
 int hbdia_event_elapsed_ms(float* ms, hbdia_event_t start, hbdia_event_t stop);
 int hbdia_graph_begin_capture(hbdia_stream_t stream);
@@ -250,6 +251,30 @@ int hbdia_batch_spmv_f64(const hbdia_batch_t* batch, const hbdia_batch_schedule_
                          double* y, hbdia_stream_t stream);
 int hbdia_batch_spmv_f32(const hbdia_batch_t* batch, const hbdia_batch_schedule_t* schedule, const float* x,
                          float* y, hbdia_stream_t stream);
+/* Serving: one persistent launch over a list of jobs, each job one matrix of a
+ * resident pool (tiles [tile_begin, tile_begin + its tile count) of the pool's
+ * tile records) applied to its own slot of the x and y arenas.  The launch's
+ * work is the concatenation of the jobs' tiles: job k owns work indices
+ * [work_begin_k, work_begin_{k+1}) (work_begin ascending, starting at 0;
+ * total_tiles = the sum of all jobs' tiles).  A tile of job k reads x and
+ * writes y at (record rows + row_shift_k) + row, so a request whose slot
+ * starts at S for a matrix whose records carry rows R uses row_shift = S - R.
+ * Interleaved schedule (runs of tiles_per_step tiles, num_workgroups
+ * workgroups); every job's rows come out bit for bit as in the batch and
+ * single-matrix kernels. */
+typedef struct hbdia_serve_job {
+  int64_t work_begin;
+  int64_t tile_begin;
+  int32_t row_shift;
+  int32_t pad_;
+} hbdia_serve_job_t;
+int hbdia_serve_spmv_f64(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t num_jobs,
+                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, const double* x,
+                         double* y, hbdia_stream_t stream);
+int hbdia_serve_spmv_f32(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t num_jobs,
+                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, const float* x,
+                         float* y, hbdia_stream_t stream);
+
 /* Workgroups of the batch kernel that one compute unit holds at once (the
  * kernel variant that `batch`, `kind` and value_size (4 or 8) select); a grid
  * of at most this times the compute units is resident as a whole. */

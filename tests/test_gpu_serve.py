@@ -1,0 +1,160 @@
+"""GPU tests of the continuous-batching server.
+
+Every request's y must be the same, bit for bit, whether the request runs
+alone or in a mix with others (admitted into shared launches, submitted from
+several threads), and equal hbdiaSpMV on its matrix with y starting at zero
+and the oracle.  Also: the plain form, device-resident vectors, admission
+under a tight HBM budget and rejection of requests that can never fit.
+"""
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+import pytest
+
+import oracle
+from hbdia_amd import HBDIA, DeviceArray, HBDIAError, HBDIAServer, HBDIAVector, gen_vector, hbdiaSpMV
+
+pytestmark = pytest.mark.gpu
+
+
+def pool(dtype=np.float64):
+    ms = [HBDIA.stencil(8, 27, 0.01, seed=1, dtype=dtype), HBDIA.stencil(13, 7, 0.02, seed=2, dtype=dtype),
+          HBDIA.stencil(20, 27, 0.005, seed=3, dtype=dtype), HBDIA.banded(3000, 5, 64, 0.01, seed=4, dtype=dtype),
+          HBDIA.banded(777, 9, 64, 0.0, seed=5, dtype=dtype), HBDIA.stencil(16, 7, 0.0, seed=6, dtype=dtype)]
+    for i, m in enumerate(ms):
+        m.convertToHBDIAFormatOnHost([64, 128, 256][i % 3], 2)
+    return ms
+
+
+def xs_for(ms, dtype, variants=3):
+    return [[gen_vector(m.getNumRows(), 100 * i + v, dtype=dtype) for v in range(variants)] for i, m in enumerate(ms)]
+
+
+def expected(A: HBDIA, x: np.ndarray) -> np.ndarray:
+    hb = A.getHBDIA()
+    hb["num_blocks"], hb["block_width"] = A.getNumBlocks(), A.getBlockWidth()
+    return oracle.hbdia_spmv(hb, x.copy(), np.zeros_like(x), A.getNumRows())
+
+
+def single(A: HBDIA, x: np.ndarray) -> np.ndarray:
+    A.prepareForGPU()
+    X, Y = HBDIAVector(x), HBDIAVector(np.zeros_like(x))
+    hbdiaSpMV(A, X, Y, False, True)
+    A.synchronize()
+    return Y.download()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_same_result_alone_and_in_the_mix(gpu, dtype):
+    print(f"case: serve alone vs mix {np.dtype(dtype).name}", flush=True)
+    ms = pool(dtype)
+    xs = xs_for(ms, dtype)
+    S = HBDIAServer(ms, max_jobs=64, max_in_flight=2)
+    # alone: one request at a time
+    alone = {}
+    for m in range(len(ms)):
+        for v in range(3):
+            y = np.full(ms[m].getNumRows(), np.nan, dtype)
+            S.wait(S.submit(m, xs[m][v], y))
+            alone[m, v] = y
+            np.testing.assert_array_equal(y, expected(ms[m], xs[m][v]), err_msg=f"matrix {m} vs oracle")
+    for m in range(len(ms)):
+        np.testing.assert_array_equal(alone[m, 0], single(ms[m], xs[m][0]), err_msg=f"matrix {m} vs hbdiaSpMV")
+    # in the mix: a burst of requests from four threads at once
+    rng = np.random.default_rng(7)
+    picks = [(int(rng.integers(len(ms))), int(rng.integers(3))) for _ in range(240)]
+    outs = [np.full(ms[m].getNumRows(), np.nan, dtype) for m, _ in picks]
+    ids = [None] * len(picks)
+    launches_before = S.info()["launches"]
+
+    def submitter(k):
+        for i in range(k, len(picks), 4):
+            m, v = picks[i]
+            ids[i] = S.submit(m, xs[m][v], outs[i])
+
+    th = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    S.drain()
+    for i, (m, v) in enumerate(picks):
+        np.testing.assert_array_equal(outs[i], alone[m, v], err_msg=f"request {i} (matrix {m}) differs in the mix")
+    info = S.info()
+    assert info["completed"] == 18 + len(picks)
+    assert info["launches"] - launches_before < len(picks)  # requests shared launches
+    assert info["largest_launch"] > 1
+    for rid in ids:
+        ts, ta, tc = S.times(rid)
+        assert ts <= ta <= tc
+    S.close()
+
+
+def test_plain_form_and_device_vectors_agree(gpu):
+    print("case: serve plain form and device vectors", flush=True)
+    ms = pool()
+    xs = xs_for(ms, np.float64, variants=1)
+    P = HBDIAServer(ms, batching=False)
+    B = HBDIAServer(ms)
+    for m in range(len(ms)):
+        want = expected(ms[m], xs[m][0])
+        y1 = np.zeros(ms[m].getNumRows())
+        P.wait(P.submit(m, xs[m][0], y1))
+        np.testing.assert_array_equal(y1, want)
+        X = DeviceArray.from_host(xs[m][0])
+        Y = DeviceArray(ms[m].getNumRows(), np.float64, zero=True)
+        B.wait(B.submit(m, X, Y, device=True))
+        np.testing.assert_array_equal(Y.download(), want)
+    assert P.info()["largest_launch"] == 1
+    P.close()
+    B.close()
+
+
+def test_tight_budget_queues_and_rejects(gpu):
+    print("case: serve tight budget", flush=True)
+    ms = pool()
+    xs = xs_for(ms, np.float64, variants=1)
+    probe = HBDIAServer(ms)
+    resident = probe.info()["resident_bytes"]
+    probe.close()
+    big = max(m.getNumRows() for m in ms)  # 8000 rows (20^3)
+    # room for the x and y of two of the largest requests
+    budget = resident + 2 * 2 * 8 * (big + 64)
+    S = HBDIAServer(ms, hbm_budget=budget)
+    assert S.info()["arena_bytes"] <= budget - resident
+    outs, ids, want = [], [], []
+    for i in range(60):
+        m = i % len(ms)
+        y = np.zeros(ms[m].getNumRows())
+        outs.append(y)
+        ids.append(S.submit(m, xs[m][0], y))
+        want.append(m)
+    S.drain()
+    for y, m in zip(outs, want):
+        np.testing.assert_array_equal(y, expected(ms[m], xs[m][0]))
+    S.close()
+    # a budget whose arena cannot hold the largest matrix's vectors: that request is refused
+    small = resident + 2 * 8 * 4096
+    T = HBDIAServer(ms, hbm_budget=small)
+    with pytest.raises(HBDIAError, match="exceed the request arena"):
+        T.submit(2, xs[2][0], np.zeros(ms[2].getNumRows()))
+    y = np.zeros(ms[0].getNumRows())
+    T.wait(T.submit(0, xs[0][0], y))  # a small one still runs
+    np.testing.assert_array_equal(y, expected(ms[0], xs[0][0]))
+    T.close()
+
+
+def test_open_loop_driver_reports_latencies(gpu):
+    print("case: serve open loop", flush=True)
+    ms = pool()
+    xs = xs_for(ms, np.float64, variants=2)
+    S = HBDIAServer(ms)
+    flat = [x for per in xs for x in per]
+    ys = [np.zeros(m.getNumRows()) for m in ms]
+    lat, wall = S.open_loop(2000.0, 200, 3, flat, 2, ys)
+    assert lat.shape == (200,) and np.all(lat > 0) and wall > 0
+    lat0, _ = S.open_loop(0.0, 100, 4, flat, 2, ys)  # burst
+    assert np.all(lat0 > 0)
+    S.close()
