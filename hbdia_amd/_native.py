@@ -72,6 +72,10 @@ _HIP_SIGS = {
     "hbdia_bdia_spmv_f32": (_i, [_p, _p, _p, _p, _p, c_int64, c_int32, c_int32, _p]),
     "hbdia_hbdia_spmv_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int64, _p]),
     "hbdia_hbdia_spmv_f32": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int64, _p]),
+    "hbdia_bdia_spmv_ex_f64": (_i, [_p, _p, _p, _p, _p, c_int64, c_int32, c_int32, _p, _p]),
+    "hbdia_bdia_spmv_ex_f32": (_i, [_p, _p, _p, _p, _p, c_int64, c_int32, c_int32, _p, _p]),
+    "hbdia_hbdia_spmv_ex_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int64, _p, _p]),
+    "hbdia_hbdia_spmv_ex_f32": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int64, _p, _p]),
     "hbdia_hbdia_spmm_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, c_int64, _p, c_int64, c_int32,
                                   c_int64, _p]),
     "hbdia_hbdia_spmm_f32": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, c_int64, _p, c_int64, c_int32,

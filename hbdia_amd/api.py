@@ -29,7 +29,8 @@ RAND_SEED = 42
 
 F64, F32 = 0, 1
 _INFO = ["rows", "cols", "nnz", "has_coo", "has_hbdia", "block_width", "threshold", "num_blocks", "num_values",
-         "num_offsets", "num_coo", "num_diagonals", "hist_len", "on_device", "device_coo", "stored_bytes"]
+         "num_offsets", "num_coo", "num_diagonals", "hist_len", "on_device", "device_coo", "stored_bytes",
+         "runs3_blocks"]
 
 
 class ExecutionMode(enum.IntEnum):
@@ -159,7 +160,7 @@ class HBDIA:
 
     # ---- getters ----------------------------------------------------------
     def info(self) -> dict:
-        arr = np.zeros(16, dtype=np.int64)
+        arr = np.zeros(len(_INFO), dtype=np.int64)
         check_capi(capi.hbdia_matrix_info(self._h, arr.ctypes.data))
         return dict(zip(_INFO, (int(v) for v in arr)))
 

@@ -186,6 +186,85 @@ __device__ __forceinline__ T bdia_sum_uniform(const T* __restrict__ vb, const in
   return sum;
 }
 
+// Whole-wave lane shifts (DPP wave_shr:1 / wave_shl:1, 32 bits at a time):
+// shr1 gives lane i the value of lane i-1 and lane 0 `first`; shl1 gives
+// lane i the value of lane i+1 and lane 63 `last`.
+__device__ __forceinline__ int dpp_shr1(int v, int first) {
+  return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xF, 0xF, false);
+}
+__device__ __forceinline__ int dpp_shl1(int v, int last) {
+  return __builtin_amdgcn_update_dpp(last, v, 0x130, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float wave_shr1(float v, float first) {
+  return __builtin_bit_cast(float, dpp_shr1(__builtin_bit_cast(int, v), __builtin_bit_cast(int, first)));
+}
+__device__ __forceinline__ float wave_shl1(float v, float last) {
+  return __builtin_bit_cast(float, dpp_shl1(__builtin_bit_cast(int, v), __builtin_bit_cast(int, last)));
+}
+__device__ __forceinline__ double wave_shr1(double v, double first) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v), f = __builtin_bit_cast(uint64_t, first);
+  const uint32_t lo = static_cast<uint32_t>(dpp_shr1(static_cast<int>(b), static_cast<int>(f)));
+  const uint32_t hi = static_cast<uint32_t>(dpp_shr1(static_cast<int>(b >> 32), static_cast<int>(f >> 32)));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double wave_shl1(double v, double last) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v), l = __builtin_bit_cast(uint64_t, last);
+  const uint32_t lo = static_cast<uint32_t>(dpp_shl1(static_cast<int>(b), static_cast<int>(l)));
+  const uint32_t hi = static_cast<uint32_t>(dpp_shl1(static_cast<int>(b >> 32), static_cast<int>(l >> 32)));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// bdia_sum_uniform for a block whose offsets are runs of three consecutive
+// values (o-1, o, o+1), as a 27-point stencil's are: a chunk of nine offsets
+// is three runs, and each run's three x vectors come from one load at its
+// centre o -- the neighbours are that vector shifted by one lane, with the
+// lane that falls off taken from x[row0+o-1] or x[row0+o+64].  The x values,
+// skips and FMA order are those of bdia_sum_uniform, with a third of its x
+// loads.  The x values, skips and FMA order are exactly those of
+// bdia_sum_uniform, with a third of its x loads (and of their cache-line
+// requests).
+template <typename T, bool X32>
+__device__ __forceinline__ T bdia_sum_runs3(const T* __restrict__ vb, const int* __restrict__ op, int cnt, int bw,
+                                            int lane, const T* __restrict__ x, int64_t row0, int64_t row,
+                                            int64_t nrows, bool active) {
+  static_assert(kChunk % 3 == 0, "a chunk holds whole runs");
+  T sum = T(0);
+  const int64_t last = nrows - 1;
+  for (int o0 = 0; o0 < cnt; o0 += kChunk) {
+    int off[kChunk];
+    T v[kChunk], xl[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
+#pragma unroll
+    for (int w = 0; w < kChunk / 3; ++w) {
+      // runs past the block's count read padding: clamped, their products unused
+      const int64_t c = off[3 * w + 1];
+      int64_t cl = row + c;
+      cl = cl < 0 ? 0 : (cl > last ? last : cl);
+      int64_t cf = row0 + c - 1, ce = row0 + c + kWave;
+      cf = cf < 0 ? 0 : (cf > last ? last : cf);
+      ce = ce < 0 ? 0 : (ce > last ? last : ce);
+      // lanes 0-31 load x[row0+o-1] (shifted in at lane 0), lanes 32-63
+      // x[row0+o+64] (shifted in at lane 63): one load, two lines
+      const T mid = load_x<T, X32>(x, cl);
+      const T edge = load_x<T, X32>(x, lane < kWave / 2 ? cf : ce);
+      xl[3 * w] = wave_shr1(mid, edge);
+      xl[3 * w + 1] = mid;
+      xl[3 * w + 2] = wave_shl1(mid, edge);
+    }
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const int64_t c = row + off[k];
+      const bool ok = (o0 + k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
+      sum = masked_fma(ok, v[k], xl[k], sum);
+    }
+  }
+  return sum;
+}
+
 // The same for a row whose block is per lane (block width not a multiple of
 // 64): `vp` is the lane's address in its block's first offset line, `op` the
 // lane's offset list; lanes leave the loop at their own count.
@@ -224,12 +303,12 @@ __device__ __forceinline__ T bdia_sum_lane(const T* __restrict__ vp, const int* 
 // vector_add_kernel does for all rows.  X32: x fits 32-bit byte offsets.
 // Workgroups are taken in row order: measured on MI355X, row order beats an
 // XCD-contiguous remap (neighbouring rows run on all eight XCDs at once).
-template <typename T, bool UNIFORM, bool HAS_COO, bool X32>
+template <typename T, bool UNIFORM, bool HAS_COO, bool X32, bool RUNS3>
 __global__ __launch_bounds__(kWG) void k_hbdia(
     const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const T* __restrict__ x, T* __restrict__ y, int64_t nrows, int nb, int bw,
     const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
-    const int* __restrict__ gptr) {
+    const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks) {
   __shared__ T s_v[kWavesPerWG][kWave];
   __shared__ T s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][kWave];
@@ -256,7 +335,10 @@ __global__ __launch_bounds__(kWG) void k_hbdia(
     const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(T)) < (1ll << 32);
     if (cnt > 0 && small) {
       const T* vb = vals + static_cast<int64_t>(start) * bw + (row0 - b * bw);
-      sum = bdia_sum_uniform<T, X32>(vb, offs + start, cnt, bw, lane, x, row, nrows, active);
+      bool runs = false;
+      if constexpr (RUNS3) runs = (runs3_blocks[b >> 5] >> (b & 31)) & 1u;
+      if (runs) sum = bdia_sum_runs3<T, X32>(vb, offs + start, cnt, bw, lane, x, row0, row, nrows, active);
+      else sum = bdia_sum_uniform<T, X32>(vb, offs + start, cnt, bw, lane, x, row, nrows, active);
     } else if (cnt > 0 && active) {
       const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
       sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
@@ -938,7 +1020,7 @@ template <typename T>
 int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t nb, int32_t bw,
                  const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr,
                  const T* x, T* y, int64_t nrows, hbdia_stream_t stream, bool with_coo,
-                 const char* name) {
+                 const char* name, const uint32_t* runs3_blocks = nullptr) {
   if (nrows < 0 || nb < 0 || bw <= 0)
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative size or block_width <= 0");
   if (nrows == 0) return HBDIA_SUCCESS;
@@ -955,15 +1037,19 @@ int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   // (checked by the caller through the largest block).
   const bool x32 = static_cast<uint64_t>(nrows) * sizeof(T) < (1ull << 32);
   hipStream_t s = hs(stream);
-#define HBDIA_LAUNCH(U, C, X)                                                                          \
-  hipLaunchKernelGGL((k_hbdia<T, U, C, X>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, \
-                     cv, gptr)
-  if (uniform) {
-    if (with_coo) { if (x32) HBDIA_LAUNCH(true, true, true); else HBDIA_LAUNCH(true, true, false); }
-    else { if (x32) HBDIA_LAUNCH(true, false, true); else HBDIA_LAUNCH(true, false, false); }
+  const bool runs3 = uniform && runs3_blocks != nullptr;
+#define HBDIA_LAUNCH(U, C, X, R)                                                                             \
+  hipLaunchKernelGGL((k_hbdia<T, U, C, X, R>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, \
+                     cv, gptr, runs3_blocks)
+  if (runs3) {
+    if (with_coo) { if (x32) HBDIA_LAUNCH(true, true, true, true); else HBDIA_LAUNCH(true, true, false, true); }
+    else { if (x32) HBDIA_LAUNCH(true, false, true, true); else HBDIA_LAUNCH(true, false, false, true); }
+  } else if (uniform) {
+    if (with_coo) { if (x32) HBDIA_LAUNCH(true, true, true, false); else HBDIA_LAUNCH(true, true, false, false); }
+    else { if (x32) HBDIA_LAUNCH(true, false, true, false); else HBDIA_LAUNCH(true, false, false, false); }
   } else {
-    if (with_coo) { if (x32) HBDIA_LAUNCH(false, true, true); else HBDIA_LAUNCH(false, true, false); }
-    else { if (x32) HBDIA_LAUNCH(false, false, true); else HBDIA_LAUNCH(false, false, false); }
+    if (with_coo) { if (x32) HBDIA_LAUNCH(false, true, true, false); else HBDIA_LAUNCH(false, true, false, false); }
+    else { if (x32) HBDIA_LAUNCH(false, false, true, false); else HBDIA_LAUNCH(false, false, false, false); }
   }
 #undef HBDIA_LAUNCH
   return check_hip(hipGetLastError(), name);
@@ -1369,6 +1455,35 @@ int hbdia_hbdia_spmv_f32(const float* values, const int32_t* offsets, const int3
   return launch_hbdia<float>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols,
                              coo_values, coo_group_ptr, x, y, num_rows, stream, true,
                              "hbdia_hbdia_spmv_f32");
+}
+
+int hbdia_bdia_spmv_ex_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                           const double* x, double* y, int64_t num_rows, int32_t num_blocks, int32_t block_width,
+                           const uint32_t* runs3_blocks, hbdia_stream_t stream) {
+  return launch_hbdia<double>(values, offsets, block_start, num_blocks, block_width, nullptr, nullptr, nullptr,
+                              nullptr, x, y, num_rows, stream, false, "hbdia_bdia_spmv_ex_f64", runs3_blocks);
+}
+int hbdia_bdia_spmv_ex_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                           const float* x, float* y, int64_t num_rows, int32_t num_blocks, int32_t block_width,
+                           const uint32_t* runs3_blocks, hbdia_stream_t stream) {
+  return launch_hbdia<float>(values, offsets, block_start, num_blocks, block_width, nullptr, nullptr, nullptr,
+                             nullptr, x, y, num_rows, stream, false, "hbdia_bdia_spmv_ex_f32", runs3_blocks);
+}
+int hbdia_hbdia_spmv_ex_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const double* coo_values, const int32_t* coo_group_ptr,
+                            const double* x, double* y, int64_t num_rows, const uint32_t* runs3_blocks,
+                            hbdia_stream_t stream) {
+  return launch_hbdia<double>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols, coo_values,
+                              coo_group_ptr, x, y, num_rows, stream, true, "hbdia_hbdia_spmv_ex_f64", runs3_blocks);
+}
+int hbdia_hbdia_spmv_ex_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const float* coo_values, const int32_t* coo_group_ptr,
+                            const float* x, float* y, int64_t num_rows, const uint32_t* runs3_blocks,
+                            hbdia_stream_t stream) {
+  return launch_hbdia<float>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols, coo_values,
+                             coo_group_ptr, x, y, num_rows, stream, true, "hbdia_hbdia_spmv_ex_f32", runs3_blocks);
 }
 int hbdia_hbdia_spmm_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
                          int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,

@@ -183,6 +183,8 @@ void HBDIA<T>::moveFrom(HBDIA& o) noexcept {
   cpuValues_d_ = std::exchange(o.cpuValues_d_, nullptr);
   cpuRowPtr_d_ = std::exchange(o.cpuRowPtr_d_, nullptr);
   cooGroupPtr_d_ = std::exchange(o.cooGroupPtr_d_, nullptr);
+  runs3Blocks_d_ = std::exchange(o.runs3Blocks_d_, nullptr);
+  runs3Count_ = std::exchange(o.runs3Count_, 0);
   sBDIA_ = std::exchange(o.sBDIA_, nullptr);
   sD2H_ = std::exchange(o.sD2H_, nullptr);
   sH2D_ = std::exchange(o.sH2D_, nullptr);
@@ -688,6 +690,23 @@ void HBDIA<T>::prepareForGPU() {
     flattenedOffsets.insert(flattenedOffsets.end(), offsetsPerBlock[b].begin(), offsetsPerBlock[b].end());
   }
   blockStartIndices.push_back(static_cast<int>(flattenedOffsets.size()));
+  // layout hint for the kernels: the blocks whose offsets are runs of three
+  // consecutive values (a multiple of three offsets, each triple consecutive)
+  std::vector<uint32_t> runs3Bits;
+  runs3Count_ = 0;
+  if (blockWidth % HBDIA_ROW_GROUP == 0 && nb > 0) {
+    runs3Bits.assign((static_cast<size_t>(nb) + 31) / 32, 0u);
+    for (int b = 0; b < nb; ++b) {
+      const auto& o = offsetsPerBlock[b];
+      bool runs3 = !o.empty() && o.size() % 3 == 0;
+      for (size_t t = 0; runs3 && t + 2 < o.size(); t += 3)
+        runs3 = o[t + 1] == o[t] + 1 && o[t + 2] == o[t + 1] + 1;
+      if (runs3) {
+        runs3Bits[b >> 5] |= 1u << (b & 31);
+        ++runs3Count_;
+      }
+    }
+  }
   // the kernels read a chunk's offsets as one wide load: keep the tail readable
   std::vector<int> paddedOffsets(flattenedOffsets);
   paddedOffsets.resize(flattenedOffsets.size() + HBDIA_OFFSETS_PAD, 0);
@@ -695,6 +714,7 @@ void HBDIA<T>::prepareForGPU() {
 
   dev_upload(&hbdiaData_d_, hbdiaData.data(), hbdiaData.size());
   dev_upload(&flattenedOffsets_d_, paddedOffsets.data(), paddedOffsets.size());
+  if (runs3Count_ > 0) dev_upload(&runs3Blocks_d_, runs3Bits.data(), runs3Bits.size());
   dev_upload(&blockStartIndices_d_, blockStartIndices.data(), blockStartIndices.size());
   dev_upload(&blockSizes_d_, blockSizes.data(), blockSizes.size());
 
@@ -740,6 +760,8 @@ void HBDIA<T>::releaseDevice() noexcept {
   }
   dev_free(hbdiaData_d_);
   dev_free(flattenedOffsets_d_);
+  dev_free(runs3Blocks_d_);
+  runs3Count_ = 0;
   dev_free(blockStartIndices_d_);
   dev_free(blockSizes_d_);
   dev_free(cpuRowIndices_d_);

@@ -16,15 +16,15 @@ struct Dev;
 template <>
 struct Dev<double> {
   static int bdia(const HBDIA<double>& m, const double* x, double* y, hbdia_stream_t s) {
-    return hbdia_bdia_spmv_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
-                               m.getBlockStartIndicesDevice(), x, y, m.getNumRows(), m.getNumBlocks(),
-                               m.getBlockWidth(), s);
+    return hbdia_bdia_spmv_ex_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
+                                  m.getBlockStartIndicesDevice(), x, y, m.getNumRows(), m.getNumBlocks(),
+                                  m.getBlockWidth(), m.getRuns3BlocksDevice(), s);
   }
   static int fused(const HBDIA<double>& m, const double* x, double* y, hbdia_stream_t s) {
-    return hbdia_hbdia_spmv_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
-                                m.getBlockStartIndicesDevice(), m.getNumBlocks(), m.getBlockWidth(),
-                                m.getCpuRowIndicesDevice(), m.getCpuColIndicesDevice(),
-                                m.getCpuValuesDevice(), m.getCooGroupPtrDevice(), x, y, m.getNumRows(), s);
+    return hbdia_hbdia_spmv_ex_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
+                                   m.getBlockStartIndicesDevice(), m.getNumBlocks(), m.getBlockWidth(),
+                                   m.getCpuRowIndicesDevice(), m.getCpuColIndicesDevice(), m.getCpuValuesDevice(),
+                                   m.getCooGroupPtrDevice(), x, y, m.getNumRows(), m.getRuns3BlocksDevice(), s);
   }
   static int coo(const HBDIA<double>& m, const double* x, double* res, hbdia_stream_t s) {
     return hbdia_coo_spmv_f64(m.getCpuRowIndicesDevice(), m.getCpuColIndicesDevice(), m.getCpuValuesDevice(),
@@ -46,15 +46,15 @@ struct Dev<double> {
 template <>
 struct Dev<float> {
   static int bdia(const HBDIA<float>& m, const float* x, float* y, hbdia_stream_t s) {
-    return hbdia_bdia_spmv_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
-                               m.getBlockStartIndicesDevice(), x, y, m.getNumRows(), m.getNumBlocks(),
-                               m.getBlockWidth(), s);
+    return hbdia_bdia_spmv_ex_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
+                                  m.getBlockStartIndicesDevice(), x, y, m.getNumRows(), m.getNumBlocks(),
+                                  m.getBlockWidth(), m.getRuns3BlocksDevice(), s);
   }
   static int fused(const HBDIA<float>& m, const float* x, float* y, hbdia_stream_t s) {
-    return hbdia_hbdia_spmv_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
-                                m.getBlockStartIndicesDevice(), m.getNumBlocks(), m.getBlockWidth(),
-                                m.getCpuRowIndicesDevice(), m.getCpuColIndicesDevice(),
-                                m.getCpuValuesDevice(), m.getCooGroupPtrDevice(), x, y, m.getNumRows(), s);
+    return hbdia_hbdia_spmv_ex_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
+                                   m.getBlockStartIndicesDevice(), m.getNumBlocks(), m.getBlockWidth(),
+                                   m.getCpuRowIndicesDevice(), m.getCpuColIndicesDevice(), m.getCpuValuesDevice(),
+                                   m.getCooGroupPtrDevice(), x, y, m.getNumRows(), m.getRuns3BlocksDevice(), s);
   }
   static int coo(const HBDIA<float>& m, const float* x, float* res, hbdia_stream_t s) {
     return hbdia_coo_spmv_f32(m.getCpuRowIndicesDevice(), m.getCpuColIndicesDevice(), m.getCpuValuesDevice(),

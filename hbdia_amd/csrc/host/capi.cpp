@@ -100,6 +100,7 @@ void info_of(const HBDIA<T>& m, int64_t* info) {
   info[HBDIA_INFO_ON_DEVICE] = m.areStreamsInitialized();
   info[HBDIA_INFO_DEVICE_COO] = m.hasDeviceCOO();
   info[HBDIA_INFO_STORED_BYTES] = static_cast<int64_t>(m.storedBytes());
+  info[HBDIA_INFO_RUNS3_BLOCKS] = m.getRuns3BlockCount();
 }
 
 template <typename T>

@@ -49,6 +49,38 @@ def hbdia_spmv(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArr
                  coo_cols.ptr, coo_values.ptr, coo_group_ptr.ptr, x.ptr, y.ptr, int(num_rows), _h(stream)))
 
 
+def runs3_bits(offsets: np.ndarray, block_start: np.ndarray, block_width: int) -> np.ndarray | None:
+    """The runs3_blocks hint: bit b set when block b's offsets are runs of three consecutive values
+    (a multiple of three offsets, each triple consecutive); None when no block qualifies or bw % 64 != 0."""
+    nb = len(block_start) - 1
+    if block_width % ROW_GROUP or nb <= 0:
+        return None
+    bits = np.zeros((nb + 31) // 32, np.uint32)
+    for b in range(nb):
+        o = np.asarray(offsets[block_start[b]:block_start[b + 1]])
+        if len(o) and len(o) % 3 == 0 and np.all(o[1::3] == o[0::3] + 1) and np.all(o[2::3] == o[1::3] + 1):
+            bits[b >> 5] |= np.uint32(1 << (b & 31))
+    return bits if bits.any() else None
+
+
+def bdia_spmv_ex(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArray, x: DeviceArray,
+                 y: DeviceArray, num_rows: int, num_blocks: int, block_width: int, runs3: DeviceArray | None,
+                 stream: Stream | None = None) -> None:
+    fn = getattr(hip, "hbdia_bdia_spmv_ex_" + _sfx(values.dtype))
+    check_hip(fn(values.ptr, offsets.ptr, block_start.ptr, x.ptr, y.ptr, int(num_rows), int(num_blocks),
+                 int(block_width), runs3.ptr if runs3 is not None else None, _h(stream)))
+
+
+def hbdia_spmv_ex(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArray, num_blocks: int,
+                  block_width: int, coo_rows: DeviceArray, coo_cols: DeviceArray, coo_values: DeviceArray,
+                  coo_group_ptr: DeviceArray, x: DeviceArray, y: DeviceArray, num_rows: int,
+                  runs3: DeviceArray | None, stream: Stream | None = None) -> None:
+    fn = getattr(hip, "hbdia_hbdia_spmv_ex_" + _sfx(values.dtype))
+    check_hip(fn(values.ptr, offsets.ptr, block_start.ptr, int(num_blocks), int(block_width), coo_rows.ptr,
+                 coo_cols.ptr, coo_values.ptr, coo_group_ptr.ptr, x.ptr, y.ptr, int(num_rows),
+                 runs3.ptr if runs3 is not None else None, _h(stream)))
+
+
 def hbdia_spmm(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArray, num_blocks: int,
                block_width: int, coo, X: DeviceArray, ldx: int, Y: DeviceArray, ldy: int, nrhs: int, num_rows: int,
                stream: Stream | None = None) -> None:
@@ -91,6 +123,8 @@ class DeviceLayout:
         self.num_rows = int(num_rows)
         self.block_width = int(block_width)
         self.num_blocks = int(len(block_start) - 1)
+        bits = runs3_bits(np.asarray(offsets), np.asarray(block_start, np.int32), self.block_width)
+        self.runs3 = DeviceArray.from_host(bits) if bits is not None else None
         self.dtype = np.dtype(values.dtype)
         self.values = DeviceArray.from_host(values if values.size else np.zeros(1, self.dtype))
         # blocked kernels read HBDIA_OFFSETS_PAD entries past the last offset
@@ -118,11 +152,14 @@ class DeviceLayout:
         hbdia_spmm(self.values, self.offsets, self.block_start, self.num_blocks, self.block_width, coo, X,
                    self.num_rows, Y, self.num_rows, nrhs, self.num_rows, stream)
 
-    def spmv(self, x: DeviceArray, y: DeviceArray, stream: Stream | None = None) -> None:
-        """y = A x with the fused kernel (blocked-DIA only when there is no fallback)."""
+    def spmv(self, x: DeviceArray, y: DeviceArray, stream: Stream | None = None, hints: bool = False) -> None:
+        """y = A x with the fused kernel (blocked-DIA only when there is no fallback); `hints` passes
+        the runs-of-three block bitset (computed from the host offsets)."""
+        runs3 = self.runs3 if hints else None
         if self.coo_nnz:
-            hbdia_spmv(self.values, self.offsets, self.block_start, self.num_blocks, self.block_width,
-                       self.coo_rows, self.coo_cols, self.coo_values, self.group_ptr, x, y, self.num_rows, stream)
+            hbdia_spmv_ex(self.values, self.offsets, self.block_start, self.num_blocks, self.block_width,
+                          self.coo_rows, self.coo_cols, self.coo_values, self.group_ptr, x, y, self.num_rows, runs3,
+                          stream)
         else:
-            bdia_spmv(self.values, self.offsets, self.block_start, x, y, self.num_rows, self.num_blocks,
-                      self.block_width, stream)
+            bdia_spmv_ex(self.values, self.offsets, self.block_start, x, y, self.num_rows, self.num_blocks,
+                         self.block_width, runs3, stream)

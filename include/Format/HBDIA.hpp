@@ -167,6 +167,11 @@ class HBDIA {
   // Per-64-row-group start into the device COO arrays (fused kernel input).
   const int* getCooGroupPtrDevice() const { return cooGroupPtr_d_; }
   bool hasDeviceCOO() const { return cooGroupPtr_d_ != nullptr; }
+  // Device bitset of the blocks whose offsets are runs of three consecutive
+  // values (the kernels' runs3_blocks hint), or null when no block is; built
+  // by prepareForGPU for block widths that are multiples of 64.
+  const uint32_t* getRuns3BlocksDevice() const { return runs3Blocks_d_; }
+  int64_t getRuns3BlockCount() const { return runs3Count_; }
 
   void prepareForGPU();
   void cleanupGPUData();
@@ -246,6 +251,8 @@ class HBDIA {
   T* cpuValues_d_ = nullptr;
   int* cpuRowPtr_d_ = nullptr;
   int* cooGroupPtr_d_ = nullptr;
+  uint32_t* runs3Blocks_d_ = nullptr;
+  int64_t runs3Count_ = 0;
 
   hbdia_stream_t sBDIA_ = nullptr;
   hbdia_stream_t sD2H_ = nullptr;

@@ -52,7 +52,8 @@ enum {
   HBDIA_INFO_ON_DEVICE = 13,
   HBDIA_INFO_DEVICE_COO = 14,
   HBDIA_INFO_STORED_BYTES = 15,
-  HBDIA_INFO_COUNT = 16
+  HBDIA_INFO_RUNS3_BLOCKS = 16, /* blocks given the runs-of-three hint (0 before prepareForGPU) */
+  HBDIA_INFO_COUNT = 17
 };
 
 typedef struct hbdia_matrix_opaque* hbdia_matrix_t;

@@ -122,6 +122,31 @@ int hbdia_hbdia_spmv_f32(const float* values, const int32_t* offsets, const int3
                          const int32_t* coo_group_ptr, const float* x, float* y,
                          int64_t num_rows, hbdia_stream_t stream);
 
+/* The same two products with a layout hint: runs3_blocks is a device bitset
+ * (bit b & 31 of word b >> 5) whose bit b says that block b's offsets are runs
+ * of three consecutive values (o-1, o, o+1) -- a multiple of three offsets,
+ * each triple consecutive -- as a 27-point stencil's are.  Such a block loads
+ * each run's x once and shifts it across lanes for the run's neighbours.  A
+ * set bit must be true of its block; clear bits are always safe; NULL means no
+ * hints.  Used when block_width % 64 == 0.  Results are bit-identical to the
+ * entry points above. */
+int hbdia_bdia_spmv_ex_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                           const double* x, double* y, int64_t num_rows, int32_t num_blocks, int32_t block_width,
+                           const uint32_t* runs3_blocks, hbdia_stream_t stream);
+int hbdia_bdia_spmv_ex_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                           const float* x, float* y, int64_t num_rows, int32_t num_blocks, int32_t block_width,
+                           const uint32_t* runs3_blocks, hbdia_stream_t stream);
+int hbdia_hbdia_spmv_ex_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const double* coo_values, const int32_t* coo_group_ptr,
+                            const double* x, double* y, int64_t num_rows, const uint32_t* runs3_blocks,
+                            hbdia_stream_t stream);
+int hbdia_hbdia_spmv_ex_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const float* coo_values, const int32_t* coo_group_ptr,
+                            const float* x, float* y, int64_t num_rows, const uint32_t* runs3_blocks,
+                            hbdia_stream_t stream);
+
 /* Multiple right-hand sides (no reference counterpart: the reference's
  * hbdiaSpMV takes one vector; this is the batched form of the same product).
  * For j < nrhs: Y[:, j] = A X[:, j], column j at X + j*ldx and Y + j*ldy

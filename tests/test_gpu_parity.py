@@ -265,3 +265,106 @@ def test_graph_capture_replays(gpu):
         g.launch(s)
         s.synchronize()
         np.testing.assert_array_equal(Y.download(), eager)
+
+
+# ------------------------------------------------------------ runs-of-three layout hint
+def _runs3_case(N, bw, seed, with_coo):
+    """A random layout whose blocks hold runs of three consecutive offsets (some partly or wholly
+    outside [0, N)), with zero values over NaN x entries."""
+    rng = np.random.default_rng(seed)
+    nb = -(-N // bw)
+    offs_per = []
+    for _ in range(nb):
+        k = int(rng.integers(0, 6))
+        centres = 3 * np.sort(rng.choice(np.arange(-N // 2, N // 2), size=k, replace=False)) + 1
+        offs_per.append([int(c + d) for c in centres for d in (-1, 0, 1)])
+    block_start = np.zeros(nb + 1, np.int32)
+    block_start[1:] = np.cumsum([len(o) for o in offs_per])
+    offsets = np.array(sum(offs_per, []), np.int32)
+    values = rng.standard_normal(len(offsets) * bw)
+    values[rng.random(values.size) < 0.2] = 0.0
+    x = rng.standard_normal(N)
+    x[rng.random(N) < 0.02] = np.nan
+    coo = None
+    if with_coo:
+        m = 3 * N // 2
+        r = np.sort(rng.integers(0, N, m)).astype(np.int32)
+        c = rng.integers(0, N, m).astype(np.int32)
+        coo = (r, c, rng.standard_normal(m))
+    return values, offsets, block_start, x, coo
+
+
+@pytest.mark.parametrize("N,bw,with_coo", [(1000, 64, False), (1000, 64, True), (4099, 128, True), (64, 64, False),
+                                           (777, 256, True)])
+def test_runs3_hint_is_bit_identical(gpu, N, bw, with_coo):
+    print(f"case: runs3 N={N} bw={bw} coo={with_coo}", flush=True)
+    values, offsets, block_start, x, coo = _runs3_case(N, bw, 31 + N + bw, with_coo)
+    L = _layout(values, offsets, block_start, N, bw, coo)
+    assert L.runs3 is not None
+    y0 = np.linspace(-1.0, 1.0, N)
+    X = DeviceArray.from_host(x)
+    outs = []
+    for hints in (False, True):
+        Y = DeviceArray.from_host(y0)
+        L.spmv(X, Y, hints=hints)
+        synchronize()
+        outs.append(Y.download())
+    np.testing.assert_array_equal(outs[1], outs[0])
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": len(block_start) - 1,
+          "block_width": bw}
+    if coo is not None:
+        hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
+    else:
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0))
+    want = oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N)
+    np.testing.assert_array_equal(outs[1], want)
+
+
+def test_stencil27_gets_the_runs3_hint_and_stays_exact(gpu):
+    A = HBDIA.stencil(20, 27, 0.01, seed=77)
+    A.convertToHBDIAFormat(64, 2)
+    # most blocks qualify; a block where two scatter entries reached the threshold on one offset does not
+    assert A.info()["runs3_blocks"] >= 0.9 * A.getNumBlocks()
+    B = HBDIA.stencil(20, 7, 0.0, seed=78)
+    B.convertToHBDIAFormat(64, 2)
+    assert B.info()["runs3_blocks"] == 0
+    N = 20 ** 3
+    x = gen_vector(N, 3)
+    X, Y = HBDIAVector(x), HBDIAVector(np.zeros(N))
+    hbdiaSpMV(A, X, Y, False, True)
+    A.synchronize()
+    hb = A.getHBDIA()
+    hb["num_blocks"], hb["block_width"] = A.getNumBlocks(), 64
+    np.testing.assert_array_equal(Y.download(), oracle.hbdia_spmv(hb, x, np.zeros(N), N))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_runs3_hint_on_mixed_layouts(gpu, seed):
+    # blocks whose runs are broken keep the per-offset path; the others take the shifted loads
+    print(f"case: runs3 hint on mixed layout seed={seed}", flush=True)
+    N, bw = 3000, 64
+    values, offsets, block_start, x, coo = _runs3_case(N, bw, 500 + seed, True)
+    rng = np.random.default_rng(seed)
+    keep = np.ones(len(offsets), bool)
+    keep[rng.choice(len(offsets), size=len(offsets) // 7, replace=False)] = False
+    nb = len(block_start) - 1
+    blk_of = np.repeat(np.arange(nb), np.diff(block_start))
+    offsets2 = offsets[keep]
+    block_start2 = np.zeros(nb + 1, np.int32)
+    block_start2[1:] = np.cumsum(np.bincount(blk_of[keep], minlength=nb))
+    values2 = values.reshape(-1, bw)[keep].ravel()
+    L = _layout(values2, offsets2, block_start2, N, bw, coo)
+    bits = kernels.runs3_bits(offsets2, block_start2, bw)
+    assert bits is not None and 0 < int(sum(bin(int(w)).count("1") for w in bits)) < nb
+    y0 = np.linspace(-1.0, 1.0, N)
+    X = DeviceArray.from_host(x)
+    outs = []
+    for hints in (False, True):
+        Y = DeviceArray.from_host(y0)
+        L.spmv(X, Y, hints=hints)
+        synchronize()
+        outs.append(Y.download())
+    np.testing.assert_array_equal(outs[1], outs[0])
+    hb = {"values": values2, "offsets": offsets2, "block_start": block_start2, "num_blocks": nb, "block_width": bw,
+          "coo_rows": coo[0], "coo_cols": coo[1], "coo_values": coo[2]}
+    np.testing.assert_array_equal(outs[1], oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N))
