@@ -45,7 +45,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", choices=["c2", "ref128", "precision", "multirhs", "batch", "serve"], default="c2")
+    p.add_argument("--workload", choices=["c2", "ref128", "precision", "multirhs", "batch", "serve", "copy"],
+                   default="c2")
     p.add_argument("--count", type=int, default=None, help="override the number of batched matrices (testing only)")
     p.add_argument("--grid", type=int, default=None, help="override the grid edge (testing only)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full-size parity check")
@@ -66,6 +67,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "copy":
+        return bench_copy(args)
 
     import hbdia_amd
     from hbdia_amd import HBDIA, Event, HBDIAVector, gen_vector, hbdiaSpMV
@@ -543,6 +546,31 @@ def bench_serve(args) -> int:
                        "pool": desc, "x_variants_per_matrix": variants, "loads": "30/60/90% of measured capacity",
                        "error_vs_fp64_oracle": "0 (bit-exact)"},
             "results": results}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_copy(args) -> int:
+    """Counter calibration: a 1 GiB device-to-device copy (hipMemcpyAsync on the null stream), the
+    plain copy whose FETCH_SIZE / WRITE_SIZE readings scale the SpMV kernels' counters to bytes."""
+    from hbdia_amd import DeviceArray, Event
+    from hbdia_amd._native import check_hip, hip
+    n = 1 << 27  # doubles: 1 GiB
+    src = DeviceArray(n, np.float64, zero=True)
+    dst = DeviceArray(n, np.float64, zero=True)
+    for _ in range(args.warmup):
+        check_hip(hip.hbdia_memcpy_d2d(dst.ptr, src.ptr, n * 8, None))
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(args.steps):
+        check_hip(hip.hbdia_memcpy_d2d(dst.ptr, src.ptr, n * 8, None))
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    line = {"metric": "device-to-device copy GB/s (read + write; counter calibration)", "value": round(2 * n * 8 / (ms * 1e-3) / 1e9, 1),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "zeros",
+            "config": {"workload": "copy_1GiB", "bytes_read": n * 8, "bytes_written": n * 8}}
     print(json.dumps(line), flush=True)
     return 0
 
