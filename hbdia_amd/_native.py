@@ -130,6 +130,7 @@ _CAPI_SIGS = {
     "hbdia_gen_stencil": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_banded": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_vector": (_i, [c_int, c_int64, c_uint64, c_double, c_double, _p]),
+    "hbdia_matrix_load_mtx": (_i, [_p, ctypes.c_char_p, c_int, c_int, POINTER(c_int)]),
     "hbdia_server_create": (_i, [POINTER(c_void_p), POINTER(c_void_p), c_int, c_int64, c_int, c_int, c_int, c_int,
                                  c_int]),
     "hbdia_server_destroy": (_i, [_p]),

@@ -149,6 +149,17 @@ class HBDIA:
         except HBDIAError as e:
             raise _map_error(e, "convert") from None
 
+    def loadMTX(self, filename: str, numRows: int | None = None, numCols: int | None = None) -> bool:
+        """Matrix Market input (reference HBDIA<T>::loadMTX): without dimensions the file's own, entries
+        sorted by (row, col); with them, indices checked (IndexError) and file order kept."""
+        ok = ctypes.c_int(0)
+        try:
+            check_capi(capi.hbdia_matrix_load_mtx(self._h, str(filename).encode(), int(numRows or 0),
+                                                  int(numCols or 0), byref(ok)))
+        except HBDIAError as e:
+            raise _map_error(e, "loadMTX") from None
+        return bool(ok.value)
+
     def removeCOODuplicates(self) -> None:
         check_capi(capi.hbdia_matrix_remove_duplicates(self._h))
 

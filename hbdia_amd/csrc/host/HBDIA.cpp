@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <iostream>
 #include <numeric>
 #include <random>
@@ -630,6 +632,158 @@ void HBDIA<T>::removeCOODuplicates() {
   colIndices = std::move(nc);
   values = std::move(nv);
   numNonZeros = static_cast<int>(values.size());
+}
+
+// ---------------------------------------------------------------------------
+// Matrix Market input (reference src/Format/HBDIA.cpp:161-370)
+
+namespace {
+
+// One entry line as the reference parses it (`is >> row >> col`, then
+// `is >> value`, 1 when that fails); lines without two integers do not count.
+template <typename T>
+bool parseMtxEntry(const std::string& line, int& row, int& col, T& value) {
+  std::istringstream is(line);
+  if (!(is >> row >> col)) return false;
+  if (!(is >> value)) value = T(1);
+  return true;
+}
+
+struct MtxFile {
+  bool ok = false, symmetric = false;
+  int rows = 0, cols = 0, nnz = 0;
+  std::vector<std::string> lines;  // the lines after the size line
+};
+
+MtxFile readMtx(const std::string& filename) {
+  MtxFile f;
+  std::ifstream file(filename);
+  if (!file.is_open()) {
+    std::cerr << "Error: Cannot open file " << filename << std::endl;
+    return f;
+  }
+  std::string line;
+  while (std::getline(file, line)) {
+    if (line[0] == '%') {
+      if (line.find("symmetric") != std::string::npos) f.symmetric = true;
+    } else {
+      break;  // the first non-comment line holds the sizes
+    }
+  }
+  std::istringstream header(line);
+  header >> f.rows >> f.cols >> f.nnz;
+  while (std::getline(file, line)) f.lines.push_back(std::move(line));
+  f.ok = true;
+  return f;
+}
+
+// Parses the entry lines in parallel and keeps, in file order, the first
+// `limit` lines holding two integers.
+template <typename T>
+void parseMtxEntries(const MtxFile& f, int limit, std::vector<int>& r, std::vector<int>& c, std::vector<T>& v,
+                     std::vector<uint8_t>& good) {
+  const int64_t n = static_cast<int64_t>(f.lines.size());
+  r.assign(n, 0);
+  c.assign(n, 0);
+  v.assign(n, T(0));
+  good.assign(n, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t i = 0; i < n; ++i) good[i] = parseMtxEntry<T>(f.lines[i], r[i], c[i], v[i]);
+  int taken = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!good[i]) continue;
+    if (taken >= limit) good[i] = 0;
+    else ++taken;
+  }
+}
+
+}  // namespace
+
+template <typename T>
+bool HBDIA<T>::loadMTX(const std::string& filename) {
+  MtxFile f = readMtx(filename);
+  if (!f.ok) return false;
+  if (f.symmetric && verbose()) std::cout << "Detected symmetric matrix - will expand to full matrix" << std::endl;
+  numRows = f.rows;
+  numCols = f.cols;
+  std::vector<int> r, c;
+  std::vector<T> v;
+  std::vector<uint8_t> good;
+  parseMtxEntries<T>(f, f.nnz, r, c, v, good);
+  int original = 0;  // entries are appended to any already held, as the reference does
+  for (size_t i = 0; i < good.size(); ++i) {
+    if (!good[i]) continue;
+    const int row = r[i] - 1, col = c[i] - 1;  // 1-based in the file
+    rowIndices.push_back(row);
+    colIndices.push_back(col);
+    values.push_back(v[i]);
+    ++original;
+    if (f.symmetric && row != col) {
+      rowIndices.push_back(col);
+      colIndices.push_back(row);
+      values.push_back(v[i]);
+    }
+  }
+  numNonZeros = static_cast<int>(values.size());
+  hasCOO = true;
+  // sort by (row, col) with the reference's call: std::sort on an index array
+  // (its order of equal keys is libstdc++'s, so it is reproduced exactly)
+  std::vector<size_t> indices(values.size());
+  std::iota(indices.begin(), indices.end(), size_t{0});
+  std::sort(indices.begin(), indices.end(), [&](size_t a, size_t b) {
+    if (rowIndices[a] != rowIndices[b]) return rowIndices[a] < rowIndices[b];
+    return colIndices[a] < colIndices[b];
+  });
+  std::vector<T> sv(values.size());
+  std::vector<int> sr(values.size()), sc(values.size());
+  for (size_t i = 0; i < indices.size(); ++i) {
+    sv[i] = values[indices[i]];
+    sr[i] = rowIndices[indices[i]];
+    sc[i] = colIndices[indices[i]];
+  }
+  values = std::move(sv);
+  rowIndices = std::move(sr);
+  colIndices = std::move(sc);
+  return original > 0;
+}
+
+template <typename T>
+bool HBDIA<T>::loadMTX(const std::string& filename, int nRows, int nCols) {
+  if (nRows <= 0 || nCols <= 0) {
+    std::cerr << "Error: Matrix dimensions must be positive" << std::endl;
+    return false;
+  }
+  MtxFile f = readMtx(filename);
+  if (!f.ok) return false;
+  numRows = nRows;
+  numCols = nCols;
+  std::vector<int> r, c;
+  std::vector<T> v;
+  std::vector<uint8_t> good;
+  parseMtxEntries<T>(f, f.nnz, r, c, v, good);
+  int original = 0;
+  for (size_t i = 0; i < good.size(); ++i) {
+    if (!good[i]) continue;
+    const int row = r[i] - 1, col = c[i] - 1;
+    if (row < 0 || row >= nRows)
+      throw std::out_of_range("Row index " + std::to_string(row) + " from file is out of range [0, " +
+                              std::to_string(nRows - 1) + "]");
+    if (col < 0 || col >= nCols)
+      throw std::out_of_range("Column index " + std::to_string(col) + " from file is out of range [0, " +
+                              std::to_string(nCols - 1) + "]");
+    rowIndices.push_back(row);
+    colIndices.push_back(col);
+    values.push_back(v[i]);
+    ++original;
+    if (f.symmetric && row != col) {
+      rowIndices.push_back(col);
+      colIndices.push_back(row);
+      values.push_back(v[i]);
+    }
+  }
+  numNonZeros = static_cast<int>(values.size());
+  hasCOO = true;
+  return original > 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -198,7 +198,13 @@ bool hbdiaSpMVMulti(HBDIA<T>& matrix, const T* X, int64_t ldx, T* Y, int64_t ldy
   if (hasCOOWork && !execCOOGPU) throw std::runtime_error("Invalid COO execution flags");
   if (hasCOOWork && !matrix.hasDeviceCOO())
     throw std::runtime_error("hbdiaSpMVMulti: the fallback is not on the device (convert with ExecutionMode::GPU_COO)");
-  CHECK_HBDIA(Dev<T>::multi(matrix, hasCOOWork, X, ldx, Y, ldy, nrhs, matrix.getBDIAStream()));
+  if (nrhs == 1) {
+    // one column: the single-vector kernel (same arithmetic, and its runs-of-three x loads)
+    if (hasCOOWork) CHECK_HBDIA(Dev<T>::fused(matrix, X, Y, matrix.getBDIAStream()));
+    else CHECK_HBDIA(Dev<T>::bdia(matrix, X, Y, matrix.getBDIAStream()));
+  } else {
+    CHECK_HBDIA(Dev<T>::multi(matrix, hasCOOWork, X, ldx, Y, ldy, nrhs, matrix.getBDIAStream()));
+  }
   CHECK_HBDIA(hbdia_event_record(matrix.getBDIAEvent(), matrix.getBDIAStream()));
   return true;
 }

@@ -237,6 +237,18 @@ int hbdia_matrix_remove_duplicates(hbdia_matrix_t m) {
   return guard("hbdia_matrix_remove_duplicates", [&] { DISPATCH(m, M.removeCOODuplicates(), M.removeCOODuplicates()); });
 }
 
+int hbdia_matrix_load_mtx(hbdia_matrix_t m, const char* filename, int num_rows, int num_cols, int* loaded) {
+  return guard("hbdia_matrix_load_mtx", [&] {
+    need(m != nullptr && filename != nullptr, "null argument");
+    bool ok;
+    auto load = [&](auto& A) {
+      return (num_rows <= 0 && num_cols <= 0) ? A.loadMTX(filename) : A.loadMTX(filename, num_rows, num_cols);
+    };
+    ok = m->dtype == HBDIA_DTYPE_F64 ? load(*m->d) : load(*m->f);
+    if (loaded) *loaded = ok ? 1 : 0;
+  });
+}
+
 int hbdia_matrix_prepare_gpu(hbdia_matrix_t m) {
   return guard("hbdia_matrix_prepare_gpu", [&] { DISPATCH(m, M.prepareForGPU(), M.prepareForGPU()); });
 }

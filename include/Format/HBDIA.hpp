@@ -93,6 +93,15 @@ class HBDIA {
   // Sums duplicate (row, col) entries and leaves the COO sorted by (row, col).
   void removeCOODuplicates();
 
+  // Matrix Market input (reference src/Format/HBDIA.cpp:161-370).  The first
+  // form takes the dimensions from the file and sorts the entries by
+  // (row, col); the second uses the given dimensions, checks every index
+  // against them (std::out_of_range) and keeps file order.  "symmetric" in a
+  // header comment mirrors every off-diagonal entry (right after it); pattern
+  // files get value 1.  Returns whether any entry was read.
+  bool loadMTX(const std::string& filename);
+  bool loadMTX(const std::string& filename, int numRows, int numCols);
+
   // dimensions
   int getNumRows() const { return numRows; }
   int getNumCols() const { return numCols; }

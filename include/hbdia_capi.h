@@ -12,6 +12,7 @@
  *   hbdia_matrix_stencil27             -> HBDIA<T>::create3DStencil27Point, HBDIA.hpp:45
  *   hbdia_matrix_convert[_host]        -> HBDIA<T>::convertToHBDIAFormat, HBDIA.hpp:52-54
  *   hbdia_matrix_remove_duplicates     -> HBDIA<T>::removeCOODuplicates, HBDIA.hpp:71
+ *   hbdia_matrix_load_mtx              -> HBDIA<T>::loadMTX, HBDIA.hpp:43-44
  *   hbdia_matrix_info / _get_*         -> the getters of HBDIA.hpp:74-110
  *   hbdia_vector_create                -> HBDIAVector<T>(localData[, four memory flags]),
  *                                         include/Format/HBDIAVector.hpp:31-32
@@ -77,6 +78,10 @@ int hbdia_matrix_convert(hbdia_matrix_t m, int block_width, int threshold, int m
 int hbdia_matrix_convert_host(hbdia_matrix_t m, int block_width, int threshold, int max_coo_entries,
                               int coo_is_unique, int gpu_coo);
 int hbdia_matrix_remove_duplicates(hbdia_matrix_t m);
+/* loadMTX: num_rows, num_cols <= 0 take the file's dimensions (and sort by
+ * (row, col)); otherwise the given ones, with index checks.  *loaded = the
+ * reference's return value (any entry read). */
+int hbdia_matrix_load_mtx(hbdia_matrix_t m, const char* filename, int num_rows, int num_cols, int* loaded);
 int hbdia_matrix_prepare_gpu(hbdia_matrix_t m);
 int hbdia_matrix_synchronize(hbdia_matrix_t m);
 int hbdia_matrix_info(hbdia_matrix_t m, int64_t* info /* [HBDIA_INFO_COUNT] */);
