@@ -45,7 +45,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", choices=["c2", "ref128", "precision", "multirhs", "batch", "serve", "copy"],
+    p.add_argument("--workload", choices=["c2", "ref128", "precision", "multirhs", "batch", "serve", "copy", "convert"],
                    default="c2")
     p.add_argument("--count", type=int, default=None, help="override the number of batched matrices (testing only)")
     p.add_argument("--grid", type=int, default=None, help="override the grid edge (testing only)")
@@ -69,6 +69,8 @@ def main() -> int:
         return bench_serve(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "convert":
+        return bench_convert(args)
 
     import hbdia_amd
     from hbdia_amd import HBDIA, Event, HBDIAVector, gen_vector, hbdiaSpMV
@@ -571,6 +573,79 @@ def bench_copy(args) -> int:
             "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "zeros",
             "config": {"workload": "copy_1GiB", "bytes_read": n * 8, "bytes_written": n * 8}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_convert(args) -> int:
+    """COO -> HBDIA conversion of the 256^3 stencil + scatter matrix (453M entries, block width 64,
+    threshold 2): the device conversion with the COO already in HBM (hbdia_convert_coo, every result
+    array left in HBM), the whole convertToHBDIAFormatOnDevice (COO upload, device conversion, host
+    copies of the result), and the host conversion + upload (convertToHBDIAFormat); the two methods
+    must leave the same host arrays.  value = entries per second of the device conversion."""
+    import ctypes
+
+    import hbdia_amd
+    from hbdia_amd import HBDIA, DeviceArray
+    from hbdia_amd import kernels as K
+    from hbdia_amd._native import check_hip, hip
+
+    if not hbdia_amd.gpu_available():
+        print("no HIP device visible", file=sys.stderr)
+        return 3
+    n = args.grid or 256
+    bw, thr, seed = 64, 2, 0xC2
+    t0 = time.time()
+    A = HBDIA.stencil(n, 27, 0.005, seed=seed)
+    r, c, v = A.getCOO()
+    nnz = len(v)
+    log(f"generated {n}^3 stencil + scatter ({nnz} entries) in {time.time() - t0:.1f}s")
+
+    R, C, V = DeviceArray.from_host(r), DeviceArray.from_host(c), DeviceArray.from_host(v)
+    cargs = K.ConvertArgs(R.ptr, C.ptr, V.ptr, None, nnz, n ** 3, n ** 3, bw, thr, 1_000_000_000, 8, 1, 1)
+    dev_s = []
+    for k in range(max(1, args.warmup) + max(1, args.steps)):
+        res = K.ConvertResult()
+        t = time.perf_counter()
+        check_hip(hip.hbdia_convert_coo(ctypes.byref(cargs), ctypes.byref(res), None))
+        dt = time.perf_counter() - t
+        pairs = res.num_pairs
+        check_hip(hip.hbdia_convert_release(ctypes.byref(res)))
+        if k >= max(1, args.warmup):
+            dev_s.append(dt)
+        log(f"device conversion {dt * 1e3:.1f} ms ({pairs} pairs)")
+    del R, C, V
+
+    method_s, host_s = [], []
+    reps = 2
+    for k in range(reps):
+        A.deleteHBDIAFormat() if A.isHBDIAFormat() else None
+        t = time.perf_counter()
+        A.convertToHBDIAFormatOnDevice(bw, thr)
+        method_s.append(time.perf_counter() - t)
+        log(f"convertToHBDIAFormatOnDevice {method_s[-1]:.2f} s")
+    dev_state = A.getHBDIA()
+    for k in range(reps):
+        A.deleteHBDIAFormat()
+        t = time.perf_counter()
+        A.convertToHBDIAFormat(bw, thr)
+        host_s.append(time.perf_counter() - t)
+        log(f"convertToHBDIAFormat (host + upload) {host_s[-1]:.2f} s")
+    host_state = A.getHBDIA()
+    same = all(np.array_equal(dev_state[k], host_state[k]) for k in host_state)
+    if not same:
+        print("device and host conversions differ", file=sys.stderr)
+        return 1
+    ms = min(dev_s) * 1e3
+    line = {"metric": "COO->HBDIA conversion entries/s (device, COO resident in HBM)",
+            "value": round(nnz / (ms * 1e-3), 1), "unit": "entries/s", "n_gpus": 1, "steps": len(dev_s),
+            "warmup": max(1, args.warmup), "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int32 keys, f64 values", "data": "synthetic",
+            "config": {"workload": f"convert_stencil27_scatter_{n}^3", "entries": nnz, "block_width": bw,
+                       "threshold": thr, "pairs": int(pairs)},
+            "method_s": round(min(method_s), 3), "host_convert_upload_s": round(min(host_s), 3),
+            "host_threads": hbdia_amd.api.capi.hbdia_get_num_threads(),
+            "host_over_method": round(min(host_s) / min(method_s), 2), "identical": same}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -95,6 +95,8 @@ _HIP_SIGS = {
     "hbdia_coo_group_ptr": (_i, [_p, c_int64, c_int64, _p, _p]),
     "hbdia_vector_add_f64": (_i, [_p, _p, c_int64, _p]),
     "hbdia_vector_add_f32": (_i, [_p, _p, c_int64, _p]),
+    "hbdia_convert_coo": (_i, [_p, _p, _p]),
+    "hbdia_convert_release": (_i, [_p]),
 }
 
 _CAPI_SIGS = {
@@ -107,7 +109,9 @@ _CAPI_SIGS = {
     "hbdia_matrix_stencil27": (_i, [_p, c_int, c_int, c_int, c_double, c_int]),
     "hbdia_matrix_convert": (_i, [_p, c_int, c_int, c_int, c_int, c_int]),
     "hbdia_matrix_convert_host": (_i, [_p, c_int, c_int, c_int, c_int, c_int]),
+    "hbdia_matrix_convert_device": (_i, [_p, c_int, c_int, c_int, c_int, c_int]),
     "hbdia_matrix_remove_duplicates": (_i, [_p]),
+    "hbdia_matrix_delete_hbdia": (_i, [_p]),
     "hbdia_matrix_prepare_gpu": (_i, [_p]),
     "hbdia_matrix_synchronize": (_i, [_p]),
     "hbdia_matrix_info": (_i, [_p, _p]),

@@ -149,6 +149,19 @@ class HBDIA:
         except HBDIAError as e:
             raise _map_error(e, "convert") from None
 
+    def convertToHBDIAFormatOnDevice(self, blockWidth: int = DEFAULT_BLOCK_WIDTH,
+                                     threshold: int = DEFAULT_THRESHOLD, max_coo_entries: int = MAX_COO_ENTRIES,
+                                     COOisUnique: bool = False,
+                                     execMode: ExecutionMode = ExecutionMode.GPU_COO) -> None:
+        """The conversion computed on the GPU: the host state of convertToHBDIAFormatOnHost, bit for bit,
+        and the device arrays of prepareForGPU built in place."""
+        try:
+            check_capi(capi.hbdia_matrix_convert_device(self._h, int(blockWidth), int(threshold),
+                                                        int(max_coo_entries), int(bool(COOisUnique)),
+                                                        int(execMode == ExecutionMode.GPU_COO)))
+        except HBDIAError as e:
+            raise _map_error(e, "convert") from None
+
     def loadMTX(self, filename: str, numRows: int | None = None, numCols: int | None = None) -> bool:
         """Matrix Market input (reference HBDIA<T>::loadMTX): without dimensions the file's own, entries
         sorted by (row, col); with them, indices checked (IndexError) and file order kept."""
@@ -159,6 +172,10 @@ class HBDIA:
         except HBDIAError as e:
             raise _map_error(e, "loadMTX") from None
         return bool(ok.value)
+
+    def deleteHBDIAFormat(self) -> None:
+        """Drops the blocked format, host and device; the COO stays for another conversion."""
+        check_capi(capi.hbdia_matrix_delete_hbdia(self._h))
 
     def removeCOODuplicates(self) -> None:
         check_capi(capi.hbdia_matrix_remove_duplicates(self._h))

@@ -28,8 +28,9 @@
 #include <string>
 
 #include "hbdia_hip.h"
+#include "hbdia_internal.hpp"
 
-namespace {
+namespace hbdia_detail {
 
 thread_local std::string g_last_error;
 
@@ -44,6 +45,14 @@ int check_hip(hipError_t e, const char* what) {
   if (e == hipErrorOutOfMemory) return fail(HBDIA_ERROR_OUT_OF_MEMORY, m);
   return fail(HBDIA_ERROR_DEVICE, m);
 }
+
+}  // namespace hbdia_detail
+
+namespace {
+
+using hbdia_detail::check_hip;
+using hbdia_detail::fail;
+using hbdia_detail::g_last_error;
 
 inline hipStream_t hs(hbdia_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 

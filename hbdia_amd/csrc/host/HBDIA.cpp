@@ -72,6 +72,42 @@ void dev_free(U*& p) noexcept {
   p = nullptr;
 }
 
+// cpuRowPtr by the reference's gap-filling rule (src/Format/HBDIA.cpp:676-708):
+// on each change of row r, every slot from the previous row + 1 through r gets
+// the current entry index; the tail gets the entry count.  For row-sorted
+// entries this is the CSR row pointer.
+void fill_cpu_row_ptr(const std::vector<int>& rows, int numRows, std::vector<int>& ptr) {
+  ptr.assign(static_cast<size_t>(numRows) + 1, 0);
+  int currentRow = -1, nextOffset = 0;
+  for (size_t q = 0; q < rows.size(); ++q) {
+    const int r = rows[q];
+    if (r != currentRow) {
+      for (int z = currentRow + 1; z <= r; z++) ptr[z] = nextOffset;
+      currentRow = r;
+    }
+    nextOffset++;
+  }
+  for (int z = currentRow + 1; z <= numRows; z++) ptr[z] = nextOffset;
+}
+
+// Owns the device arrays of one hbdia_convert_coo result until they are adopted.
+struct ConvertResult {
+  hbdia_convert_result_t r{};
+  ~ConvertResult() { (void)hbdia_convert_release(&r); }
+  template <typename P>
+  P* take(P*& field) {
+    P* p = field;
+    field = nullptr;
+    return p;
+  }
+};
+
+template <typename U>
+void dev_download(std::vector<U>& dst, const void* src, size_t count) {
+  dst.resize(count);
+  if (count) CHECK_HBDIA(hbdia_memcpy(dst.data(), src, count * sizeof(U)));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -574,25 +610,147 @@ bool HBDIA<T>::convertToHBDIAFormatOnHost(int bw, int thr, int maxCoo, bool /*CO
     }
   }
 
-  // 8. cpuRowPtr with the reference's gap-filling rule (HBDIA.cpp:676-708)
-  cpuRowPtr.assign(static_cast<size_t>(numRows) + 1, 0);
-  {
-    int currentRow = -1, nextOffset = 0;
-    for (size_t q = 0; q < cpuRowIndices.size(); ++q) {
-      const int r = cpuRowIndices[q];
-      if (r != currentRow) {
-        for (int z = currentRow + 1; z <= r; z++) cpuRowPtr[z] = nextOffset;
-        currentRow = r;
-      }
-      nextOffset++;
-    }
-    for (int z = currentRow + 1; z <= numRows; z++) cpuRowPtr[z] = nextOffset;
-  }
+  // 8. cpuRowPtr with the reference's gap-filling rule
+  fill_cpu_row_ptr(cpuRowIndices, numRows, cpuRowPtr);
 
   if (verbose())
     std::cout << "HBDIA conversion complete: GPU storage " << (nnz - (int64_t)cpuValues.size())
               << " entries, CPU fallback " << cpuValues.size() << " entries" << std::endl;
   hasHBDIA = true;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// conversion on the device: the same rules and results as the host conversion
+// above (reference src/Format/HBDIA.cpp:507-718), computed by hbdia_convert_coo.
+
+template <typename T>
+bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*COOisUnique*/,
+                                            ExecutionMode execMode) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "values are moved as 4- or 8-byte words");
+  if (hasHBDIA) {
+    std::cout << "Matrix is already in HBDIA format" << std::endl;
+    return false;
+  }
+  if (!hasCOO) {
+    std::cout << "No COO format available for conversion to HBDIA" << std::endl;
+    return false;
+  }
+  if (values.empty()) {
+    std::cout << "Cannot convert empty matrix to HBDIA format" << std::endl;
+    return false;
+  }
+  if (bw <= 0) throw std::invalid_argument("blockWidth must be positive");
+  if (values.size() > static_cast<size_t>(0x7fffffff))
+    throw std::runtime_error("device conversion takes at most 2^31-1 entries");
+  const bool useGlobal = partialMatrix && !globalRowMapping.empty();
+
+  cleanupGPUData();
+  initializeStreams();
+  int *R = nullptr, *C = nullptr, *G = nullptr;
+  T* V = nullptr;
+  struct Inputs {
+    int **r, **c, **g;
+    T** v;
+    ~Inputs() {
+      dev_free(*r);
+      dev_free(*c);
+      dev_free(*g);
+      dev_free(*v);
+    }
+  } inputs{&R, &C, &G, &V};
+  dev_upload(&R, rowIndices.data(), rowIndices.size());
+  dev_upload(&C, colIndices.data(), colIndices.size());
+  dev_upload(&V, values.data(), values.size());
+  if (useGlobal) dev_upload(&G, globalRowMapping.data(), globalRowMapping.size());
+
+  hbdia_convert_args_t a{};
+  a.rows = R;
+  a.cols = C;
+  a.values = V;
+  a.global_rows = G;
+  a.nnz = static_cast<int64_t>(values.size());
+  a.num_rows = numRows;
+  a.num_cols = numCols;
+  a.block_width = bw;
+  a.threshold = thr;
+  a.max_coo_entries = maxCoo;
+  a.elem_size = static_cast<int32_t>(sizeof(T));
+  a.want_row_sorted_coo = execMode == ExecutionMode::GPU_COO ? 1 : 0;
+  a.want_runs3 = 1;
+  ConvertResult res;
+  CHECK_HBDIA(hbdia_convert_coo(&a, &res.r, sBDIA_));
+  const hbdia_convert_result_t& r = res.r;
+  const int nb = r.num_blocks;
+
+  // host state, as the host conversion leaves it
+  std::vector<int> flat, starts, placed;
+  dev_download(flat, r.offsets, static_cast<size_t>(r.num_pairs));
+  dev_download(starts, r.block_start, static_cast<size_t>(nb) + 1);
+  dev_download(placed, r.block_placed, static_cast<size_t>(nb));
+  std::vector<T> data, cooV;
+  std::vector<int> cooR, cooC;
+  dev_download(data, r.values, static_cast<size_t>(r.num_pairs) * static_cast<size_t>(bw));
+  dev_download(cooR, r.coo_rows, static_cast<size_t>(r.num_coo));
+  dev_download(cooC, r.coo_cols, static_cast<size_t>(r.num_coo));
+  dev_download(cooV, r.coo_values, static_cast<size_t>(r.num_coo));
+
+  blockWidth = bw;
+  threshold = thr;
+  max_coo_entries = maxCoo;
+  executionMode = execMode;
+  numberDiagonals = r.number_diagonals;
+  hbdiaData = std::move(data);
+  offsetsPerBlock.assign(nb, {});
+  ptrToBlock.assign(nb, nullptr);
+  int maxDiag = 0;
+  for (int b = 0; b < nb; ++b) {
+    offsetsPerBlock[b].assign(flat.begin() + starts[b], flat.begin() + starts[b + 1]);
+    if (starts[b + 1] > starts[b]) ptrToBlock[b] = hbdiaData.data() + static_cast<size_t>(starts[b]) * bw;
+    maxDiag = std::max(maxDiag, starts[b + 1] - starts[b]);
+  }
+  histogramBlocks.assign(maxDiag + 1, 0);
+  histogramNnz.assign(maxDiag + 1, 0);
+  for (int b = 0; b < nb; ++b) {
+    const int k = starts[b + 1] - starts[b];
+    histogramBlocks[k]++;
+    histogramNnz[k] += k ? placed[b] : 0;
+  }
+  cpuRowIndices = std::move(cooR);
+  cpuColIndices = std::move(cooC);
+  cpuValues = std::move(cooV);
+  fill_cpu_row_ptr(cpuRowIndices, numRows, cpuRowPtr);
+  hasHBDIA = true;
+  if (partialMatrix) {  // the reference uploads partial matrices later, with their halo layout
+    cleanupGPUData();
+    return true;
+  }
+
+  // device state, as prepareForGPU leaves it
+  hbdiaData_d_ = static_cast<T*>(res.take(res.r.values));
+  flattenedOffsets_d_ = res.take(res.r.offsets);
+  blockStartIndices_d_ = res.take(res.r.block_start);
+  blockSizes_d_ = res.take(res.r.block_sizes);
+  runs3Blocks_d_ = res.take(res.r.runs3_blocks);
+  runs3Count_ = r.runs3_count;
+  if (executionMode == ExecutionMode::GPU_COO && r.num_coo > 0) {
+    if (r.coo_sorted) {
+      cpuRowIndices_d_ = res.take(res.r.coo_rows);
+      cpuColIndices_d_ = res.take(res.r.coo_cols);
+      cpuValues_d_ = static_cast<T*>(res.take(res.r.coo_values));
+    } else {
+      cpuRowIndices_d_ = res.take(res.r.coo_rows_sorted);
+      cpuColIndices_d_ = res.take(res.r.coo_cols_sorted);
+      cpuValues_d_ = static_cast<T*>(res.take(res.r.coo_values_sorted));
+    }
+    dev_upload(&cpuRowPtr_d_, cpuRowPtr.data(), cpuRowPtr.size());
+    const int64_t groups = (static_cast<int64_t>(numRows) + HBDIA_ROW_GROUP - 1) / HBDIA_ROW_GROUP;
+    void* p = nullptr;
+    CHECK_HBDIA(hbdia_malloc(&p, static_cast<size_t>(groups + 1) * sizeof(int)));
+    cooGroupPtr_d_ = static_cast<int*>(p);
+    CHECK_HBDIA(hbdia_coo_group_ptr(cpuRowIndices_d_, r.num_coo, numRows, cooGroupPtr_d_, sBDIA_));
+    CHECK_HBDIA(hbdia_stream_synchronize(sBDIA_));
+  }
   return true;
 }
 

@@ -233,6 +233,14 @@ int hbdia_matrix_convert_host(hbdia_matrix_t m, int bw, int thr, int max_coo, in
   });
 }
 
+int hbdia_matrix_convert_device(hbdia_matrix_t m, int bw, int thr, int max_coo, int coo_unique, int gpu_coo) {
+  return guard("hbdia_matrix_convert_device", [&] {
+    const ExecutionMode em = gpu_coo ? ExecutionMode::GPU_COO : ExecutionMode::CPU_COO;
+    DISPATCH(m, M.convertToHBDIAFormatOnDevice(bw, thr, max_coo, coo_unique != 0, em),
+             M.convertToHBDIAFormatOnDevice(bw, thr, max_coo, coo_unique != 0, em));
+  });
+}
+
 int hbdia_matrix_remove_duplicates(hbdia_matrix_t m) {
   return guard("hbdia_matrix_remove_duplicates", [&] { DISPATCH(m, M.removeCOODuplicates(), M.removeCOODuplicates()); });
 }
@@ -247,6 +255,10 @@ int hbdia_matrix_load_mtx(hbdia_matrix_t m, const char* filename, int num_rows, 
     ok = m->dtype == HBDIA_DTYPE_F64 ? load(*m->d) : load(*m->f);
     if (loaded) *loaded = ok ? 1 : 0;
   });
+}
+
+int hbdia_matrix_delete_hbdia(hbdia_matrix_t m) {
+  return guard("hbdia_matrix_delete_hbdia", [&] { DISPATCH(m, M.deleteHBDIAFormat(), M.deleteHBDIAFormat()); });
 }
 
 int hbdia_matrix_prepare_gpu(hbdia_matrix_t m) {

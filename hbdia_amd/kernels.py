@@ -6,6 +6,8 @@ on `stream` (None = null stream).
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from ._native import check_hip, hip
@@ -109,6 +111,65 @@ def coo_group_ptr(coo_rows: DeviceArray, nnz: int, num_rows: int, out: DeviceArr
 def vector_add(dst: DeviceArray, src: DeviceArray, n: int, stream: Stream | None = None) -> None:
     fn = getattr(hip, "hbdia_vector_add_" + _sfx(dst.dtype))
     check_hip(fn(dst.ptr, src.ptr, int(n), _h(stream)))
+
+
+class ConvertArgs(ctypes.Structure):
+    """hbdia_convert_args_t of include/hbdia_hip.h."""
+    _fields_ = [("rows", ctypes.c_void_p), ("cols", ctypes.c_void_p), ("values", ctypes.c_void_p),
+                ("global_rows", ctypes.c_void_p), ("nnz", ctypes.c_int64), ("num_rows", ctypes.c_int32),
+                ("num_cols", ctypes.c_int32), ("block_width", ctypes.c_int32), ("threshold", ctypes.c_int32),
+                ("max_coo_entries", ctypes.c_int32), ("elem_size", ctypes.c_int32),
+                ("want_row_sorted_coo", ctypes.c_int32), ("want_runs3", ctypes.c_int32)]
+
+
+class ConvertResult(ctypes.Structure):
+    """hbdia_convert_result_t of include/hbdia_hip.h."""
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "values", "offsets", "block_start", "block_sizes", "block_placed", "runs3_blocks", "coo_rows", "coo_cols",
+        "coo_values", "coo_rows_sorted", "coo_cols_sorted", "coo_values_sorted")] + [
+        ("num_pairs", ctypes.c_int64), ("num_coo", ctypes.c_int64), ("num_blocks", ctypes.c_int32),
+        ("number_diagonals", ctypes.c_int32), ("runs3_count", ctypes.c_int32), ("coo_sorted", ctypes.c_int32)]
+
+
+def _fetch(ptr, count: int, dtype) -> np.ndarray:
+    out = np.empty(int(count), dtype)
+    if out.nbytes:
+        check_hip(hip.hbdia_memcpy(out.ctypes.data, ptr, out.nbytes))
+    return out
+
+
+def convert_coo(rows: DeviceArray, cols: DeviceArray, values: DeviceArray, num_rows: int, num_cols: int,
+                block_width: int, threshold: int, max_coo_entries: int = 1_000_000_000,
+                global_rows: DeviceArray | None = None, row_sorted_coo: bool = True, runs3: bool = True,
+                stream: Stream | None = None) -> dict:
+    """COO -> HBDIA on the device (hbdia_convert_coo); returns every result array downloaded to the host
+    (None where the result holds NULL) and the counts, and releases the device arrays."""
+    a = ConvertArgs(rows.ptr, cols.ptr, values.ptr, global_rows.ptr if global_rows is not None else None,
+                    int(values.size), int(num_rows), int(num_cols), int(block_width), int(threshold),
+                    int(max_coo_entries), values.dtype.itemsize, int(bool(row_sorted_coo)), int(bool(runs3)))
+    r = ConvertResult()
+    check_hip(hip.hbdia_convert_coo(ctypes.byref(a), ctypes.byref(r), _h(stream)))
+    try:
+        nb, pairs, ncoo, vt = r.num_blocks, r.num_pairs, r.num_coo, values.dtype
+        get = lambda ptr, n, dt: _fetch(ptr, n, dt) if ptr else None  # noqa: E731
+        return {
+            "values": get(r.values, pairs * block_width, vt) if pairs else np.empty(0, vt),
+            "offsets": get(r.offsets, pairs + OFFSETS_PAD, np.int32),
+            "block_start": get(r.block_start, nb + 1, np.int32),
+            "block_sizes": get(r.block_sizes, nb + 1, np.int32),
+            "block_placed": get(r.block_placed, nb, np.int32) if nb else np.empty(0, np.int32),
+            "runs3_blocks": get(r.runs3_blocks, (nb + 31) // 32, np.uint32),
+            "coo_rows": get(r.coo_rows, ncoo, np.int32) if ncoo else np.empty(0, np.int32),
+            "coo_cols": get(r.coo_cols, ncoo, np.int32) if ncoo else np.empty(0, np.int32),
+            "coo_values": get(r.coo_values, ncoo, vt) if ncoo else np.empty(0, vt),
+            "coo_rows_sorted": get(r.coo_rows_sorted, ncoo, np.int32),
+            "coo_cols_sorted": get(r.coo_cols_sorted, ncoo, np.int32),
+            "coo_values_sorted": get(r.coo_values_sorted, ncoo, vt),
+            "num_pairs": pairs, "num_coo": ncoo, "num_blocks": nb, "number_diagonals": r.number_diagonals,
+            "runs3_count": r.runs3_count, "coo_sorted": bool(r.coo_sorted),
+        }
+    finally:
+        check_hip(hip.hbdia_convert_release(ctypes.byref(r)))
 
 
 class DeviceLayout:

@@ -79,6 +79,15 @@ class HBDIA {
   bool convertToHBDIAFormatOnHost(int blockWidth = DEFAULT_BLOCK_WIDTH, int threshold = DEFAULT_THRESHOLD,
                                   int max_coo_entries = MAX_COO_ENTRIES, bool COOisUnique = false,
                                   ExecutionMode execMode = ExecutionMode::GPU_COO);
+  // convertToHBDIAFormat computed on the GPU (sort by (block, offset), run
+  // counts, a scan for the threshold rule, scatter): the same host state as
+  // convertToHBDIAFormatOnHost, bit for bit, and -- unless the matrix is
+  // partial -- the device arrays of prepareForGPU built in place on the GPU
+  // rather than uploaded.  The COO is uploaded once.  Same return value and
+  // messages as the host conversion; throws hbdia::DeviceError on device failures.
+  bool convertToHBDIAFormatOnDevice(int blockWidth = DEFAULT_BLOCK_WIDTH, int threshold = DEFAULT_THRESHOLD,
+                                    int max_coo_entries = MAX_COO_ENTRIES, bool COOisUnique = false,
+                                    ExecutionMode execMode = ExecutionMode::GPU_COO);
 
   // print wrappers (short textual dumps)
   void print() const;

@@ -10,8 +10,9 @@
  * checkout):
  *   hbdia_matrix_create / _create_coo  -> HBDIA<T> constructors, include/Format/HBDIA.hpp:31-39
  *   hbdia_matrix_stencil27             -> HBDIA<T>::create3DStencil27Point, HBDIA.hpp:45
- *   hbdia_matrix_convert[_host]        -> HBDIA<T>::convertToHBDIAFormat, HBDIA.hpp:52-54
+ *   hbdia_matrix_convert[_host|_device] -> HBDIA<T>::convertToHBDIAFormat, HBDIA.hpp:52-54
  *   hbdia_matrix_remove_duplicates     -> HBDIA<T>::removeCOODuplicates, HBDIA.hpp:71
+ *   hbdia_matrix_delete_hbdia          -> HBDIA<T>::deleteHBDIAFormat, HBDIA.hpp:67
  *   hbdia_matrix_load_mtx              -> HBDIA<T>::loadMTX, HBDIA.hpp:43-44
  *   hbdia_matrix_info / _get_*         -> the getters of HBDIA.hpp:74-110
  *   hbdia_vector_create                -> HBDIAVector<T>(localData[, four memory flags]),
@@ -77,6 +78,13 @@ int hbdia_matrix_convert(hbdia_matrix_t m, int block_width, int threshold, int m
  * (no GPU needed); hbdia_matrix_prepare_gpu uploads afterwards. */
 int hbdia_matrix_convert_host(hbdia_matrix_t m, int block_width, int threshold, int max_coo_entries,
                               int coo_is_unique, int gpu_coo);
+/* convertToHBDIAFormatOnDevice: the same conversion computed on the GPU
+ * (hbdia_convert_coo), leaving the host state of the host conversion and the
+ * device arrays of hbdia_matrix_prepare_gpu. */
+int hbdia_matrix_convert_device(hbdia_matrix_t m, int block_width, int threshold, int max_coo_entries,
+                                int coo_is_unique, int gpu_coo);
+/* deleteHBDIAFormat: drops the blocked format (host and device); the COO stays. */
+int hbdia_matrix_delete_hbdia(hbdia_matrix_t m);
 int hbdia_matrix_remove_duplicates(hbdia_matrix_t m);
 /* loadMTX: num_rows, num_cols <= 0 take the file's dimensions (and sort by
  * (row, col)); otherwise the given ones, with index checks.  *loaded = the

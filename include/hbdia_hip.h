@@ -324,6 +324,66 @@ int hbdia_coo_group_ptr(const int32_t* coo_rows, int64_t coo_nnz, int64_t num_ro
 int hbdia_vector_add_f64(double* dst, const double* src, int64_t n, hbdia_stream_t stream);
 int hbdia_vector_add_f32(float* dst, const float* src, int64_t n, hbdia_stream_t stream);
 
+/* ---- format conversion on the device ----
+ * The COO -> HBDIA conversion of HBDIA<T>::convertToHBDIAFormat
+ * (src/Format/HBDIA.cpp:507-718) computed on the GPU, with the same results:
+ * nb = ceil(min(num_rows, num_cols) / block_width) blocks of block_width rows;
+ * per block the distinct offsets col - row (col - global_rows[row] when
+ * global_rows is given) with their entry counts; a (block, offset) pair stays
+ * in the block storage iff count >= threshold or the entries already sent to
+ * the fallback (pairs visited blocks ascending, offsets ascending) number
+ * max_coo_entries or more; kept offsets ascending per block; each entry of a
+ * kept pair is assigned to values[pair * block_width + row % block_width], the
+ * last one in input order winning; empty slots hold 0; every other entry goes
+ * to the fallback COO in input order.  Entries must satisfy
+ * 0 <= row < num_rows and 0 <= col < num_cols. */
+typedef struct hbdia_convert_args {
+  const int32_t* rows;        /* device, nnz entries */
+  const int32_t* cols;        /* device, nnz entries */
+  const void* values;         /* device, nnz elements of elem_size bytes */
+  const int32_t* global_rows; /* device, num_rows entries, or NULL */
+  int64_t nnz;                /* 1 .. 2^31 - 1 */
+  int32_t num_rows, num_cols;
+  int32_t block_width;        /* > 0 */
+  int32_t threshold;
+  int32_t max_coo_entries;
+  int32_t elem_size;          /* 4 (int32, float) or 8 (double); values are copied bit for bit */
+  int32_t want_row_sorted_coo; /* also return a row-sorted copy of the fallback when it is not */
+  int32_t want_runs3;         /* also return the runs-of-three block bitset (block_width % 64 == 0) */
+} hbdia_convert_args_t;
+
+/* Device arrays in the result are allocated by hbdia_convert_coo and owned by the
+ * caller: release each with hbdia_free, or all of them with
+ * hbdia_convert_release.  NULL when empty. */
+typedef struct hbdia_convert_result {
+  void* values;               /* num_pairs * block_width elements: the block storage */
+  int32_t* offsets;           /* num_pairs + HBDIA_OFFSETS_PAD; kept offsets, blocks ascending; zero tail */
+  int32_t* block_start;       /* num_blocks + 1: first pair of each block */
+  int32_t* block_sizes;       /* num_blocks + 1: kept pairs per block (last entry 0) */
+  int32_t* block_placed;      /* num_blocks: entries placed in each block's storage */
+  uint32_t* runs3_blocks;     /* ceil(num_blocks / 32) words: bit b set when block b's offsets are
+                                 runs of three consecutive values; NULL if no block is */
+  int32_t* coo_rows;          /* num_coo: fallback in input order */
+  int32_t* coo_cols;
+  void* coo_values;
+  int32_t* coo_rows_sorted;   /* num_coo: the fallback stably sorted by row; NULL when coo_sorted */
+  int32_t* coo_cols_sorted;
+  void* coo_values_sorted;
+  int64_t num_pairs;          /* kept (block, offset) pairs */
+  int64_t num_coo;            /* fallback entries */
+  int32_t num_blocks;
+  int32_t number_diagonals;   /* distinct offsets over all pairs, kept or not */
+  int32_t runs3_count;        /* blocks with the runs-of-three bit */
+  int32_t coo_sorted;         /* 1 when the input-order fallback is already sorted by row */
+} hbdia_convert_result_t;
+
+/* Runs on `stream` and returns after the stream has finished the conversion.
+ * On failure every array it allocated is released and *out is zeroed.  Uses
+ * temporary device memory of about 28 bytes per entry. */
+int hbdia_convert_coo(const hbdia_convert_args_t* args, hbdia_convert_result_t* out, hbdia_stream_t stream);
+/* hbdia_free on every non-NULL array of *res, then zero it. */
+int hbdia_convert_release(hbdia_convert_result_t* res);
+
 #ifdef __cplusplus
 }
 #endif
