@@ -97,6 +97,7 @@ _HIP_SIGS = {
     "hbdia_vector_add_f32": (_i, [_p, _p, c_int64, _p]),
     "hbdia_convert_coo": (_i, [_p, _p, _p]),
     "hbdia_convert_release": (_i, [_p]),
+    "hbdia_convert_trim": (_i, []),
 }
 
 _CAPI_SIGS = {

@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>  // before rocPRIM: its texture iterator calls memset unqualified
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,13 @@ int bit_width(uint64_t v) {
   }
   return b;
 }
+
+#define HBDIA_TRY(expr)                   \
+  do {                                    \
+    const int rc_ = (expr);               \
+    if (rc_ != HBDIA_SUCCESS) return rc_; \
+  } while (0)
+#define HBDIA_TRY_HIP(call, what) HBDIA_TRY(check_hip((call), (what)))
 
 #define HBDIA_GRID_STRIDE(i, n)                                                  \
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; \
@@ -235,71 +243,111 @@ __global__ void k_runs3(const int* __restrict__ offsets, const int* __restrict__
   }
 }
 
-// Device allocations of one conversion: temporaries are released on return,
-// outputs only on failure.
+// Temporaries come from a cache of device blocks per device that outlives
+// the call: repeated conversions of similar sizes reuse the same blocks
+// instead of paying hipMalloc / hipFree (and page mapping) for tens of GB each
+// time; hbdia_convert_trim frees the cache.  A call takes blocks out of the
+// cache, keeps them (also the ones it no longer needs) until its stream has
+// finished, and then puts them all back.
+constexpr int kMaxDevices = 64;
+struct CachedBlock {
+  void* p;
+  size_t bytes;
+};
+std::mutex g_cache_mu;
+std::vector<CachedBlock> g_cache[kMaxDevices];
+
+// Smallest cached block of at least `bytes` and at most twice that plus 1 MiB.
+bool cache_take(std::vector<CachedBlock>& pool, size_t bytes, CachedBlock* out) {
+  size_t best = pool.size();
+  for (size_t k = 0; k < pool.size(); ++k) {
+    const size_t b = pool[k].bytes;
+    if (b >= bytes && b <= 2 * bytes + (size_t(1) << 20) && (best == pool.size() || b < pool[best].bytes)) best = k;
+  }
+  if (best == pool.size()) return false;
+  *out = pool[best];
+  pool.erase(pool.begin() + static_cast<std::ptrdiff_t>(best));
+  return true;
+}
+
+// Device allocations of one conversion: temporaries from the cache, returned
+// on destruction after the stream has finished; outputs from hipMalloc, kept
+// unless the conversion fails.
 class Arena {
  public:
+  Arena(int dev, hipStream_t s) : dev_(dev), s_(s) {}
   ~Arena() {
-    for (void* p : temps_) (void)hipFree(p);
+    (void)hipStreamSynchronize(s_);
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    for (const CachedBlock& b : live_) g_cache[dev_].push_back(b);
+    for (const CachedBlock& b : spare_) g_cache[dev_].push_back(b);
   }
   template <typename P>
   int temp(P** p, size_t count, const char* what) {
-    return alloc(reinterpret_cast<void**>(p), count * sizeof(P), what, true);
+    *p = nullptr;
+    if (count == 0) return HBDIA_SUCCESS;
+    const size_t bytes = count * sizeof(P);
+    CachedBlock b{};
+    bool hit = cache_take(spare_, bytes, &b);  // stream-ordered reuse within this call
+    if (!hit) {
+      std::lock_guard<std::mutex> lock(g_cache_mu);
+      hit = cache_take(g_cache[dev_], bytes, &b);
+    }
+    if (!hit) {
+      b.bytes = bytes;
+      HBDIA_TRY_HIP(hipMalloc(&b.p, bytes), what);
+    }
+    live_.push_back(b);
+    *p = static_cast<P*>(b.p);
+    return HBDIA_SUCCESS;
   }
   template <typename P>
   int out(P** p, size_t count, const char* what) {
-    return alloc(reinterpret_cast<void**>(p), count * sizeof(P), what, false);
+    *p = nullptr;
+    if (count == 0) return HBDIA_SUCCESS;
+    void* q = nullptr;
+    HBDIA_TRY_HIP(hipMalloc(&q, count * sizeof(P)), what);
+    *p = static_cast<P*>(q);
+    return HBDIA_SUCCESS;
   }
+  // p's contents are no longer needed; later temporaries of this call may reuse it
   void drop(void* p) {
-    auto it = std::find(temps_.begin(), temps_.end(), p);
-    if (it != temps_.end()) {
-      (void)hipFree(p);
-      temps_.erase(it);
-    }
+    for (size_t k = 0; k < live_.size(); ++k)
+      if (live_[k].p == p) {
+        spare_.push_back(live_[k]);
+        live_.erase(live_.begin() + static_cast<std::ptrdiff_t>(k));
+        return;
+      }
   }
 
  private:
-  int alloc(void** p, size_t bytes, const char* what, bool temp) {
-    *p = nullptr;
-    if (bytes == 0) return HBDIA_SUCCESS;
-    const int rc = check_hip(hipMalloc(p, bytes), what);
-    if (rc != HBDIA_SUCCESS) {
-      *p = nullptr;
-      return rc;
-    }
-    if (temp) temps_.push_back(*p);
-    return HBDIA_SUCCESS;
-  }
-  std::vector<void*> temps_;
+  int dev_;
+  hipStream_t s_;
+  std::vector<CachedBlock> live_, spare_;
 };
 
-// rocPRIM's two-phase calls: query the temporary size, then run.
+// rocPRIM's two-phase calls: query the temporary size, then run; one growing
+// buffer of the arena serves them all.
 class Scratch {
  public:
-  ~Scratch() {
-    if (p_) (void)hipFree(p_);
-  }
+  explicit Scratch(Arena& ar) : ar_(ar) {}
   int ensure(size_t bytes) {
     if (bytes <= size_) return HBDIA_SUCCESS;
-    if (p_) (void)hipFree(p_);
+    if (p_) ar_.drop(p_);
     p_ = nullptr;
     size_ = 0;
-    const int rc = check_hip(hipMalloc(&p_, std::max<size_t>(bytes, 256)), "hbdia_convert_coo: temporary storage");
-    if (rc == HBDIA_SUCCESS) size_ = std::max<size_t>(bytes, 256);
+    const size_t want = std::max<size_t>(bytes, 256);
+    const int rc = ar_.temp(reinterpret_cast<unsigned char**>(&p_), want, "hbdia_convert_coo: temporary storage");
+    if (rc == HBDIA_SUCCESS) size_ = want;
     return rc;
   }
   void* get() const { return p_; }
 
  private:
+  Arena& ar_;
   void* p_ = nullptr;
   size_t size_ = 0;
 };
-
-#define HBDIA_TRY(expr)                  \
-  do {                                   \
-    const int rc_ = (expr);              \
-    if (rc_ != HBDIA_SUCCESS) return rc_; \
-  } while (0)
 
 template <typename F>
 int prim(Scratch& sc, const char* what, F&& call) {
@@ -317,8 +365,11 @@ int fetch(V* host, const V* dev, hipStream_t s, const char* what) {
 
 template <typename U>
 int convert(const hbdia_convert_args_t& a, hbdia_convert_result_t& res, hipStream_t s) {
-  Arena ar;
-  Scratch sc;
+  int dev = 0;
+  HBDIA_TRY_HIP(hipGetDevice(&dev), "hbdia_convert_coo: device");
+  if (dev < 0 || dev >= kMaxDevices) return fail(HBDIA_ERROR_NOT_SUPPORTED, "hbdia_convert_coo: device index");
+  Arena ar(dev, s);
+  Scratch sc(ar);
   const int64_t n = a.nnz;
   const int bw = a.block_width;
   const int nb = static_cast<int>((static_cast<int64_t>(std::min(a.num_rows, a.num_cols)) + bw - 1) / bw);
@@ -529,7 +580,7 @@ int convert(const hbdia_convert_args_t& a, hbdia_convert_result_t& res, hipStrea
     HBDIA_TRY(fetch(&c, count, s, "hbdia_convert_coo: runs-of-three count"));
     res.runs3_count = c;
     if (c == 0) {
-      (void)hipFree(res.runs3_blocks);
+      HBDIA_TRY_HIP(hipFree(res.runs3_blocks), "hbdia_convert_coo: runs-of-three bits");
       res.runs3_blocks = nullptr;
     }
   }
@@ -550,6 +601,23 @@ int hbdia_convert_release(hbdia_convert_result_t* res) {
     if (p && rc == HBDIA_SUCCESS) rc = check_hip(hipFree(p), "hbdia_convert_release");
     else if (p) (void)hipFree(p);
   std::memset(res, 0, sizeof(*res));
+  return rc;
+}
+
+int hbdia_convert_trim(void) {
+  int dev = 0;
+  HBDIA_TRY_HIP(hipGetDevice(&dev), "hbdia_convert_trim");
+  if (dev < 0 || dev >= kMaxDevices) return HBDIA_SUCCESS;
+  std::vector<CachedBlock> blocks;
+  {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    blocks.swap(g_cache[dev]);
+  }
+  int rc = HBDIA_SUCCESS;
+  for (const CachedBlock& b : blocks) {
+    const int r = check_hip(hipFree(b.p), "hbdia_convert_trim");
+    if (rc == HBDIA_SUCCESS) rc = r;
+  }
   return rc;
 }
 

@@ -9,7 +9,10 @@
 
 #include <omp.h>
 
+#include "staging.hpp"
+
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -29,6 +32,18 @@ bool verbose() {
   }();
   return v;
 }
+
+// Phase timer for HBDIA_VERBOSE output.
+struct Phases {
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void mark(const char* what) {
+    const auto now = std::chrono::steady_clock::now();
+    if (verbose())
+      std::cout << "  " << what << ": " << std::chrono::duration<double, std::milli>(now - t).count() << " ms"
+                << std::endl;
+    t = now;
+  }
+};
 
 template <typename T>
 void validate_coo(const std::vector<int>& r, const std::vector<int>& c, const std::vector<T>& v,
@@ -63,7 +78,7 @@ void dev_upload(U** dst, const U* src, size_t count) {
   void* p = nullptr;
   CHECK_HBDIA(hbdia_malloc(&p, count * sizeof(U)));
   *dst = static_cast<U*>(p);
-  CHECK_HBDIA(hbdia_memcpy(p, src, count * sizeof(U)));
+  hbdia_host::upload_staged(p, src, count * sizeof(U));
 }
 
 template <typename U>
@@ -104,8 +119,8 @@ struct ConvertResult {
 
 template <typename U>
 void dev_download(std::vector<U>& dst, const void* src, size_t count) {
-  dst.resize(count);
-  if (count) CHECK_HBDIA(hbdia_memcpy(dst.data(), src, count * sizeof(U)));
+  hbdia_host::assign_zeros(dst, count);
+  if (count) hbdia_host::download_staged(dst.data(), src, count * sizeof(U));
 }
 
 }  // namespace
@@ -393,8 +408,11 @@ void HBDIA<T>::create3DStencil27Point(int nx, int ny, int nz, double noise, int 
 template <typename T>
 void HBDIA<T>::convertToHBDIAFormat(int bw, int thr, int maxCoo, bool COOisUnique,
                                     ExecutionMode execMode) {
+  Phases ph;
   if (!convertToHBDIAFormatOnHost(bw, thr, maxCoo, COOisUnique, execMode)) return;
+  ph.mark("host conversion");
   if (!partialMatrix) prepareForGPU();
+  ph.mark("upload (prepareForGPU)");
 }
 
 template <typename T>
@@ -525,7 +543,7 @@ bool HBDIA<T>::convertToHBDIAFormatOnHost(int bw, int thr, int maxCoo, bool /*CO
     throw std::runtime_error("Excessive memory requirement for HBDIA format");
 
   // 4. zeroed contiguous storage and block pointers
-  hbdiaData.assign(storageRequired, T(0));
+  hbdia_host::assign_zeros(hbdiaData, storageRequired);
   ptrToBlock.assign(nb, nullptr);
   std::vector<size_t> blockBase(nb, 0);
   {
@@ -645,6 +663,7 @@ bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*
     throw std::runtime_error("device conversion takes at most 2^31-1 entries");
   const bool useGlobal = partialMatrix && !globalRowMapping.empty();
 
+  Phases ph;
   cleanupGPUData();
   initializeStreams();
   int *R = nullptr, *C = nullptr, *G = nullptr;
@@ -663,6 +682,7 @@ bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*
   dev_upload(&C, colIndices.data(), colIndices.size());
   dev_upload(&V, values.data(), values.size());
   if (useGlobal) dev_upload(&G, globalRowMapping.data(), globalRowMapping.size());
+  ph.mark("upload COO");
 
   hbdia_convert_args_t a{};
   a.rows = R;
@@ -680,6 +700,7 @@ bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*
   a.want_runs3 = 1;
   ConvertResult res;
   CHECK_HBDIA(hbdia_convert_coo(&a, &res.r, sBDIA_));
+  ph.mark("device conversion");
   const hbdia_convert_result_t& r = res.r;
   const int nb = r.num_blocks;
 
@@ -694,6 +715,7 @@ bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*
   dev_download(cooR, r.coo_rows, static_cast<size_t>(r.num_coo));
   dev_download(cooC, r.coo_cols, static_cast<size_t>(r.num_coo));
   dev_download(cooV, r.coo_values, static_cast<size_t>(r.num_coo));
+  ph.mark("download result");
 
   blockWidth = bw;
   threshold = thr;
@@ -721,6 +743,7 @@ bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*
   cpuValues = std::move(cooV);
   fill_cpu_row_ptr(cpuRowIndices, numRows, cpuRowPtr);
   hasHBDIA = true;
+  ph.mark("host structures");
   if (partialMatrix) {  // the reference uploads partial matrices later, with their halo layout
     cleanupGPUData();
     return true;
@@ -751,6 +774,7 @@ bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*
     CHECK_HBDIA(hbdia_coo_group_ptr(cpuRowIndices_d_, r.num_coo, numRows, cooGroupPtr_d_, sBDIA_));
     CHECK_HBDIA(hbdia_stream_synchronize(sBDIA_));
   }
+  ph.mark("device structures");
   return true;
 }
 

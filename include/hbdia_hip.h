@@ -378,9 +378,14 @@ typedef struct hbdia_convert_result {
 } hbdia_convert_result_t;
 
 /* Runs on `stream` and returns after the stream has finished the conversion.
- * On failure every array it allocated is released and *out is zeroed.  Uses
- * temporary device memory of about 28 bytes per entry. */
+ * On failure every array it allocated is released and *out is zeroed.
+ * Temporaries (28 to 60 bytes per entry) come from a cache of device blocks of
+ * the current device that keeps them for the next conversion until
+ * hbdia_convert_trim.  Calls on different threads may overlap. */
 int hbdia_convert_coo(const hbdia_convert_args_t* args, hbdia_convert_result_t* out, hbdia_stream_t stream);
+/* Frees the current device's cached conversion temporaries.  Must not overlap a
+ * conversion on the same device. */
+int hbdia_convert_trim(void);
 /* hbdia_free on every non-NULL array of *res, then zero it. */
 int hbdia_convert_release(hbdia_convert_result_t* res);
 
