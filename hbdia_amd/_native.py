@@ -116,6 +116,8 @@ _CAPI_SIGS = {
     "hbdia_matrix_prepare_gpu": (_i, [_p]),
     "hbdia_matrix_synchronize": (_i, [_p]),
     "hbdia_matrix_info": (_i, [_p, _p]),
+    "hbdia_matrix_storage_report": (_i, [_p, _p, _p, _p, c_int64]),
+    "hbdia_matrix_storage_report_csv": (_i, [_p, c_char_p, _p, c_size_t, _p]),
     "hbdia_matrix_get_coo": (_i, [_p, _p, _p, _p]),
     "hbdia_matrix_get_hbdia": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "hbdia_matrix_device_ptrs": (_i, [_p, _p]),

@@ -33,6 +33,11 @@ _INFO = ["rows", "cols", "nnz", "has_coo", "has_hbdia", "block_width", "threshol
          "runs3_blocks"]
 
 
+_REPORT = ["rows", "cols", "nnz", "number_diagonals", "num_blocks", "block_width", "threshold", "max_coo_entries",
+           "coo_fallback_entries", "hbdia_data_size", "stored_offsets", "value_bytes", "hbdia_bytes", "coo_bytes",
+           "dia_bytes", "sell_bytes", "csr_bytes", "sell_applies", "hist_rows"]
+
+
 class ExecutionMode(enum.IntEnum):
     CPU_COO = 0
     GPU_COO = 1
@@ -191,6 +196,31 @@ class HBDIA:
         arr = np.zeros(len(_INFO), dtype=np.int64)
         check_capi(capi.hbdia_matrix_info(self._h, arr.ctypes.data))
         return dict(zip(_INFO, (int(v) for v in arr)))
+
+    def storage_report(self) -> dict:
+        """Statistics and storage costs (reference printMatrixInfo, benchmarking/benchmark.cpp:144-230):
+        byte counts as HBDIA / COO / DIA / SELL / CSR, their ratios to COO, and the per-block histogram
+        rows (diagonals, blocks, entries)."""
+        counts = np.zeros(len(_REPORT), np.int64)
+        ratios = np.zeros(4, np.float64)
+        check_capi(capi.hbdia_matrix_storage_report(self._h, counts.ctypes.data, ratios.ctypes.data, None, 0))
+        rows = int(counts[_REPORT.index("hist_rows")])
+        hist = np.zeros((max(rows, 1), 3), np.int64)
+        check_capi(capi.hbdia_matrix_storage_report(self._h, counts.ctypes.data, ratios.ctypes.data,
+                                                    hist.ctypes.data, rows))
+        out = dict(zip(_REPORT, (int(v) for v in counts)))
+        out["sell_applies"] = bool(out["sell_applies"])
+        out.update(zip(("hbdia_vs_coo", "dia_vs_coo", "csr_vs_coo", "sell_vs_coo"), (float(r) for r in ratios)))
+        out["histogram"] = [tuple(int(v) for v in row) for row in hist[:rows]]
+        return out
+
+    def storage_report_csv(self, name: str = "") -> str:
+        """The storage report as the reference's matrix_info.csv and histogram.csv text."""
+        n = ctypes.c_size_t(0)
+        check_capi(capi.hbdia_matrix_storage_report_csv(self._h, name.encode(), None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        check_capi(capi.hbdia_matrix_storage_report_csv(self._h, name.encode(), buf, n.value + 1, ctypes.byref(n)))
+        return buf.value.decode()
 
     def getNumRows(self) -> int: return self.info()["rows"]
     def getNumCols(self) -> int: return self.info()["cols"]

@@ -15,6 +15,7 @@
 #include "Operations/HBDIAMixed.hpp"
 #include "Operations/HBDIAServe.hpp"
 #include "Operations/HBDIASpMV.hpp"
+#include "Operations/HBDIAStats.hpp"
 #include "Workloads.hpp"
 
 struct hbdia_matrix_opaque {
@@ -273,6 +274,48 @@ int hbdia_matrix_info(hbdia_matrix_t m, int64_t* info) {
   return guard("hbdia_matrix_info", [&] {
     need(info != nullptr, "null info");
     DISPATCH(m, info_of(M, info), info_of(M, info));
+  });
+}
+
+int hbdia_matrix_storage_report(hbdia_matrix_t m, int64_t* counts, double* ratios, int64_t* hist,
+                                int64_t hist_capacity) {
+  return guard("hbdia_matrix_storage_report", [&] {
+    need(counts != nullptr && ratios != nullptr, "null output");
+    need(hist_capacity >= 0 && (hist_capacity == 0 || hist != nullptr), "bad histogram buffer");
+    HBDIAStorageReport r;
+    DISPATCH(m, r = hbdiaStorageReport(M), r = hbdiaStorageReport(M));
+    const int64_t c[HBDIA_REPORT_COUNT] = {r.rows, r.cols, r.nnz, r.numberDiagonals, r.numBlocks, r.blockWidth,
+                                           r.threshold, r.maxCooEntries, r.cooFallbackEntries, r.hbdiaDataSize,
+                                           r.storedOffsets, r.valueBytes, r.hbdiaBytes, r.cooBytes, r.diaBytes,
+                                           r.sellBytes, r.csrBytes, r.sellApplies ? 1 : 0,
+                                           static_cast<int64_t>(r.histogram.size())};
+    std::copy(c, c + HBDIA_REPORT_COUNT, counts);
+    ratios[0] = r.hbdiaVsCoo;
+    ratios[1] = r.diaVsCoo;
+    ratios[2] = r.csrVsCoo;
+    ratios[3] = r.sellVsCoo;
+    const int64_t n = std::min<int64_t>(hist_capacity, static_cast<int64_t>(r.histogram.size()));
+    for (int64_t k = 0; k < n; ++k) {
+      hist[3 * k] = r.histogram[k].diagonals;
+      hist[3 * k + 1] = r.histogram[k].blocks;
+      hist[3 * k + 2] = r.histogram[k].nnz;
+    }
+  });
+}
+
+int hbdia_matrix_storage_report_csv(hbdia_matrix_t m, const char* name, char* buf, size_t capacity,
+                                    size_t* length) {
+  return guard("hbdia_matrix_storage_report_csv", [&] {
+    need(length != nullptr && (capacity == 0 || buf != nullptr), "bad output buffer");
+    HBDIAStorageReport r;
+    DISPATCH(m, r = hbdiaStorageReport(M), r = hbdiaStorageReport(M));
+    const std::string text = hbdiaStorageReportCSV(r, name ? name : "");
+    *length = text.size();
+    if (capacity > 0) {
+      const size_t n = std::min(capacity - 1, text.size());
+      std::memcpy(buf, text.data(), n);
+      buf[n] = '\0';
+    }
   });
 }
 

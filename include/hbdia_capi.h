@@ -15,6 +15,7 @@
  *   hbdia_matrix_delete_hbdia          -> HBDIA<T>::deleteHBDIAFormat, HBDIA.hpp:67
  *   hbdia_matrix_load_mtx              -> HBDIA<T>::loadMTX, HBDIA.hpp:43-44
  *   hbdia_matrix_info / _get_*         -> the getters of HBDIA.hpp:74-110
+ *   hbdia_matrix_storage_report        -> printMatrixInfo's figures, benchmarking/benchmark.cpp:144-230
  *   hbdia_vector_create                -> HBDIAVector<T>(localData[, four memory flags]),
  *                                         include/Format/HBDIAVector.hpp:31-32
  *   hbdia_spmv                         -> hbdiaSpMV<T>, include/Operations/HBDIASpMV.hpp:8-10
@@ -93,6 +94,40 @@ int hbdia_matrix_load_mtx(hbdia_matrix_t m, const char* filename, int num_rows, 
 int hbdia_matrix_prepare_gpu(hbdia_matrix_t m);
 int hbdia_matrix_synchronize(hbdia_matrix_t m);
 int hbdia_matrix_info(hbdia_matrix_t m, int64_t* info /* [HBDIA_INFO_COUNT] */);
+
+/* Storage report of a converted matrix (hbdiaStorageReport, include/Operations/HBDIAStats.hpp;
+ * the figures of the reference's printMatrixInfo, benchmarking/benchmark.cpp:144-230). */
+enum {
+  HBDIA_REPORT_ROWS = 0,
+  HBDIA_REPORT_COLS = 1,
+  HBDIA_REPORT_NNZ = 2,
+  HBDIA_REPORT_NUMBER_DIAGONALS = 3,
+  HBDIA_REPORT_NUM_BLOCKS = 4,
+  HBDIA_REPORT_BLOCK_WIDTH = 5,
+  HBDIA_REPORT_THRESHOLD = 6,
+  HBDIA_REPORT_MAX_COO_ENTRIES = 7,
+  HBDIA_REPORT_COO_FALLBACK_ENTRIES = 8,
+  HBDIA_REPORT_HBDIA_DATA_SIZE = 9,
+  HBDIA_REPORT_STORED_OFFSETS = 10,
+  HBDIA_REPORT_VALUE_BYTES = 11,
+  HBDIA_REPORT_HBDIA_BYTES = 12,
+  HBDIA_REPORT_COO_BYTES = 13,
+  HBDIA_REPORT_DIA_BYTES = 14,
+  HBDIA_REPORT_SELL_BYTES = 15,
+  HBDIA_REPORT_CSR_BYTES = 16,
+  HBDIA_REPORT_SELL_APPLIES = 17,
+  HBDIA_REPORT_HIST_ROWS = 18,
+  HBDIA_REPORT_COUNT = 19
+};
+/* counts: [HBDIA_REPORT_COUNT]; ratios: [4] = HBDIA, DIA, CSR, SELL bytes over COO bytes;
+ * hist: up to hist_capacity rows of (diagonals, blocks, entries), counts[HBDIA_REPORT_HIST_ROWS]
+ * rows in all (hist may be NULL with capacity 0). */
+int hbdia_matrix_storage_report(hbdia_matrix_t m, int64_t* counts, double* ratios, int64_t* hist,
+                                int64_t hist_capacity);
+/* The report as CSV text (hbdiaStorageReportCSV): writes at most capacity bytes
+ * (NUL-terminated when they fit) and the full length, without the NUL, to *length. */
+int hbdia_matrix_storage_report_csv(hbdia_matrix_t m, const char* name, char* buf, size_t capacity,
+                                    size_t* length);
 int hbdia_matrix_get_coo(hbdia_matrix_t m, int32_t* rows, int32_t* cols, void* values);
 /* Any output pointer may be NULL to skip it.  Sizes from hbdia_matrix_info:
  * values NUM_VALUES, offsets NUM_OFFSETS, block_start NUM_BLOCKS+1, coo_* NUM_COO,

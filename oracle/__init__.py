@@ -159,3 +159,35 @@ def coo_spmv_longdouble(rows, cols, values, x: np.ndarray, num_rows: int) -> np.
     y = np.empty(num_rows, np.float64)
     _lib.orc_coo_spmv_ld(_ptr(r), _ptr(c), _ptr(v), len(v), xx.ctypes.data, int(num_rows), y.ctypes.data)
     return y
+
+
+def storage_report(hb: dict, num_rows: int, num_cols: int, nnz: int, nnz_per_row, block_width: int,
+                   threshold: int, value_bytes: int = 8) -> dict:
+    """Restates printMatrixInfo's storage costs (benchmarking/benchmark.cpp:144-230) over an oracle
+    conversion `hb` (convert()'s dict): HBDIA (:176-186), COO (:189-190), DIA (:193-194), SELL over
+    slices of block_width rows with the per-row counts (:197-215; empty counts leave only the slice
+    pointers), CSR (:218-220); Python integers, so nothing overflows."""
+    hist_blocks = [int(v) for v in hb["hist_blocks"]]
+    hist_nnz = [int(v) for v in hb["hist_nnz"]]
+    ncoo = len(hb["coo_values"])
+    offsets = sum(i * h for i, h in enumerate(hist_blocks))
+    nb = int(hb["num_blocks"])
+    nd = int(hb["num_diagonals"])
+    hbdia = len(hb["values"]) * value_bytes + (nb + 1) * 4 + offsets * 4 + ncoo * 4 * 2 + ncoo * value_bytes
+    coo = nnz * 4 * 2 + nnz * value_bytes
+    dia = nd * min(num_rows, num_cols) * value_bytes + nd * 4
+    sell, mx = 0, 0
+    per = [int(v) for v in nnz_per_row]
+    for i, k in enumerate(per):
+        if i % block_width == 0 and i > 0:
+            sell += mx * block_width * value_bytes + mx * block_width * 4
+            mx = 0
+        mx = max(mx, k)
+    sell += mx * block_width * value_bytes + mx * block_width * 4
+    sell += (num_rows + block_width - 1) // block_width * 4
+    csr = nnz * value_bytes + nnz * 4 + (num_rows + 1) * 4
+    hist = [(i, hist_blocks[i], hist_nnz[i] if i < len(hist_nnz) else 0) for i in range(len(hist_blocks))
+            if hist_blocks[i] > 0 or (i < len(hist_nnz) and hist_nnz[i] > 0)]
+    return {"hbdia_bytes": hbdia, "coo_bytes": coo, "dia_bytes": dia, "sell_bytes": sell, "csr_bytes": csr,
+            "stored_offsets": offsets, "number_diagonals": nd, "num_blocks": nb, "coo_fallback_entries": ncoo,
+            "hbdia_data_size": len(hb["values"]), "sell_applies": threshold == 1, "histogram": hist}
