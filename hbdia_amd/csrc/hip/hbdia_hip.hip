@@ -225,44 +225,50 @@ __device__ __forceinline__ double wave_shl1(double v, double last) {
 
 // bdia_sum_uniform for a block whose offsets are runs of three consecutive
 // values (o-1, o, o+1), as a 27-point stencil's are: a chunk of nine offsets
-// is three runs, and each run's three x vectors come from one load at its
-// centre o -- the neighbours are that vector shifted by one lane, with the
-// lane that falls off taken from x[row0+o-1] or x[row0+o+64].  The x values,
-// skips and FMA order are those of bdia_sum_uniform, with a third of its x
-// loads.  The x values, skips and FMA order are exactly those of
-// bdia_sum_uniform, with a third of its x loads (and of their cache-line
-// requests).
+// is three runs, and each run's three x vectors come from one vector load at
+// its centre o -- the neighbours are that vector shifted by one lane.  The two
+// values shifted in, x[row0+o-1] at lane 0 and x[row0+o+64] at lane 63, have
+// wave-uniform addresses and are scalar loads, issued after all vector loads
+// of the chunk (a scalar-load wait before a vector load would hold that load
+// back).  The x values, skips and FMA order are exactly those of
+// bdia_sum_uniform, with a third of its x loads.
 template <typename T, bool X32>
 __device__ __forceinline__ T bdia_sum_runs3(const T* __restrict__ vb, const int* __restrict__ op, int cnt, int bw,
                                             int lane, const T* __restrict__ x, int64_t row0, int64_t row,
                                             int64_t nrows, bool active) {
   static_assert(kChunk % 3 == 0, "a chunk holds whole runs");
+  constexpr int kRuns = kChunk / 3;
   T sum = T(0);
   const int64_t last = nrows - 1;
   for (int o0 = 0; o0 < cnt; o0 += kChunk) {
     int off[kChunk];
-    T v[kChunk], xl[kChunk];
+    T v[kChunk], xl[kChunk], mid[kRuns], first[kRuns], after[kRuns];
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
       v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
+    // runs past the block's count read padding: clamped, their products unused
 #pragma unroll
-    for (int w = 0; w < kChunk / 3; ++w) {
-      // runs past the block's count read padding: clamped, their products unused
-      const int64_t c = off[3 * w + 1];
-      int64_t cl = row + c;
-      cl = cl < 0 ? 0 : (cl > last ? last : cl);
-      int64_t cf = row0 + c - 1, ce = row0 + c + kWave;
+    for (int w = 0; w < kRuns; ++w) {
+      int64_t c = row + off[3 * w + 1];
+      c = c < 0 ? 0 : (c > last ? last : c);
+      mid[w] = load_x<T, X32>(x, c);
+    }
+#pragma unroll
+    for (int w = 0; w < kRuns; ++w) {
+      const int64_t o = off[3 * w + 1];
+      int64_t cf = row0 + o - 1, ce = row0 + o + kWave;
       cf = cf < 0 ? 0 : (cf > last ? last : cf);
       ce = ce < 0 ? 0 : (ce > last ? last : ce);
-      // lanes 0-31 load x[row0+o-1] (shifted in at lane 0), lanes 32-63
-      // x[row0+o+64] (shifted in at lane 63): one load, two lines
-      const T mid = load_x<T, X32>(x, cl);
-      const T edge = load_x<T, X32>(x, lane < kWave / 2 ? cf : ce);
-      xl[3 * w] = wave_shr1(mid, edge);
-      xl[3 * w + 1] = mid;
-      xl[3 * w + 2] = wave_shl1(mid, edge);
+      first[w] = load_x<T, X32>(x, cf);
+      after[w] = load_x<T, X32>(x, ce);
+    }
+#pragma unroll
+    for (int w = 0; w < kRuns; ++w) {
+      xl[3 * w] = wave_shr1(mid[w], first[w]);
+      xl[3 * w + 1] = mid[w];
+      xl[3 * w + 2] = wave_shl1(mid[w], after[w]);
     }
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) {
