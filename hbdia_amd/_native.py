@@ -88,6 +88,12 @@ _HIP_SIGS = {
     "hbdia_serve_spmv_f32": (_i, [_p, _p, c_int32, c_int64, c_int32, c_int32, _p, _p, _p]),
     "hbdia_batch_spmv_f32": (_i, [_p, _p, _p, _p, _p]),
     "hbdia_batch_occupancy": (_i, [_p, c_int32, c_int32, POINTER(c_int)]),
+    "hbdia_hbdia_spmm_ex_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, c_int64, _p, c_int64, c_int32,
+                                     c_int64, _p, _p]),
+    "hbdia_hbdia_spmm_ex_f32": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, c_int64, _p, c_int64, c_int32,
+                                     c_int64, _p, _p]),
+    "hbdia_hbdia_spmv_mixed_ex": (_i, [c_int, _p, _p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, _p, c_int64,
+                                       c_double, _p, _p, _p]),
     "hbdia_hbdia_spmv_mixed": (_i, [c_int, _p, _p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, _p, c_int64,
                                     c_double, _p, _p]),
     "hbdia_coo_spmv_f64": (_i, [_p, _p, _p, _p, _p, _p, c_int64, _p]),

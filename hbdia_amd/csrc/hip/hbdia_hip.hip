@@ -223,53 +223,61 @@ __device__ __forceinline__ double wave_shl1(double v, double last) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
+// x for R runs of three consecutive offsets (o-1, o, o+1), centres off[3w+1]:
+// xl[3w + d] = x[row + o - 1 + d] for the calling lane, clamped into
+// [0, last].  One vector load per run at its centre; the neighbours are that
+// vector shifted by one lane, and the two values shifted in, x[row0+o-1] at
+// lane 0 and x[row0+o+64] at lane 63, have wave-uniform addresses and are
+// scalar loads.  They are issued after the vector loads: a scalar-load wait
+// placed before a vector load would hold that load back.
+template <typename T, bool X32, int R>
+__device__ __forceinline__ void runs3_x(const T* __restrict__ x, const int* off, int64_t row0, int64_t row,
+                                        int64_t last, T* xl) {
+  T mid[R], first[R], after[R];
+#pragma unroll
+  for (int w = 0; w < R; ++w) {
+    int64_t c = row + off[3 * w + 1];
+    c = c < 0 ? 0 : (c > last ? last : c);
+    mid[w] = load_x<T, X32>(x, c);
+  }
+#pragma unroll
+  for (int w = 0; w < R; ++w) {
+    const int64_t o = off[3 * w + 1];
+    int64_t cf = row0 + o - 1, ce = row0 + o + kWave;
+    cf = cf < 0 ? 0 : (cf > last ? last : cf);
+    ce = ce < 0 ? 0 : (ce > last ? last : ce);
+    first[w] = load_x<T, X32>(x, cf);
+    after[w] = load_x<T, X32>(x, ce);
+  }
+#pragma unroll
+  for (int w = 0; w < R; ++w) {
+    xl[3 * w] = wave_shr1(mid[w], first[w]);
+    xl[3 * w + 1] = mid[w];
+    xl[3 * w + 2] = wave_shl1(mid[w], after[w]);
+  }
+}
+
 // bdia_sum_uniform for a block whose offsets are runs of three consecutive
-// values (o-1, o, o+1), as a 27-point stencil's are: a chunk of nine offsets
-// is three runs, and each run's three x vectors come from one vector load at
-// its centre o -- the neighbours are that vector shifted by one lane.  The two
-// values shifted in, x[row0+o-1] at lane 0 and x[row0+o+64] at lane 63, have
-// wave-uniform addresses and are scalar loads, issued after all vector loads
-// of the chunk (a scalar-load wait before a vector load would hold that load
-// back).  The x values, skips and FMA order are exactly those of
-// bdia_sum_uniform, with a third of its x loads.
+// values, as a 27-point stencil's are: a chunk of nine offsets is three runs
+// (runs3_x).  The x values, skips and FMA order are exactly those of
+// bdia_sum_uniform, with a third of its vector x loads.
 template <typename T, bool X32>
 __device__ __forceinline__ T bdia_sum_runs3(const T* __restrict__ vb, const int* __restrict__ op, int cnt, int bw,
                                             int lane, const T* __restrict__ x, int64_t row0, int64_t row,
                                             int64_t nrows, bool active) {
   static_assert(kChunk % 3 == 0, "a chunk holds whole runs");
-  constexpr int kRuns = kChunk / 3;
   T sum = T(0);
   const int64_t last = nrows - 1;
   for (int o0 = 0; o0 < cnt; o0 += kChunk) {
     int off[kChunk];
-    T v[kChunk], xl[kChunk], mid[kRuns], first[kRuns], after[kRuns];
+    T v[kChunk], xl[kChunk];
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
       v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
     // runs past the block's count read padding: clamped, their products unused
-#pragma unroll
-    for (int w = 0; w < kRuns; ++w) {
-      int64_t c = row + off[3 * w + 1];
-      c = c < 0 ? 0 : (c > last ? last : c);
-      mid[w] = load_x<T, X32>(x, c);
-    }
-#pragma unroll
-    for (int w = 0; w < kRuns; ++w) {
-      const int64_t o = off[3 * w + 1];
-      int64_t cf = row0 + o - 1, ce = row0 + o + kWave;
-      cf = cf < 0 ? 0 : (cf > last ? last : cf);
-      ce = ce < 0 ? 0 : (ce > last ? last : ce);
-      first[w] = load_x<T, X32>(x, cf);
-      after[w] = load_x<T, X32>(x, ce);
-    }
-#pragma unroll
-    for (int w = 0; w < kRuns; ++w) {
-      xl[3 * w] = wave_shr1(mid[w], first[w]);
-      xl[3 * w + 1] = mid[w];
-      xl[3 * w + 2] = wave_shl1(mid[w], after[w]);
-    }
+    runs3_x<T, X32, kChunk / 3>(x, off, row0, row, last, xl);
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) {
       const int64_t c = row + off[k];
@@ -455,21 +463,25 @@ __device__ __forceinline__ void coo_group_sum_multi(const int* __restrict__ cr, 
 }
 
 template <int KB> struct MrhsChunk { static constexpr int value = KB >= 16 ? 1 : (KB >= 8 ? 2 : (KB >= 4 ? 3 : 5)); };
+// runs of three per chunk on the runs-of-three path
+template <int KB> struct MrhsRuns { static constexpr int value = KB >= 4 ? 1 : 2; };
 
-template <typename T, int KB, bool HAS_COO, bool X32>
+template <typename T, int KB, bool HAS_COO, bool X32, bool RUNS3>
 __global__ __launch_bounds__(kWG) void k_hbdia_multi(
     const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy, int nrhs, int64_t nrows, int nb,
     int bw, const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
-    const int* __restrict__ gptr) {
+    const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
   constexpr int CH = MrhsChunk<KB>::value;
+  constexpr int RR = MrhsRuns<KB>::value;
   __shared__ T s_v[kWavesPerWG][kWave];
   __shared__ T s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][kWave];
   __shared__ int s_hi[kWavesPerWG][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
+  const unsigned work = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  const int64_t grp = static_cast<int64_t>(work) * kWavesPerWG + wv;
   const int64_t row0 = grp * kWave;
   if (row0 >= nrows) return;
   const int j0 = static_cast<int>(blockIdx.y) * KB;
@@ -488,7 +500,39 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
   T sum[KB];
 #pragma unroll
   for (int j = 0; j < KB; ++j) sum[j] = T(0);
-  if (cnt > 0) {
+  bool runs = false;
+  if constexpr (RUNS3) runs = cnt > 0 && ((runs3_blocks[b >> 5] >> (b & 31)) & 1u);
+  if (runs) {
+    // a multiple of three offsets, each triple consecutive: per column one vector
+    // x load per run (runs3_x); same x values, skips and FMA order as below
+    const T* vb = vals + static_cast<int64_t>(start) * bw + (row0 - b * bw);
+    const int* op = offs + start;
+    const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(T)) < (1ll << 32);
+    for (int o0 = 0; o0 < cnt; o0 += 3 * RR) {
+      int off[3 * RR];
+      T v[3 * RR];
+#pragma unroll
+      for (int k = 0; k < 3 * RR; ++k) off[k] = op[o0 + k];  // padded array
+#pragma unroll
+      for (int k = 0; k < 3 * RR; ++k) {
+        const int kk = min(o0 + k, cnt - 1);
+        v[k] = small ? __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(kk * bw + lane)))
+                     : __builtin_nontemporal_load(vb + static_cast<int64_t>(kk) * bw + lane);
+      }
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        if (j >= kb) break;
+        T xl[3 * RR];
+        runs3_x<T, X32, RR>(Xj + static_cast<int64_t>(j) * ldx, off, row0, row, last, xl);
+#pragma unroll
+        for (int k = 0; k < 3 * RR; ++k) {
+          const int64_t c = row + off[k];
+          const bool ok = (o0 + k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
+          sum[j] = masked_fma(ok, v[k], xl[k], sum[j]);
+        }
+      }
+    }
+  } else if (cnt > 0) {
     const T* vb = vals + static_cast<int64_t>(start) * bw + (row0 - b * bw);
     const int* op = offs + start;
     const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(T)) < (1ll << 32);
@@ -819,19 +863,21 @@ __global__ void k_f64_to_bf16(const double* __restrict__ src, uint16_t* __restri
 
 __global__ void k_zero_i32(int* __restrict__ p) { *p = 0; }
 
-template <typename TV, bool HAS_COO>
+template <typename TV, bool HAS_COO, bool RUNS3>
 __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
     const TV* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb, int bw,
     const int* __restrict__ cr, const int* __restrict__ cc, const TV* __restrict__ cv,
-    const int* __restrict__ gptr, float ratio, float u_store, uint64_t* __restrict__ flag_mask) {
+    const int* __restrict__ gptr, float ratio, float u_store, uint64_t* __restrict__ flag_mask,
+    const uint32_t* __restrict__ runs3_blocks, int kshift) {
   __shared__ float s_v[kWavesPerWG][kWave];
   __shared__ float s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][kWave];
   __shared__ int s_hi[kWavesPerWG][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t grp = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
+  const unsigned work = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  const int64_t grp = static_cast<int64_t>(work) * kWavesPerWG + wv;
   const int64_t row0 = grp * kWave;
   if (row0 >= nrows) return;
   const int64_t row = row0 + lane;
@@ -844,6 +890,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
     cnt = blk[b + 1] - start;
   }
   float sum = 0.0f, sabs = 0.0f;
+  bool runs = false;
+  if constexpr (RUNS3) runs = cnt > 0 && ((runs3_blocks[b >> 5] >> (b & 31)) & 1u);
   if (cnt > 0) {
     const TV* vb = vals + static_cast<int64_t>(start) * bw + (row0 - b * bw);
     const int* op = offs + start;
@@ -852,15 +900,19 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
       float v[kChunk];
       double xl[kChunk];
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];
+      for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
 #pragma unroll
       for (int k = 0; k < kChunk; ++k)
         v[k] = lp_to_f32<TV>(__builtin_nontemporal_load(vb + static_cast<int64_t>(min(o0 + k, cnt - 1)) * bw + lane));
+      if (runs) {
+        runs3_x<double, false, kChunk / 3>(x, off, row0, row, last, xl);
+      } else {
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        int64_t c = row + off[k];
-        c = c < 0 ? 0 : (c > last ? last : c);
-        xl[k] = x[c];
+        for (int k = 0; k < kChunk; ++k) {
+          int64_t c = row + off[k];
+          c = c < 0 ? 0 : (c > last ? last : c);
+          xl[k] = x[c];
+        }
       }
 #pragma unroll
       for (int k = 0; k < kChunk; ++k) {
@@ -1100,19 +1152,25 @@ template <typename T, int KB>
 void launch_multi_kb(const T* vals, const int32_t* offs, const int32_t* blk, int32_t nb, int32_t bw,
                      const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr, const T* X,
                      int64_t ldx, T* Y, int64_t ldy, int nrhs, int64_t nrows, bool with_coo, bool x32,
-                     dim3 grid, hipStream_t s) {
-#define HBDIA_LAUNCH_M(C, XX)                                                                               \
-  hipLaunchKernelGGL((k_hbdia_multi<T, KB, C, XX>), grid, dim3(kWG), 0, s, vals, offs, blk, X, ldx, Y, ldy, nrhs, \
-                     nrows, nb, bw, cr, cc, cv, gptr)
-  if (with_coo) { if (x32) HBDIA_LAUNCH_M(true, true); else HBDIA_LAUNCH_M(true, false); }
-  else { if (x32) HBDIA_LAUNCH_M(false, true); else HBDIA_LAUNCH_M(false, false); }
+                     const uint32_t* runs3, int kshift, dim3 grid, hipStream_t s) {
+#define HBDIA_LAUNCH_M(C, XX, R)                                                                              \
+  hipLaunchKernelGGL((k_hbdia_multi<T, KB, C, XX, R>), grid, dim3(kWG), 0, s, vals, offs, blk, X, ldx, Y, ldy, \
+                     nrhs, nrows, nb, bw, cr, cc, cv, gptr, runs3, kshift)
+  if (runs3) {
+    if (with_coo) { if (x32) HBDIA_LAUNCH_M(true, true, true); else HBDIA_LAUNCH_M(true, false, true); }
+    else { if (x32) HBDIA_LAUNCH_M(false, true, true); else HBDIA_LAUNCH_M(false, false, true); }
+  } else {
+    if (with_coo) { if (x32) HBDIA_LAUNCH_M(true, true, false); else HBDIA_LAUNCH_M(true, false, false); }
+    else { if (x32) HBDIA_LAUNCH_M(false, true, false); else HBDIA_LAUNCH_M(false, false, false); }
+  }
 #undef HBDIA_LAUNCH_M
 }
 
 template <typename T>
 int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t nb, int32_t bw,
                  const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr, const T* X, int64_t ldx,
-                 T* Y, int64_t ldy, int32_t nrhs, int64_t nrows, hbdia_stream_t stream, const char* name) {
+                 T* Y, int64_t ldy, int32_t nrhs, int64_t nrows, hbdia_stream_t stream, const char* name,
+                 const uint32_t* runs3 = nullptr) {
   if (nrows < 0 || nb < 0 || bw <= 0 || nrhs < 0)
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative size or block_width <= 0");
   if (nrows == 0 || nrhs == 0) return HBDIA_SUCCESS;
@@ -1134,11 +1192,12 @@ int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   const int kb = nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1));
   const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>((nrhs + kb - 1) / kb));
   hipStream_t s = hs(stream);
+  const int ks = xcd_chunk_shift(wgs);
   switch (kb) {
-    case 8: launch_multi_kb<T, 8>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, grid, s); break;
-    case 4: launch_multi_kb<T, 4>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, grid, s); break;
-    case 2: launch_multi_kb<T, 2>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, grid, s); break;
-    default: launch_multi_kb<T, 1>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, grid, s); break;
+    case 8: launch_multi_kb<T, 8>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks, grid, s); break;
+    case 4: launch_multi_kb<T, 4>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks, grid, s); break;
+    case 2: launch_multi_kb<T, 2>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks, grid, s); break;
+    default: launch_multi_kb<T, 1>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks, grid, s); break;
   }
   return check_hip(hipGetLastError(), name);
 }
@@ -1146,7 +1205,7 @@ int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
 int launch_mixed(int vdtype, const void* vals_lp, const double* vals64, const int32_t* offs, const int32_t* blk,
                  int32_t nb, int32_t bw, const int32_t* cr, const int32_t* cc, const void* cv_lp, const double* cv64,
                  const int32_t* gptr, const double* x, double* y, int64_t nrows, double ratio, void* workspace,
-                 hbdia_stream_t stream) {
+                 hbdia_stream_t stream, const uint32_t* runs3 = nullptr) {
   const char* name = "hbdia_hbdia_spmv_mixed";
   if (nrows < 0 || nb < 0 || bw <= 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": bad size");
   if (nrows == 0) return HBDIA_SUCCESS;
@@ -1166,17 +1225,18 @@ int launch_mixed(int vdtype, const void* vals_lp, const double* vals64, const in
   const dim3 grid(static_cast<unsigned>((groups + kWavesPerWG - 1) / kWavesPerWG)), block(kWG);
   const float r = static_cast<float>(ratio);
   const float u_store = vdtype == HBDIA_VALUES_F32 ? 5.9604645e-08f : 3.9062500e-03f + 5.9604645e-08f;
+  const int ks = xcd_chunk_shift(grid.x);
+#define HBDIA_LAUNCH_MX(TV, C, R)                                                                             \
+  hipLaunchKernelGGL((k_hbdia_mixed<TV, C, R>), grid, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, x, \
+                     y, nrows, nb, bw, cr, cc, static_cast<const TV*>(cv_lp), gptr, r, u_store, mask, runs3, ks)
   if (vdtype == HBDIA_VALUES_F32) {
-    const float* v = static_cast<const float*>(vals_lp);
-    const float* c = static_cast<const float*>(cv_lp);
-    if (with_coo) hipLaunchKernelGGL((k_hbdia_mixed<float, true>), grid, block, 0, s, v, offs, blk, x, y, nrows, nb, bw, cr, cc, c, gptr, r, u_store, mask);
-    else hipLaunchKernelGGL((k_hbdia_mixed<float, false>), grid, block, 0, s, v, offs, blk, x, y, nrows, nb, bw, cr, cc, c, gptr, r, u_store, mask);
+    if (runs3) { if (with_coo) HBDIA_LAUNCH_MX(float, true, true); else HBDIA_LAUNCH_MX(float, false, true); }
+    else { if (with_coo) HBDIA_LAUNCH_MX(float, true, false); else HBDIA_LAUNCH_MX(float, false, false); }
   } else {
-    const uint16_t* v = static_cast<const uint16_t*>(vals_lp);
-    const uint16_t* c = static_cast<const uint16_t*>(cv_lp);
-    if (with_coo) hipLaunchKernelGGL((k_hbdia_mixed<uint16_t, true>), grid, block, 0, s, v, offs, blk, x, y, nrows, nb, bw, cr, cc, c, gptr, r, u_store, mask);
-    else hipLaunchKernelGGL((k_hbdia_mixed<uint16_t, false>), grid, block, 0, s, v, offs, blk, x, y, nrows, nb, bw, cr, cc, c, gptr, r, u_store, mask);
+    if (runs3) { if (with_coo) HBDIA_LAUNCH_MX(uint16_t, true, true); else HBDIA_LAUNCH_MX(uint16_t, false, true); }
+    else { if (with_coo) HBDIA_LAUNCH_MX(uint16_t, true, false); else HBDIA_LAUNCH_MX(uint16_t, false, false); }
   }
+#undef HBDIA_LAUNCH_MX
   const int64_t scan_waves = (groups + kWave - 1) / kWave;
   const int64_t fix_blocks = std::min<int64_t>((scan_waves + 3) / 4, 2048);
   hipLaunchKernelGGL(k_fix_groups, dim3(static_cast<unsigned>(fix_blocks)), dim3(256), 0, s, vals64, offs, blk, x, y,
@@ -1542,6 +1602,24 @@ int hbdia_hbdia_spmm_f32(const float* values, const int32_t* offsets, const int3
   return launch_multi<float>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols, coo_values,
                              coo_group_ptr, X, ldx, Y, ldy, nrhs, num_rows, stream, "hbdia_hbdia_spmm_f32");
 }
+int hbdia_hbdia_spmm_ex_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const double* coo_values, const int32_t* coo_group_ptr,
+                            const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t nrhs, int64_t num_rows,
+                            const uint32_t* runs3_blocks, hbdia_stream_t stream) {
+  return launch_multi<double>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols, coo_values,
+                              coo_group_ptr, X, ldx, Y, ldy, nrhs, num_rows, stream, "hbdia_hbdia_spmm_ex_f64",
+                              runs3_blocks);
+}
+int hbdia_hbdia_spmm_ex_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const float* coo_values, const int32_t* coo_group_ptr,
+                            const float* X, int64_t ldx, float* Y, int64_t ldy, int32_t nrhs, int64_t num_rows,
+                            const uint32_t* runs3_blocks, hbdia_stream_t stream) {
+  return launch_multi<float>(values, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols, coo_values,
+                             coo_group_ptr, X, ldx, Y, ldy, nrhs, num_rows, stream, "hbdia_hbdia_spmm_ex_f32",
+                             runs3_blocks);
+}
 
 int hbdia_batch_make_tiles(const hbdia_batch_desc_t* desc, int32_t num_matrices, const int32_t* offsets,
                            const int32_t* block_start, const int32_t* group_ptr, int32_t value_size,
@@ -1656,6 +1734,16 @@ int hbdia_hbdia_spmv_mixed(int vdtype, const void* values_lp, const double* valu
                            int64_t num_rows, double ratio, void* workspace, hbdia_stream_t stream) {
   return launch_mixed(vdtype, values_lp, values_f64, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols,
                       coo_values_lp, coo_values_f64, coo_group_ptr, x, y, num_rows, ratio, workspace, stream);
+}
+int hbdia_hbdia_spmv_mixed_ex(int vdtype, const void* values_lp, const double* values_f64, const int32_t* offsets,
+                              const int32_t* block_start, int32_t num_blocks, int32_t block_width,
+                              const int32_t* coo_rows, const int32_t* coo_cols, const void* coo_values_lp,
+                              const double* coo_values_f64, const int32_t* coo_group_ptr, const double* x, double* y,
+                              int64_t num_rows, double ratio, void* workspace, const uint32_t* runs3_blocks,
+                              hbdia_stream_t stream) {
+  return launch_mixed(vdtype, values_lp, values_f64, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols,
+                      coo_values_lp, coo_values_f64, coo_group_ptr, x, y, num_rows, ratio, workspace, stream,
+                      runs3_blocks);
 }
 int hbdia_coo_spmv_f64(const int32_t* coo_rows, const int32_t* coo_cols, const double* coo_values,
                        const int32_t* coo_group_ptr, const double* x, double* res,

@@ -58,12 +58,13 @@ void HBDIAMixed::spmv(const double* x, double* y) {
   // read the value arrays; any non-null device pointer satisfies the ABI.
   const void* vlp = values_ ? values_ : static_cast<const void*>(x);
   const double* v64 = m_.getHBDIADataDevice() ? m_.getHBDIADataDevice() : x;
-  CHECK_HBDIA(hbdia_hbdia_spmv_mixed(
+  CHECK_HBDIA(hbdia_hbdia_spmv_mixed_ex(
       vdtype(precision_), vlp, v64,
       m_.getFlattenedOffsetsDevice(), m_.getBlockStartIndicesDevice(), m_.getNumBlocks(), m_.getBlockWidth(),
       hasCOO ? m_.getCpuRowIndicesDevice() : nullptr, hasCOO ? m_.getCpuColIndicesDevice() : nullptr,
       hasCOO ? cooValues_ : nullptr, hasCOO ? m_.getCpuValuesDevice() : nullptr,
-      hasCOO ? m_.getCooGroupPtrDevice() : nullptr, x, y, m_.getNumRows(), ratio_, workspace_, m_.getBDIAStream()));
+      hasCOO ? m_.getCooGroupPtrDevice() : nullptr, x, y, m_.getNumRows(), ratio_, workspace_,
+      m_.getRuns3BlocksDevice(), m_.getBDIAStream()));
   CHECK_HBDIA(hbdia_event_record(m_.getBDIAEvent(), m_.getBDIAStream()));
 }
 

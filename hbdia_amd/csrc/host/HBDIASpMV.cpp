@@ -36,10 +36,11 @@ struct Dev<double> {
   static double fma(double a, double b, double c) { return std::fma(a, b, c); }
   static int multi(const HBDIA<double>& m, bool coo, const double* X, int64_t ldx, double* Y, int64_t ldy, int k,
                    hbdia_stream_t s) {
-    return hbdia_hbdia_spmm_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(), m.getBlockStartIndicesDevice(),
-                                m.getNumBlocks(), m.getBlockWidth(), coo ? m.getCpuRowIndicesDevice() : nullptr,
-                                coo ? m.getCpuColIndicesDevice() : nullptr, coo ? m.getCpuValuesDevice() : nullptr,
-                                coo ? m.getCooGroupPtrDevice() : nullptr, X, ldx, Y, ldy, k, m.getNumRows(), s);
+    return hbdia_hbdia_spmm_ex_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
+                                   m.getBlockStartIndicesDevice(), m.getNumBlocks(), m.getBlockWidth(),
+                                   coo ? m.getCpuRowIndicesDevice() : nullptr, coo ? m.getCpuColIndicesDevice() : nullptr,
+                                   coo ? m.getCpuValuesDevice() : nullptr, coo ? m.getCooGroupPtrDevice() : nullptr, X,
+                                   ldx, Y, ldy, k, m.getNumRows(), m.getRuns3BlocksDevice(), s);
   }
 };
 
@@ -66,10 +67,11 @@ struct Dev<float> {
   static float fma(float a, float b, float c) { return std::fmaf(a, b, c); }
   static int multi(const HBDIA<float>& m, bool coo, const float* X, int64_t ldx, float* Y, int64_t ldy, int k,
                    hbdia_stream_t s) {
-    return hbdia_hbdia_spmm_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(), m.getBlockStartIndicesDevice(),
-                                m.getNumBlocks(), m.getBlockWidth(), coo ? m.getCpuRowIndicesDevice() : nullptr,
-                                coo ? m.getCpuColIndicesDevice() : nullptr, coo ? m.getCpuValuesDevice() : nullptr,
-                                coo ? m.getCooGroupPtrDevice() : nullptr, X, ldx, Y, ldy, k, m.getNumRows(), s);
+    return hbdia_hbdia_spmm_ex_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
+                                   m.getBlockStartIndicesDevice(), m.getNumBlocks(), m.getBlockWidth(),
+                                   coo ? m.getCpuRowIndicesDevice() : nullptr, coo ? m.getCpuColIndicesDevice() : nullptr,
+                                   coo ? m.getCpuValuesDevice() : nullptr, coo ? m.getCooGroupPtrDevice() : nullptr, X,
+                                   ldx, Y, ldy, k, m.getNumRows(), m.getRuns3BlocksDevice(), s);
   }
 };
 

@@ -85,12 +85,14 @@ def hbdia_spmv_ex(values: DeviceArray, offsets: DeviceArray, block_start: Device
 
 def hbdia_spmm(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArray, num_blocks: int,
                block_width: int, coo, X: DeviceArray, ldx: int, Y: DeviceArray, ldy: int, nrhs: int, num_rows: int,
-               stream: Stream | None = None) -> None:
-    """Many right-hand sides; coo = (rows, cols, values, group_ptr) DeviceArrays or None."""
-    fn = getattr(hip, "hbdia_hbdia_spmm_" + _sfx(values.dtype))
+               stream: Stream | None = None, runs3: DeviceArray | None = None) -> None:
+    """Many right-hand sides; coo = (rows, cols, values, group_ptr) DeviceArrays or None; runs3 the
+    runs-of-three block bitset (None: no hints)."""
+    fn = getattr(hip, "hbdia_hbdia_spmm_ex_" + _sfx(values.dtype))
     cr, cc, cv, gp = (a.ptr for a in coo) if coo is not None else (None, None, None, None)
     check_hip(fn(values.ptr, offsets.ptr, block_start.ptr, int(num_blocks), int(block_width), cr, cc, cv, gp, X.ptr,
-                 int(ldx), Y.ptr, int(ldy), int(nrhs), int(num_rows), _h(stream)))
+                 int(ldx), Y.ptr, int(ldy), int(nrhs), int(num_rows), runs3.ptr if runs3 is not None else None,
+                 _h(stream)))
 
 
 def coo_spmv(coo_rows: DeviceArray, coo_cols: DeviceArray, coo_values: DeviceArray, coo_group_ptr: DeviceArray,
@@ -208,10 +210,11 @@ class DeviceLayout:
                 from .device import synchronize
                 synchronize()
 
-    def spmm(self, X: DeviceArray, Y: DeviceArray, nrhs: int, stream: Stream | None = None) -> None:
+    def spmm(self, X: DeviceArray, Y: DeviceArray, nrhs: int, stream: Stream | None = None,
+             hints: bool = False) -> None:
         coo = (self.coo_rows, self.coo_cols, self.coo_values, self.group_ptr) if self.coo_nnz else None
         hbdia_spmm(self.values, self.offsets, self.block_start, self.num_blocks, self.block_width, coo, X,
-                   self.num_rows, Y, self.num_rows, nrhs, self.num_rows, stream)
+                   self.num_rows, Y, self.num_rows, nrhs, self.num_rows, stream, self.runs3 if hints else None)
 
     def spmv(self, x: DeviceArray, y: DeviceArray, stream: Stream | None = None, hints: bool = False) -> None:
         """y = A x with the fused kernel (blocked-DIA only when there is no fallback); `hints` passes

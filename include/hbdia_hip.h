@@ -163,6 +163,18 @@ int hbdia_hbdia_spmm_f32(const float* values, const int32_t* offsets, const int3
                          const int32_t* coo_cols, const float* coo_values, const int32_t* coo_group_ptr,
                          const float* X, int64_t ldx, float* Y, int64_t ldy, int32_t nrhs,
                          int64_t num_rows, hbdia_stream_t stream);
+/* The same with the runs-of-three block bitset of hbdia_hbdia_spmv_ex_*
+ * (NULL: no hints); results are bit-identical. */
+int hbdia_hbdia_spmm_ex_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const double* coo_values, const int32_t* coo_group_ptr,
+                            const double* X, int64_t ldx, double* Y, int64_t ldy, int32_t nrhs, int64_t num_rows,
+                            const uint32_t* runs3_blocks, hbdia_stream_t stream);
+int hbdia_hbdia_spmm_ex_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const float* coo_values, const int32_t* coo_group_ptr,
+                            const float* X, int64_t ldx, float* Y, int64_t ldy, int32_t nrhs, int64_t num_rows,
+                            const uint32_t* runs3_blocks, hbdia_stream_t stream);
 
 /* Reduced-precision storage (no reference counterpart: the reference stores
  * and computes in one type).  values_lp / coo_values_lp are FP32 or BF16 copies
@@ -190,6 +202,14 @@ int hbdia_hbdia_spmv_mixed(int vdtype, const void* values_lp, const double* valu
                            const int32_t* coo_rows, const int32_t* coo_cols, const void* coo_values_lp,
                            const double* coo_values_f64, const int32_t* coo_group_ptr, const double* x,
                            double* y, int64_t num_rows, double ratio, void* workspace, hbdia_stream_t stream);
+/* The same with the runs-of-three block bitset (NULL: no hints); results and
+ * masks are bit-identical. */
+int hbdia_hbdia_spmv_mixed_ex(int vdtype, const void* values_lp, const double* values_f64, const int32_t* offsets,
+                              const int32_t* block_start, int32_t num_blocks, int32_t block_width,
+                              const int32_t* coo_rows, const int32_t* coo_cols, const void* coo_values_lp,
+                              const double* coo_values_f64, const int32_t* coo_group_ptr, const double* x,
+                              double* y, int64_t num_rows, double ratio, void* workspace,
+                              const uint32_t* runs3_blocks, hbdia_stream_t stream);
 
 /* Many independent matrices in one persistent launch (no reference
  * counterpart: the reference multiplies one matrix per call).  The matrices'

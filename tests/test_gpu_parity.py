@@ -320,6 +320,36 @@ def test_runs3_hint_is_bit_identical(gpu, N, bw, with_coo):
     np.testing.assert_array_equal(outs[1], want)
 
 
+@pytest.mark.parametrize("N,bw,with_coo,k", [(1000, 64, True, 3), (1000, 64, False, 8), (4099, 128, True, 5),
+                                             (777, 256, True, 16)])
+def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k):
+    """The multi-RHS kernel's runs-of-three path: every column equals the hint-free multi-RHS result,
+    the single-vector kernel and the oracle."""
+    print(f"case: runs3 multi-RHS N={N} bw={bw} coo={with_coo} k={k}", flush=True)
+    values, offsets, block_start, x0, coo = _runs3_case(N, bw, 7 + N + bw + k, with_coo)
+    L = _layout(values, offsets, block_start, N, bw, coo)
+    rng = np.random.default_rng(k)
+    Xh = np.vstack([x0] + [rng.standard_normal(N) for _ in range(k - 1)])
+    Y0 = np.vstack([np.linspace(-1.0, 1.0, N) + j for j in range(k)])
+    X = DeviceArray.from_host(Xh.ravel())
+    outs = []
+    for hints in (False, True):
+        Y = DeviceArray.from_host(Y0.ravel())
+        L.spmm(X, Y, k, hints=hints)
+        synchronize()
+        outs.append(Y.download().reshape(k, N))
+    np.testing.assert_array_equal(outs[1], outs[0])
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": len(block_start) - 1,
+          "block_width": bw}
+    if coo is not None:
+        hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
+    else:
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0))
+    for j in range(k):
+        np.testing.assert_array_equal(outs[1][j], oracle.hbdia_spmv(hb, Xh[j].copy(), Y0[j].copy(), N),
+                                      err_msg=f"column {j}")
+
+
 def test_stencil27_gets_the_runs3_hint_and_stays_exact(gpu):
     A = HBDIA.stencil(20, 27, 0.01, seed=77)
     A.convertToHBDIAFormat(64, 2)
