@@ -565,7 +565,7 @@ int convert(const hbdia_convert_args_t& a, hbdia_convert_result_t& res, hipStrea
   }
 
   // runs-of-three hint for the SpMV kernels
-  if (a.want_runs3 && bw % HBDIA_ROW_GROUP == 0 && nb > 0) {
+  if (a.want_runs3 && (bw % HBDIA_ROW_GROUP == 0 || bw == HBDIA_ROW_GROUP / 2) && nb > 0) {
     const int64_t words = (static_cast<int64_t>(nb) + 31) / 32;
     int* count = nullptr;
     HBDIA_TRY(ar.out(&res.runs3_blocks, words, "hbdia_convert_coo: runs-of-three bits"));

@@ -154,9 +154,10 @@ __device__ __forceinline__ T load_x(const T* __restrict__ x, int64_t c) {
   else return x[c];
 }
 
-// Blocked-DIA sum of one row whose 64-row wave lies in one block (block width
-// a multiple of 64).  `vb` is the wave-uniform address of the block's first
-// offset line at this wave's first row; offset o of lane l is vb[o*bw + l].
+// Blocked-DIA sum of one row whose 64-row wave shares one offset list (one
+// block of a width that is a multiple of 64, or two width-32 blocks with equal
+// offsets).  `vb` is a wave-uniform value address and vlane the lane's index
+// from it: offset o of the lane is vb[o*bw + vlane].
 // Offsets are consumed in chunks of kChunk with no control flow inside a
 // chunk: the tail of the last chunk re-reads the last valid line (a cache hit)
 // and x addresses are clamped into [0, nrows), so every load of a chunk can
@@ -167,7 +168,7 @@ __device__ __forceinline__ T load_x(const T* __restrict__ x, int64_t c) {
 // zero values and columns outside [0, nrows), exactly as the reference kernel.
 template <typename T, bool X32>
 __device__ __forceinline__ T bdia_sum_uniform(const T* __restrict__ vb, const int* __restrict__ op, int cnt,
-                                              int bw, int lane, const T* __restrict__ x, int64_t row,
+                                              int bw, int vlane, const T* __restrict__ x, int64_t row,
                                               int64_t nrows, bool active) {
   T sum = T(0);
   const int64_t last = nrows - 1;
@@ -178,7 +179,7 @@ __device__ __forceinline__ T bdia_sum_uniform(const T* __restrict__ vb, const in
     for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
-      v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
+      v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + vlane)));
 #pragma unroll
     for (int k = 0; k < kChunk; ++k) {
       int64_t c = row + off[k];
@@ -263,7 +264,7 @@ __device__ __forceinline__ void runs3_x(const T* __restrict__ x, const int* off,
 // bdia_sum_uniform, with a third of its vector x loads.
 template <typename T, bool X32>
 __device__ __forceinline__ T bdia_sum_runs3(const T* __restrict__ vb, const int* __restrict__ op, int cnt, int bw,
-                                            int lane, const T* __restrict__ x, int64_t row0, int64_t row,
+                                            int vlane, const T* __restrict__ x, int64_t row0, int64_t row,
                                             int64_t nrows, bool active) {
   static_assert(kChunk % 3 == 0, "a chunk holds whole runs");
   T sum = T(0);
@@ -275,7 +276,7 @@ __device__ __forceinline__ T bdia_sum_runs3(const T* __restrict__ vb, const int*
     for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
-      v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
+      v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + vlane)));
     // runs past the block's count read padding: clamped, their products unused
     runs3_x<T, X32, kChunk / 3>(x, off, row0, row, last, xl);
 #pragma unroll
@@ -392,7 +393,36 @@ __global__ __launch_bounds__(kWG) void k_hbdia(
       sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
     }
   } else {
-    if (cnt > 0) {
+    // Block width 32 (the reference's default): the wave's rows are blocks
+    // b0 = row0 / 32 and b0 + 1.  When both exist and hold the same offsets,
+    // the wave runs the uniform sums with a per-lane value index (block
+    // b0 + 1's lines follow block b0's: lane l >= 32 reads line
+    // pcnt + o at l - 32); otherwise each lane runs its own block.
+    bool pair = false;
+    int pstart = 0, pcnt = 0;
+    int64_t b0 = 0;
+    if (bw == 32) {
+      b0 = row0 / 32;
+      if (b0 + 1 < nb) {
+        const int s0 = blk[b0], s1 = blk[b0 + 1], e1 = blk[b0 + 2];
+        if (s1 - s0 == e1 - s1 && s1 > s0 && s1 - s0 <= kWave) {
+          const int c0 = s1 - s0;
+          const int oa = lane < c0 ? offs[s0 + lane] : 0;
+          const int ob = lane < c0 ? offs[s1 + lane] : 0;
+          pair = __ballot(oa != ob) == 0;
+          pstart = s0;
+          pcnt = c0;
+        }
+      }
+    }
+    if (pair) {
+      const T* vb = vals + static_cast<int64_t>(pstart) * 32;
+      const int vlane = lane + (lane >= 32 ? (pcnt - 1) * 32 : 0);
+      bool runs = false;
+      if constexpr (RUNS3) runs = (runs3_blocks[b0 >> 5] >> (b0 & 31)) & 1u;
+      if (runs) sum = bdia_sum_runs3<T, X32>(vb, offs + pstart, pcnt, 32, vlane, x, row0, row, nrows, active);
+      else sum = bdia_sum_uniform<T, X32>(vb, offs + pstart, pcnt, 32, vlane, x, row, nrows, active);
+    } else if (cnt > 0) {
       const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
       sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
     }
@@ -1129,14 +1159,17 @@ int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   // (checked by the caller through the largest block).
   const bool x32 = static_cast<uint64_t>(nrows) * sizeof(T) < (1ull << 32);
   hipStream_t s = hs(stream);
-  const bool runs3 = uniform && runs3_blocks != nullptr;
+  const bool runs3 = (uniform || bw == 32) && runs3_blocks != nullptr;
   const int kshift = xcd_chunk_shift(wgs);
 #define HBDIA_LAUNCH(U, C, X, R)                                                                             \
   hipLaunchKernelGGL((k_hbdia<T, U, C, X, R>), grid, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, \
                      cv, gptr, runs3_blocks, kshift)
-  if (runs3) {
+  if (runs3 && uniform) {
     if (with_coo) { if (x32) HBDIA_LAUNCH(true, true, true, true); else HBDIA_LAUNCH(true, true, false, true); }
     else { if (x32) HBDIA_LAUNCH(true, false, true, true); else HBDIA_LAUNCH(true, false, false, true); }
+  } else if (runs3) {
+    if (with_coo) { if (x32) HBDIA_LAUNCH(false, true, true, true); else HBDIA_LAUNCH(false, true, false, true); }
+    else { if (x32) HBDIA_LAUNCH(false, false, true, true); else HBDIA_LAUNCH(false, false, false, true); }
   } else if (uniform) {
     if (with_coo) { if (x32) HBDIA_LAUNCH(true, true, true, false); else HBDIA_LAUNCH(true, true, false, false); }
     else { if (x32) HBDIA_LAUNCH(true, false, true, false); else HBDIA_LAUNCH(true, false, false, false); }

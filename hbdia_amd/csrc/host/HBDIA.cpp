@@ -1030,7 +1030,7 @@ void HBDIA<T>::prepareForGPU() {
   // consecutive values (a multiple of three offsets, each triple consecutive)
   std::vector<uint32_t> runs3Bits;
   runs3Count_ = 0;
-  if (blockWidth % HBDIA_ROW_GROUP == 0 && nb > 0) {
+  if ((blockWidth % HBDIA_ROW_GROUP == 0 || blockWidth == HBDIA_ROW_GROUP / 2) && nb > 0) {
     runs3Bits.assign((static_cast<size_t>(nb) + 31) / 32, 0u);
     for (int b = 0; b < nb; ++b) {
       const auto& o = offsetsPerBlock[b];

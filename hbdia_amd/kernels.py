@@ -53,9 +53,10 @@ def hbdia_spmv(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArr
 
 def runs3_bits(offsets: np.ndarray, block_start: np.ndarray, block_width: int) -> np.ndarray | None:
     """The runs3_blocks hint: bit b set when block b's offsets are runs of three consecutive values
-    (a multiple of three offsets, each triple consecutive); None when no block qualifies or bw % 64 != 0."""
+    (a multiple of three offsets, each triple consecutive); None when no block qualifies or the block
+    width is neither a multiple of 64 nor 32."""
     nb = len(block_start) - 1
-    if block_width % ROW_GROUP or nb <= 0:
+    if (block_width % ROW_GROUP and block_width != ROW_GROUP // 2) or nb <= 0:
         return None
     bits = np.zeros((nb + 31) // 32, np.uint32)
     for b in range(nb):

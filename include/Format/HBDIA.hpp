@@ -187,7 +187,7 @@ class HBDIA {
   bool hasDeviceCOO() const { return cooGroupPtr_d_ != nullptr; }
   // Device bitset of the blocks whose offsets are runs of three consecutive
   // values (the kernels' runs3_blocks hint), or null when no block is; built
-  // by prepareForGPU for block widths that are multiples of 64.
+  // by prepareForGPU for block widths that are multiples of 64 or 32.
   const uint32_t* getRuns3BlocksDevice() const { return runs3Blocks_d_; }
   int64_t getRuns3BlockCount() const { return runs3Count_; }
 

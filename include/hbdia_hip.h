@@ -128,7 +128,7 @@ int hbdia_hbdia_spmv_f32(const float* values, const int32_t* offsets, const int3
  * each triple consecutive -- as a 27-point stencil's are.  Such a block loads
  * each run's x once and shifts it across lanes for the run's neighbours.  A
  * set bit must be true of its block; clear bits are always safe; NULL means no
- * hints.  Used when block_width % 64 == 0.  Results are bit-identical to the
+ * hints.  Used when block_width % 64 == 0 or block_width == 32.  Results are bit-identical to the
  * entry points above. */
 int hbdia_bdia_spmv_ex_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
                            const double* x, double* y, int64_t num_rows, int32_t num_blocks, int32_t block_width,
@@ -369,7 +369,7 @@ typedef struct hbdia_convert_args {
   int32_t max_coo_entries;
   int32_t elem_size;          /* 4 (int32, float) or 8 (double); values are copied bit for bit */
   int32_t want_row_sorted_coo; /* also return a row-sorted copy of the fallback when it is not */
-  int32_t want_runs3;         /* also return the runs-of-three block bitset (block_width % 64 == 0) */
+  int32_t want_runs3;         /* also return the runs-of-three block bitset (block_width % 64 == 0 or 32) */
 } hbdia_convert_args_t;
 
 /* Device arrays in the result are allocated by hbdia_convert_coo and owned by the

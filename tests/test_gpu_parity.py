@@ -350,6 +350,82 @@ def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k):
                                       err_msg=f"column {j}")
 
 
+def _width32_case(N, seed, with_coo, share):
+    """Block width 32: pairs of blocks (2k, 2k+1) that with probability `share` hold the same offset
+    list (then one wave runs them together), otherwise their own; lists mix runs of three, single
+    offsets (some out of range) and empty blocks; zero values over NaN x entries."""
+    rng = np.random.default_rng(seed)
+    bw = 32
+    nb = -(-N // bw)
+    offs_per = []
+    for b in range(nb):
+        if b % 2 == 1 and rng.random() < share:
+            offs_per.append(list(offs_per[-1]))
+            continue
+        k = int(rng.integers(0, 5))
+        if rng.random() < 0.6:
+            centres = 3 * np.sort(rng.choice(np.arange(-N // 2, N // 2), size=k, replace=False)) + 1
+            offs_per.append([int(c + d) for c in centres for d in (-1, 0, 1)])
+        else:
+            offs_per.append(sorted(int(o) for o in rng.choice(np.arange(-N, N), size=k, replace=False)))
+    block_start = np.zeros(nb + 1, np.int32)
+    block_start[1:] = np.cumsum([len(o) for o in offs_per])
+    offsets = np.array(sum(offs_per, []), np.int32)
+    values = rng.standard_normal(len(offsets) * bw)
+    values[rng.random(values.size) < 0.2] = 0.0
+    x = rng.standard_normal(N)
+    x[rng.random(N) < 0.02] = np.nan
+    coo = None
+    if with_coo:
+        m = N
+        r = np.sort(rng.integers(0, N, m)).astype(np.int32)
+        coo = (r, rng.integers(0, N, m).astype(np.int32), rng.standard_normal(m))
+    return values, offsets, block_start, x, coo
+
+
+@pytest.mark.parametrize("N,with_coo,share", [(1000, False, 0.8), (1000, True, 0.8), (4096, True, 1.0),
+                                             (777, False, 0.5), (64, True, 1.0), (96, False, 1.0)])
+def test_width32_shared_offsets_are_bit_identical(gpu, N, with_coo, share):
+    """Width-32 waves whose two blocks share their offsets run the uniform sums (with the runs-of-three
+    hint where it applies): results equal the hint-free launch and the oracle."""
+    print(f"case: width32 N={N} coo={with_coo} share={share}", flush=True)
+    values, offsets, block_start, x, coo = _width32_case(N, 90 + N, with_coo, share)
+    L = _layout(values, offsets, block_start, N, 32, coo)
+    y0 = np.linspace(-1.0, 1.0, N)
+    X = DeviceArray.from_host(x)
+    outs = []
+    for hints in (False, True):
+        Y = DeviceArray.from_host(y0)
+        L.spmv(X, Y, hints=hints)
+        synchronize()
+        outs.append(Y.download())
+    np.testing.assert_array_equal(outs[1], outs[0])
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": len(block_start) - 1,
+          "block_width": 32}
+    if coo is not None:
+        hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
+    else:
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0))
+    np.testing.assert_array_equal(outs[1], oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N))
+
+
+def test_reference_configuration_at_width32(gpu):
+    """The reference's own setup (create3DStencil27Point, block width 32, threshold 8) through
+    hbdiaSpMV: the runs-of-three hint is built for width 32 and the result equals the oracle."""
+    A = HBDIA()
+    A.create3DStencil27Point(32, 32, 32, 0.0)
+    A.convertToHBDIAFormat(32, 8)
+    assert A.info()["runs3_blocks"] == A.getNumBlocks()  # each block is one grid line: whole runs
+    N = 32 ** 3
+    x = np.arange(N, dtype=np.float64)
+    X, Y = HBDIAVector(x), HBDIAVector(np.zeros(N))
+    hbdiaSpMV(A, X, Y, False, True)
+    A.synchronize()
+    hb = A.getHBDIA()
+    hb["num_blocks"], hb["block_width"] = A.getNumBlocks(), 32
+    np.testing.assert_array_equal(Y.download(), oracle.hbdia_spmv(hb, x.copy(), np.zeros(N), N))
+
+
 def test_stencil27_gets_the_runs3_hint_and_stays_exact(gpu):
     A = HBDIA.stencil(20, 27, 0.01, seed=77)
     A.convertToHBDIAFormat(64, 2)
