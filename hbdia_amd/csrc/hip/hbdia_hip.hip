@@ -1187,6 +1187,248 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
   }
 }
 
+// Two adjacent rows per lane for the reduced-precision kernel (block width a
+// multiple of 64, runs-of-three hints): a 128-row wave loads both rows' stored
+// values of an offset in one access and both rows' x of a run's centre in one
+// 16-byte load; the per-row arithmetic, flags and stores are k_hbdia_mixed's.
+
+// both rows' stored values of one offset: p at an even element
+__device__ __forceinline__ void lp_pair(const float* p, float& a, float& b) {
+  typedef float f2a8 __attribute__((ext_vector_type(2), aligned(8)));
+  const f2a8 t = __builtin_nontemporal_load(reinterpret_cast<const f2a8*>(p));
+  a = t.x;
+  b = t.y;
+}
+__device__ __forceinline__ void lp_pair(const uint16_t* p, float& a, float& b) {
+  const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+  a = bf16_to_f32(static_cast<uint16_t>(w & 0xffffu));
+  b = bf16_to_f32(static_cast<uint16_t>(w >> 16));
+}
+
+// FP32 sum and sum of |a x| of one row, the blocked part (rows r0 + lane of a
+// 64-row half), exactly k_hbdia_mixed's arithmetic
+template <typename TV>
+__device__ __forceinline__ void mixed_half(const TV* __restrict__ vals, const int* __restrict__ offs,
+                                           const double* __restrict__ x, int64_t r0, int64_t b, int start, int cnt,
+                                           int bw, int lane, int64_t nrows, bool runs, float& sum, float& sabs) {
+  sum = 0.0f;
+  sabs = 0.0f;
+  if (cnt <= 0) return;
+  const int64_t row = r0 + lane;
+  const bool active = row < nrows;
+  const int64_t last = nrows - 1;
+  const TV* vb = vals + static_cast<int64_t>(start) * bw + (r0 - b * bw);
+  const int* op = offs + start;
+  for (int o0 = 0; o0 < cnt; o0 += kChunk) {
+    int off[kChunk];
+    float v[kChunk];
+    double xl[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      v[k] = lp_to_f32<TV>(__builtin_nontemporal_load(vb + static_cast<int64_t>(min(o0 + k, cnt - 1)) * bw + lane));
+    if (runs) {
+      runs3_x<double, false, kChunk / 3>(x, off, r0, row, last, xl);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        int64_t c = row + off[k];
+        c = c < 0 ? 0 : (c > last ? last : c);
+        xl[k] = x[c];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const int64_t c = row + off[k];
+      const bool ok = (o0 + k < cnt) & active & (v[k] != 0.0f) & (c >= 0) & (c < nrows);
+      const float xf = static_cast<float>(xl[k]);
+      sum = ok ? __builtin_fmaf(v[k], xf, sum) : sum;
+      sabs = ok ? __builtin_fmaf(__builtin_fabsf(v[k]), __builtin_fabsf(xf), sabs) : sabs;
+    }
+  }
+}
+
+// The fallback, the cancellation flag and the store of one 64-row group (rows
+// r0 + lane), exactly k_hbdia_mixed's
+template <typename TV, bool HAS_COO>
+__device__ __forceinline__ void mixed_finish(int64_t grp, int64_t r0, int lane, float sum, float sabs, int cnt,
+                                             const double* __restrict__ x, double* __restrict__ y, int64_t nrows,
+                                             const int* __restrict__ cr, const int* __restrict__ cc,
+                                             const TV* __restrict__ cv, const int* __restrict__ gptr, float ratio,
+                                             float u_store, uint64_t* __restrict__ flag_mask, float* s_v, float* s_x,
+                                             int* s_lo, int* s_hi) {
+  const int64_t row = r0 + lane;
+  const bool active = row < nrows;
+  float cs = 0.0f, cabs = 0.0f;
+  int nterms = cnt;
+  if constexpr (HAS_COO) {
+    const int p0 = gptr[grp], p1 = gptr[grp + 1];
+    for (int base = p0; base < p1; base += kWave) {
+      const int i = base + lane;
+      const bool ok = i < p1;
+      int r = -1;
+      float vv = 0.0f, xx = 0.0f;
+      if (ok) {
+        r = cr[i] - static_cast<int>(r0);
+        vv = lp_to_f32<TV>(cv[i]);
+        xx = static_cast<float>(x[cc[i]]);
+        if (r < 0 || r >= kWave) r = -1;
+      }
+      const int rp = __shfl_up(r, 1);
+      const int rn = __shfl_down(r, 1);
+      s_lo[lane] = 0;
+      s_hi[lane] = 0;
+      s_v[lane] = vv;
+      s_x[lane] = xx;
+      wave_sync();
+      if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
+      if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p1 || rn != r)) s_hi[r] = lane + 1;
+      wave_sync();
+      const int a = s_lo[lane], e = s_hi[lane];
+      nterms += e - a;
+      for (int k = a; k < e; ++k) {
+        cs = __builtin_fmaf(s_v[k], s_x[k], cs);
+        cabs = __builtin_fmaf(__builtin_fabsf(s_v[k]), __builtin_fabsf(s_x[k]), cabs);
+      }
+      wave_sync();
+    }
+  }
+  const float tot = sum + cs;
+  const float mag = sabs + cabs;
+  const float u32 = 5.9604645e-08f;  // 2^-24
+  const float err = (u_store + u32 * static_cast<float>(nterms + 2)) * mag * 1.0001f;
+  const bool fix = active && (!((__builtin_fabsf(tot) - err) * ratio >= mag) || !__builtin_isfinite(tot)) &&
+                   (cnt > 0 || cs != 0.0f || cabs != 0.0f);
+  const uint64_t m = __ballot(fix);
+  if (lane == 0) flag_mask[grp] = m;
+  if (!active || fix) return;
+  if constexpr (HAS_COO) {
+    const double base = cnt > 0 ? static_cast<double>(sum) : y[row];
+    __builtin_nontemporal_store(base + static_cast<double>(cs), y + row);
+  } else {
+    if (cnt > 0) __builtin_nontemporal_store(static_cast<double>(sum), y + row);
+  }
+}
+
+template <typename TV, bool HAS_COO>
+__global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
+    const TV* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb, int bw,
+    const int* __restrict__ cr, const int* __restrict__ cc, const TV* __restrict__ cv,
+    const int* __restrict__ gptr, float ratio, float u_store, uint64_t* __restrict__ flag_mask,
+    const uint32_t* __restrict__ runs3_blocks, int kshift) {
+  __shared__ float s_v[kWavesPerWG][kWave];
+  __shared__ float s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  __shared__ float s_y[kWavesPerWG][4 * kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned work = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  const int64_t w2 = static_cast<int64_t>(work) * kWavesPerWG + wv;  // 128-row wave
+  const int64_t row0 = w2 * 2 * kWave;
+  if (row0 >= nrows) return;
+  const int64_t bA = row0 / bw, bB = (row0 + kWave) / bw;
+  const bool hasB = row0 + kWave < nrows;
+  int sA = 0, cA = 0, sB = 0, cB = 0;
+  if (bA < nb) {
+    sA = blk[bA];
+    cA = blk[bA + 1] - sA;
+  }
+  if (hasB && bB < nb) {
+    sB = blk[bB];
+    cB = blk[bB + 1] - sB;
+  }
+  const bool runsA = bA < nb && ((runs3_blocks[bA >> 5] >> (bA & 31)) & 1u);
+  const bool runsB = hasB && bB < nb && ((runs3_blocks[bB >> 5] >> (bB & 31)) & 1u);
+  bool same = false;
+  if (runsA && cA > 0 && cA <= kWave) {
+    if (bA == bB) {
+      same = true;
+    } else if (hasB && cB == cA) {
+      const int oa = lane < cA ? offs[sA + lane] : 0;
+      const int ob = lane < cA ? offs[sB + lane] : 0;
+      same = __ballot(oa != ob) == 0;
+    }
+  }
+  float sumA, absA, sumB = 0.0f, absB = 0.0f;
+  if (same) {
+    const int64_t row = row0 + 2 * lane;
+    const bool act_a = row < nrows, act_b = row + 1 < nrows;
+    const int64_t last = nrows - 1;
+    const TV* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
+    const int vlane = 2 * lane + ((lane >= kWave / 2 && bB != bA) ? (cA - 1) * bw : 0);
+    const int* op = offs + sA;
+    float sa = 0.0f, aa = 0.0f, sb = 0.0f, ab = 0.0f;
+    for (int o0 = 0; o0 < cA; o0 += kChunk) {
+      int off[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
+      float va[kChunk], vbv[kChunk];
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k)
+        lp_pair(vb + static_cast<int64_t>(min(o0 + k, cA - 1)) * bw + vlane, va[k], vbv[k]);
+      double xa[kChunk], xb[kChunk];
+#pragma unroll
+      for (int w = 0; w < kChunk / 3; ++w) {
+        // runs past the block's count read padding: clamped, their products unused
+        const int64_t o = off[3 * w + 1];
+        dbl2a8 m;
+        if (row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last) {
+          m = *reinterpret_cast<const dbl2a8*>(x + (row + o));
+        } else {
+          int64_t c0 = row + o, c1 = row + 1 + o;
+          c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+          c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+          m.x = x[c0];
+          m.y = x[c1];
+        }
+        int64_t cf = row0 + o - 1, ce = row0 + o + 2 * kWave;
+        cf = cf < 0 ? 0 : (cf > last ? last : cf);
+        ce = ce < 0 ? 0 : (ce > last ? last : ce);
+        const double first = x[cf], after = x[ce];
+        xa[3 * w] = wave_shr1(m.y, first);
+        xa[3 * w + 1] = m.x;
+        xa[3 * w + 2] = m.y;
+        xb[3 * w] = m.x;
+        xb[3 * w + 1] = m.y;
+        xb[3 * w + 2] = wave_shl1(m.x, after);
+      }
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        const int64_t ca = row + off[k], cb = ca + 1;
+        const bool oka = (o0 + k < cA) & act_a & (va[k] != 0.0f) & (ca >= 0) & (ca < nrows);
+        const bool okb = (o0 + k < cA) & act_b & (vbv[k] != 0.0f) & (cb >= 0) & (cb < nrows);
+        const float xfa = static_cast<float>(xa[k]), xfb = static_cast<float>(xb[k]);
+        sa = oka ? __builtin_fmaf(va[k], xfa, sa) : sa;
+        aa = oka ? __builtin_fmaf(__builtin_fabsf(va[k]), __builtin_fabsf(xfa), aa) : aa;
+        sb = okb ? __builtin_fmaf(vbv[k], xfb, sb) : sb;
+        ab = okb ? __builtin_fmaf(__builtin_fabsf(vbv[k]), __builtin_fabsf(xfb), ab) : ab;
+      }
+    }
+    float* sy = s_y[wv];
+    sy[2 * lane] = sa;
+    sy[2 * lane + 1] = sb;
+    sy[2 * kWave + 2 * lane] = aa;
+    sy[2 * kWave + 2 * lane + 1] = ab;
+    wave_sync();
+    sumA = sy[lane];
+    sumB = sy[kWave + lane];
+    absA = sy[2 * kWave + lane];
+    absB = sy[3 * kWave + lane];
+    wave_sync();
+  } else {
+    mixed_half<TV>(vals, offs, x, row0, bA, sA, bA < nb ? cA : 0, bw, lane, nrows, runsA, sumA, absA);
+    if (hasB) mixed_half<TV>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB, sumB, absB);
+  }
+  mixed_finish<TV, HAS_COO>(2 * w2, row0, lane, sumA, absA, cA, x, y, nrows, cr, cc, cv, gptr, ratio, u_store,
+                            flag_mask, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv]);
+  if (hasB)
+    mixed_finish<TV, HAS_COO>(2 * w2 + 1, row0 + kWave, lane, sumB, absB, cB, x, y, nrows, cr, cc, cv, gptr, ratio,
+                              u_store, flag_mask, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv]);
+}
+
 // FP64 recomputation of one row with the FP64 kernel's arithmetic: blocked
 // part in ascending offset order (FMA, zero and range skips), fallback entries
 // of the row in entry order (FMA), y = (bdia or old y) + coo.
@@ -1451,17 +1693,28 @@ int launch_mixed(int vdtype, const void* vals_lp, const double* vals64, const in
   const float r = static_cast<float>(ratio);
   const float u_store = vdtype == HBDIA_VALUES_F32 ? 5.9604645e-08f : 3.9062500e-03f + 5.9604645e-08f;
   const int ks = xcd_chunk_shift(grid.x);
+  if (runs3) {
+    // 128-row waves, two rows per lane
+    const int64_t wgs2 = (nrows + 2 * kWG - 1) / (2 * kWG);
+    const dim3 grid2(static_cast<unsigned>(wgs2));
+    const int ks2 = xcd_chunk_shift(wgs2);
+#define HBDIA_LAUNCH_MX2(TV, C)                                                                                   \
+  hipLaunchKernelGGL((k_hbdia_mixed_row2<TV, C>), grid2, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, x, \
+                     y, nrows, nb, bw, cr, cc, static_cast<const TV*>(cv_lp), gptr, r, u_store, mask, runs3, ks2)
+    if (vdtype == HBDIA_VALUES_F32) { if (with_coo) HBDIA_LAUNCH_MX2(float, true); else HBDIA_LAUNCH_MX2(float, false); }
+    else { if (with_coo) HBDIA_LAUNCH_MX2(uint16_t, true); else HBDIA_LAUNCH_MX2(uint16_t, false); }
+#undef HBDIA_LAUNCH_MX2
+  } else {
 #define HBDIA_LAUNCH_MX(TV, C, R)                                                                             \
   hipLaunchKernelGGL((k_hbdia_mixed<TV, C, R>), grid, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, x, \
                      y, nrows, nb, bw, cr, cc, static_cast<const TV*>(cv_lp), gptr, r, u_store, mask, runs3, ks)
   if (vdtype == HBDIA_VALUES_F32) {
-    if (runs3) { if (with_coo) HBDIA_LAUNCH_MX(float, true, true); else HBDIA_LAUNCH_MX(float, false, true); }
-    else { if (with_coo) HBDIA_LAUNCH_MX(float, true, false); else HBDIA_LAUNCH_MX(float, false, false); }
+    if (with_coo) HBDIA_LAUNCH_MX(float, true, false); else HBDIA_LAUNCH_MX(float, false, false);
   } else {
-    if (runs3) { if (with_coo) HBDIA_LAUNCH_MX(uint16_t, true, true); else HBDIA_LAUNCH_MX(uint16_t, false, true); }
-    else { if (with_coo) HBDIA_LAUNCH_MX(uint16_t, true, false); else HBDIA_LAUNCH_MX(uint16_t, false, false); }
+    if (with_coo) HBDIA_LAUNCH_MX(uint16_t, true, false); else HBDIA_LAUNCH_MX(uint16_t, false, false);
   }
 #undef HBDIA_LAUNCH_MX
+  }
   const int64_t scan_waves = (groups + kWave - 1) / kWave;
   const int64_t fix_blocks = std::min<int64_t>((scan_waves + 3) / 4, 2048);
   hipLaunchKernelGGL(k_fix_groups, dim3(static_cast<unsigned>(fix_blocks)), dim3(256), 0, s, vals64, offs, blk, x, y,
