@@ -12,6 +12,9 @@ in HBM.
 ref128: the reference's published configuration -- create3DStencil27Point(128,
 128, 128, 0.0), block width 32, threshold 8, x[i] = i.
 
+ranklocal: rank 3 of an 8-way row partition of the c2 matrix at its real shape
+(2,097,152 rows), halo columns in the COO part, x's halo filled on the host.
+
 Prints one JSON line: GFLOP/s of the whole job, the roofline of the dominant
 kernel (algorithmic bytes per launch / average launch time from HIP events
 on the kernel's stream, against 8 TB/s), and the CPU baseline (the oracle's
@@ -45,8 +48,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", choices=["c2", "ref128", "precision", "multirhs", "batch", "serve", "copy", "convert"],
-                   default="c2")
+    p.add_argument("--workload", choices=["c2", "ref128", "precision", "multirhs", "batch", "serve", "copy", "convert",
+                                          "ranklocal"], default="c2")
     p.add_argument("--count", type=int, default=None, help="override the number of batched matrices (testing only)")
     p.add_argument("--grid", type=int, default=None, help="override the grid edge (testing only)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full-size parity check")
@@ -71,6 +74,8 @@ def main() -> int:
         return bench_copy(args)
     if args.workload == "convert":
         return bench_convert(args)
+    if args.workload == "ranklocal":
+        return bench_ranklocal(args)
 
     import hbdia_amd
     from hbdia_amd import HBDIA, Event, HBDIAVector, gen_vector, hbdiaSpMV
@@ -664,6 +669,125 @@ def bench_convert(args) -> int:
             "method_s": round(min(method_s), 3), "host_convert_upload_s": round(min(host_s), 3),
             "host_threads": hbdia_amd.api.capi.hbdia_get_num_threads(),
             "host_over_method": round(min(host_s) / min(method_s), 2), "identical": same, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+
+
+def bench_ranklocal(args) -> int:
+    """One rank of eight of the c2 matrix on one GPU: rows [3q, 4q) (q = N / 8) with global-row
+    offsets, the columns owned by other ranks in the COO part, x = [left halo | local | right halo]
+    filled from the global x.  Parity: bit-exact against the oracle's rank-local SpMV on the same
+    layout; CPU baseline: that oracle SpMV on the host cores."""
+    import hbdia_amd
+    from hbdia_amd import HBDIA, HBDIAVector, gen_vector, hbdiaSpMV
+
+    if not hbdia_amd.gpu_available():
+        print("no HIP device visible", file=sys.stderr)
+        return 3
+    n = args.grid or 256
+    size, rank, bw, thr, seed = 8, 3, 64, 2, 0xC2
+    t0 = time.time()
+    G = HBDIA.stencil(n, 27, 0.005, seed=seed)
+    N = n ** 3
+    x = gen_vector(N, seed + 1)
+    log(f"generated {n}^3 stencil + scatter in {time.time() - t0:.1f}s")
+    t1 = time.time()
+    A = HBDIA.createRankLocalMatrix(G, rank, size, bw, thr)
+    G.close()
+    log(f"rank {rank} of {size}: partitioned, converted, data ranges in {time.time() - t1:.1f}s")
+    t1 = time.time()
+    A.prepareForGPU()
+    lay = A.getLocalLayout()
+    info = A.info()
+    nl = A.getNumRows()
+    r0, L, R = lay["first_row"], lay["halo_left"], lay["halo_right"]
+    log(f"rank-local COO built + uploaded in {time.time() - t1:.1f}s: rows {nl} from {r0}, halo {L} + {R}, "
+        f"COO part {lay['coo']} ({lay['folded']} from the blocked part), {lay['ranges']} data ranges")
+    X = HBDIAVector.withHalo(x[r0:r0 + nl], A)
+    X.fillHalo(A, x)
+    Y = HBDIAVector(np.zeros(nl))
+
+    def step():
+        hbdiaSpMV(A, X, Y, False, True)
+
+    step()
+    A.synchronize()
+    y_gpu = Y.download()
+    nnz = info["nnz"]
+    ncoo = lay["coo"]
+    nnz_bdia = nnz - ncoo
+    flop = 2.0 * nnz
+    alg_bytes = nnz_bdia * 8 + ncoo * (8 + 4 + 4) + (nl + L + R) * 8 + nl * 8
+
+    cpu_baseline = None
+    parity = None
+    if not args.no_cpu:
+        import oracle  # the checker and the CPU baseline only
+        hb = A.getHBDIA()
+        hb["num_blocks"], hb["block_width"] = A.getNumBlocks(), bw
+        loc = {"hb": hb, "num_rows": nl, "halo_left": L, "coo_rows": lay["coo_rows"], "coo_cols": lay["coo_cols"],
+               "coo_values": lay["coo_values"]}
+        h = lay["halo_cols"]
+        x_ext = np.concatenate([x[h[:L]], x[r0:r0 + nl], x[h[L:]]])
+        y_cpu = oracle.rank_local_spmv(loc, x_ext, np.zeros(nl))
+        diff = y_gpu != y_cpu
+        rel = np.abs(y_gpu - y_cpu) / np.maximum(np.abs(y_cpu), 1e-300)
+        parity = {"rows": nl, "rows_differing": int(diff.sum()), "max_rel_err": float(rel.max()),
+                  "bit_exact": bool(not diff.any())}
+        log(f"parity vs oracle (rank-local layout): {parity}")
+        if rel.max() > 1e-12:
+            print(json.dumps({"error": "parity failure", "parity": parity}))
+            return 4
+        times = []
+        for rep in range(args.cpu_reps + 1):
+            ts = time.perf_counter()
+            oracle.rank_local_spmv(loc, x_ext, np.zeros(nl))
+            te = time.perf_counter()
+            if rep > 0:
+                times.append(te - ts)
+        t_cpu = float(np.median(times))
+        cpu_baseline = {"value": round(flop / t_cpu / 1e9, 3), "unit": "GFLOP/s", "cores": oracle.max_threads(),
+                        "kind": "port", "ms_per_spmv": round(t_cpu * 1e3, 3),
+                        "sample": f"the rank's full matrix, median of {args.cpu_reps} SpMVs after 1 warm-up "
+                                  f"(oracle: bdia_spmv_kernel over the local segment + cpu_coo_spmv_kernel over "
+                                  f"the rank-local COO part + add, OpenMP)"}
+        log(f"cpu baseline: {cpu_baseline}")
+
+    stream = A.device_ptrs()["bdia_stream"]
+    wall_ms, kern_ms = _time_launches(step, A.synchronize, stream, args.steps, args.warmup)
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    value = flop / (wall_ms * 1e-3) / 1e9
+    log(f"{args.steps} steps: {wall_ms:.4f} ms/step wall, {kern_ms:.4f} ms/launch, {achieved:.1f} GB/s algorithmic")
+    line = {
+        "metric": "SpMV GFLOP/s (FP64 hbdiaSpMV) + % HBM roofline",
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"rank_local_{rank}_of_{size}_stencil27_scatter_{n}^3", "grid": n, "rank": rank,
+                   "ranks": size, "rows": nl, "first_row": r0, "halo_left": L, "halo_right": R,
+                   "block_width": bw, "threshold": thr, "seed": seed, "coo_exec": "gpu_fused"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu_baseline,
+        "kernel_ms": round(kern_ms, 5),
+        "algorithmic_bytes": int(alg_bytes),
+        "flop": int(flop),
+        "nnz_bdia": int(nnz_bdia),
+        "nnz_coo": int(ncoo),
+        "nnz_coo_folded": int(lay["folded"]),
+        "parity": parity,
+        "setup_s": round(time.time() - t0, 1),
+    }
     print(json.dumps(line), flush=True)
     return 0
 

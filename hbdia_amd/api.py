@@ -36,6 +36,8 @@ _INFO = ["rows", "cols", "nnz", "has_coo", "has_hbdia", "block_width", "threshol
 _REPORT = ["rows", "cols", "nnz", "number_diagonals", "num_blocks", "block_width", "threshold", "max_coo_entries",
            "coo_fallback_entries", "hbdia_data_size", "stored_offsets", "value_bytes", "hbdia_bytes", "coo_bytes",
            "dia_bytes", "sell_bytes", "csr_bytes", "sell_applies", "hist_rows"]
+# hbdia_matrix_local_info entries (HBDIA_LOCAL_* in include/hbdia_capi.h)
+_LOCAL = ["first_row", "halo_left", "halo_right", "coo", "folded", "ranges"]
 
 
 class ExecutionMode(enum.IntEnum):
@@ -262,6 +264,56 @@ class HBDIA:
         check_capi(capi.hbdia_matrix_get_hbdia(self._h, *(a.ctypes.data for a in out.values())))
         return out
 
+    # ---- rank-local (partial) matrices -----------------------------------
+    @classmethod
+    def createRankLocalMatrix(cls, globalMatrix: "HBDIA", rank: int, size: int, blockWidth: int = DEFAULT_BLOCK_WIDTH,
+                              threshold: int = DEFAULT_THRESHOLD, max_coo_entries: int = MAX_COO_ENTRIES,
+                              execMode: ExecutionMode = ExecutionMode.GPU_COO) -> "HBDIA":
+        """Rows [rank*q, rank*q + q) of `globalMatrix` (q = rows // size; the last rank also takes the
+        remainder) as the reference's distributor sets up each rank (src/DataExchange/BasicDistributor.cpp:57-79,
+        partitionRowWise at BasicExtractor.cpp:125-234): partial matrix, host conversion with global-row
+        offsets, data ranges, per-process request lists, vector offsets.  prepareForGPU() then builds the
+        rank-local COO part and uploads."""
+        h = c_void_p(None)
+        try:
+            check_capi(capi.hbdia_matrix_rank_local(byref(h), globalMatrix._h, int(rank), int(size), int(blockWidth),
+                                                    int(threshold), int(max_coo_entries),
+                                                    int(execMode == ExecutionMode.GPU_COO)))
+        except HBDIAError as e:
+            raise _map_error(e, "createRankLocalMatrix") from None
+        return cls(dtype=globalMatrix.dtype, _handle=h.value)
+
+    def buildLocalCOO(self) -> None:
+        try:
+            check_capi(capi.hbdia_matrix_build_local_coo(self._h))
+        except HBDIAError as e:
+            raise _map_error(e, "buildLocalCOO") from None
+
+    def local_info(self) -> dict:
+        arr = np.zeros(len(_LOCAL), np.int64)
+        check_capi(capi.hbdia_matrix_local_info(self._h, arr.ctypes.data))
+        return dict(zip(_LOCAL, (int(v) for v in arr)))
+
+    def getLocalLayout(self) -> dict:
+        """Halo columns (left then right, global ids), the rank-local COO part (columns relative to the
+        local segment), the data ranges as a (k, 2) array, and local_info()'s counts."""
+        i = self.local_info()
+        out = {
+            "halo_cols": np.empty(i["halo_left"] + i["halo_right"], np.int32),
+            "coo_rows": np.empty(i["coo"], np.int32),
+            "coo_cols": np.empty(i["coo"], np.int32),
+            "coo_values": np.empty(i["coo"], self.dtype),
+            "data_ranges": np.empty((i["ranges"], 2), np.int32),
+        }
+        check_capi(capi.hbdia_matrix_get_local(self._h, *(a.ctypes.data for a in out.values())))
+        out.update(i)
+        return out
+
+    def getVectorOffsets(self) -> np.ndarray:
+        out = np.empty(self.info()["num_offsets"], np.int32)
+        check_capi(capi.hbdia_matrix_get_vector_offsets(self._h, out.ctypes.data))
+        return out
+
     def getOffsetsPerBlock(self) -> list[np.ndarray]:
         hb = self.getHBDIA()
         bs = hb["block_start"]
@@ -302,6 +354,25 @@ class HBDIAVector:
                                             int(unifiedMemoryMalloc), int(unifiedMemoryManagedMalloc),
                                             int(unifiedNumaAllocOnNode)))
         self._h = h.value
+
+    @classmethod
+    def withHalo(cls, localData, matrix: HBDIA, dtype=None) -> "HBDIAVector":
+        """HBDIAVector(localData, matrix, rank, size) of the reference: [left halo | local | right halo],
+        halo sizes from the rank-local matrix's request lists."""
+        a = np.ascontiguousarray(localData, dtype=dtype or matrix.dtype)
+        self = cls.__new__(cls)
+        self.dtype = a.dtype
+        self.size = int(a.size)
+        self.unified = False
+        h = c_void_p(None)
+        check_capi(capi.hbdia_vector_create_halo(byref(h), _code(a.dtype), a.ctypes.data, a.size, matrix._h))
+        self._h = h.value
+        return self
+
+    def fillHalo(self, matrix: HBDIA, globalVector) -> None:
+        """Copies the halo entries from the global vector (the one-GPU stand-in for the halo exchange)."""
+        g = np.ascontiguousarray(globalVector, dtype=self.dtype)
+        check_capi(capi.hbdia_vector_fill_halo(self._h, matrix._h, g.ctypes.data, g.size))
 
     def getLocalSize(self) -> int:
         return self.size

@@ -225,6 +225,17 @@ void HBDIA<T>::moveFrom(HBDIA& o) noexcept {
   partialMatrixMetadata_ = std::move(o.partialMatrixMetadata_);
   rank_ = o.rank_;
   size_ = o.size_;
+  dataRanges = std::move(o.dataRanges);
+  vectorOffsets = std::move(o.vectorOffsets);
+  localCOOBuilt_ = std::exchange(o.localCOOBuilt_, false);
+  localFirstRow_ = o.localFirstRow_;
+  haloLeft_ = o.haloLeft_;
+  haloRight_ = o.haloRight_;
+  haloColumns_ = std::move(o.haloColumns_);
+  localCooRows_ = std::move(o.localCooRows_);
+  localCooCols_ = std::move(o.localCooCols_);
+  localCooValues_ = std::move(o.localCooValues_);
+  localCooFolded_ = o.localCooFolded_;
   hasCOO = o.hasCOO;
   hasHBDIA = o.hasHBDIA;
   hbdiaData_d_ = std::exchange(o.hbdiaData_d_, nullptr);
@@ -993,6 +1004,15 @@ void HBDIA<T>::deleteHBDIAFormat() {
   histogramNnz.clear();
   numberDiagonals = 0;
   hasHBDIA = false;
+  dataRanges.clear();
+  vectorOffsets.clear();
+  localCOOBuilt_ = false;
+  haloLeft_ = haloRight_ = 0;
+  haloColumns_.clear();
+  localCooRows_.clear();
+  localCooCols_.clear();
+  localCooValues_.clear();
+  localCooFolded_ = 0;
 }
 
 template <typename T>
@@ -1010,6 +1030,11 @@ template <typename T>
 void HBDIA<T>::prepareForGPU() {
   if (!hasHBDIA) return;
   cleanupGPUData();
+  // a partial matrix runs rank-local: halo columns go through the COO part
+  if (partialMatrix) buildLocalCOO();
+  const std::vector<int>& fbRows = partialMatrix ? localCooRows_ : cpuRowIndices;
+  const std::vector<int>& fbCols = partialMatrix ? localCooCols_ : cpuColIndices;
+  const std::vector<T>& fbValues = partialMatrix ? localCooValues_ : cpuValues;
 
   const int nb = static_cast<int>(offsetsPerBlock.size());
   std::vector<int> flattenedOffsets, blockStartIndices, blockSizes;
@@ -1054,35 +1079,34 @@ void HBDIA<T>::prepareForGPU() {
   dev_upload(&blockStartIndices_d_, blockStartIndices.data(), blockStartIndices.size());
   dev_upload(&blockSizes_d_, blockSizes.data(), blockSizes.size());
 
-  if (executionMode == ExecutionMode::GPU_COO && !cpuValues.empty()) {
+  if (executionMode == ExecutionMode::GPU_COO && !fbValues.empty()) {
     // The fused kernel needs row-sorted entries; keep the per-row entry order.
-    const bool sorted = std::is_sorted(cpuRowIndices.begin(), cpuRowIndices.end());
+    const bool sorted = std::is_sorted(fbRows.begin(), fbRows.end());
     if (sorted) {
-      dev_upload(&cpuRowIndices_d_, cpuRowIndices.data(), cpuRowIndices.size());
-      dev_upload(&cpuColIndices_d_, cpuColIndices.data(), cpuColIndices.size());
-      dev_upload(&cpuValues_d_, cpuValues.data(), cpuValues.size());
+      dev_upload(&cpuRowIndices_d_, fbRows.data(), fbRows.size());
+      dev_upload(&cpuColIndices_d_, fbCols.data(), fbCols.size());
+      dev_upload(&cpuValues_d_, fbValues.data(), fbValues.size());
     } else {
-      std::vector<uint32_t> idx(cpuValues.size());
+      std::vector<uint32_t> idx(fbValues.size());
       std::iota(idx.begin(), idx.end(), 0u);
-      std::stable_sort(idx.begin(), idx.end(),
-                       [&](uint32_t a, uint32_t b) { return cpuRowIndices[a] < cpuRowIndices[b]; });
+      std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return fbRows[a] < fbRows[b]; });
       std::vector<int> r(idx.size()), c(idx.size());
       std::vector<T> v(idx.size());
       for (size_t q = 0; q < idx.size(); ++q) {
-        r[q] = cpuRowIndices[idx[q]];
-        c[q] = cpuColIndices[idx[q]];
-        v[q] = cpuValues[idx[q]];
+        r[q] = fbRows[idx[q]];
+        c[q] = fbCols[idx[q]];
+        v[q] = fbValues[idx[q]];
       }
       dev_upload(&cpuRowIndices_d_, r.data(), r.size());
       dev_upload(&cpuColIndices_d_, c.data(), c.size());
       dev_upload(&cpuValues_d_, v.data(), v.size());
     }
-    dev_upload(&cpuRowPtr_d_, cpuRowPtr.data(), cpuRowPtr.size());
+    if (!partialMatrix) dev_upload(&cpuRowPtr_d_, cpuRowPtr.data(), cpuRowPtr.size());
     const int64_t groups = (static_cast<int64_t>(numRows) + HBDIA_ROW_GROUP - 1) / HBDIA_ROW_GROUP;
     void* p = nullptr;
     CHECK_HBDIA(hbdia_malloc(&p, static_cast<size_t>(groups + 1) * sizeof(int)));
     cooGroupPtr_d_ = static_cast<int*>(p);
-    CHECK_HBDIA(hbdia_coo_group_ptr(cpuRowIndices_d_, static_cast<int64_t>(cpuValues.size()), numRows,
+    CHECK_HBDIA(hbdia_coo_group_ptr(cpuRowIndices_d_, static_cast<int64_t>(fbValues.size()), numRows,
                                     cooGroupPtr_d_, sBDIA_));
     CHECK_HBDIA(hbdia_stream_synchronize(sBDIA_));
   }

@@ -59,6 +59,8 @@ HBDIABatch<T>::HBDIABatch(const std::vector<const HBDIA<T>*>& ms) {
     const HBDIA<T>* A = ms[m];
     if (!A) throw std::invalid_argument("HBDIABatch: null matrix");
     if (!A->getHasHBDIA()) throw std::invalid_argument("HBDIABatch: matrix " + std::to_string(m) + " not converted");
+    if (A->isPartialMatrix())
+      throw std::invalid_argument("HBDIABatch: matrix " + std::to_string(m) + " is a rank-local (partial) matrix");
     if (A->getNumRows() != A->getNumCols())
       throw std::invalid_argument("HBDIABatch: matrix " + std::to_string(m) + " is not square");
     const int bw = A->getBlockWidth();

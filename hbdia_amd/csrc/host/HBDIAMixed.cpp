@@ -13,6 +13,8 @@ HBDIAMixed::HBDIAMixed(const HBDIA<double>& matrix, StoragePrecision precision, 
     : m_(matrix), precision_(precision), ratio_(cancellationRatio) {
   if (!matrix.isHBDIAFormat() || !matrix.areStreamsInitialized())
     throw std::runtime_error("HBDIAMixed: matrix is not converted and on the device");
+  if (matrix.isPartialMatrix())
+    throw std::invalid_argument("HBDIAMixed: rank-local (partial) matrices run through hbdiaSpMV only");
   if (matrix.getBlockWidth() % HBDIA_ROW_GROUP != 0)
     throw std::invalid_argument("HBDIAMixed: block width must be a multiple of 64");
   if (!(cancellationRatio > 0.0)) throw std::invalid_argument("HBDIAMixed: cancellation ratio must be positive");

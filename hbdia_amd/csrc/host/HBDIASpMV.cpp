@@ -83,8 +83,22 @@ void check_ready(const HBDIA<T>& m, const HBDIAVector<T>& x, const HBDIAVector<T
     throw std::runtime_error("hbdiaSpMV: matrix is not on the device (call prepareForGPU)");
   if (y.getLocalSize() < static_cast<size_t>(m.getNumRows()))
     throw std::invalid_argument("hbdiaSpMV: output vector shorter than the number of rows");
-  if (x.getTotalSize() < static_cast<size_t>(m.getNumRows()))
+  if (x.getLocalSize() < static_cast<size_t>(m.getNumRows()))
     throw std::invalid_argument("hbdiaSpMV: input vector shorter than the number of rows");
+  if (m.isPartialMatrix()) {
+    if (!m.hasLocalCOO()) throw std::runtime_error("hbdiaSpMV: partial matrix without its rank-local layout");
+    if (x.getLeftSize() != static_cast<size_t>(m.getHaloLeftSize()) ||
+        x.getRightSize() != static_cast<size_t>(m.getHaloRightSize()))
+      throw std::invalid_argument("hbdiaSpMV: input vector halo sizes do not match the matrix's halo layout");
+  }
+}
+
+// x as the kernels index it: the local segment for a rank-local matrix (its
+// COO columns are relative to it, negative in the left halo), else the data
+// pointer, as in the reference (HBDIASpMV.cpp:212).
+template <typename T>
+const T* device_x(const HBDIA<T>& m, const HBDIAVector<T>& x) {
+  return m.isPartialMatrix() ? x.getDeviceLocalPtr() : x.getDeviceDataPtr();
 }
 
 }  // namespace
@@ -126,16 +140,20 @@ template <typename T>
 void executeCOOOnCPU(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector, HBDIAVector<T>& outputVector,
                      int numRows, hbdia_stream_t sD2H, hbdia_stream_t sH2D, hbdia_stream_t /*sCOO*/,
                      hbdia_event_t cooEvent) {
+  const bool local = matrix.isPartialMatrix();
+  const auto& rows = local ? matrix.getLocalCOORowIndices() : matrix.getCpuRowIndices();
+  const auto& cols = local ? matrix.getLocalCOOColIndices() : matrix.getCpuColIndices();
+  const auto& vals = local ? matrix.getLocalCOOValues() : matrix.getCpuValues();
   if (outputVector.isUnifiedMemory()) {
-    hbdia_cpu_coo_spmv<T>(matrix.getCpuRowIndices(), matrix.getCpuColIndices(), matrix.getCpuValues(),
-                          inputVector.getDeviceLocalPtr(), numRows, outputVector.getDeviceCOOResultsPtr());
+    hbdia_cpu_coo_spmv<T>(rows, cols, vals, inputVector.getDeviceLocalPtr(), numRows,
+                          outputVector.getDeviceCOOResultsPtr());
     return;
   }
   CHECK_HBDIA(hbdia_memcpy_d2h(inputVector.getHostDataPtr(), inputVector.getDeviceDataPtr(),
                                inputVector.getTotalSize() * sizeof(T), sD2H));
   CHECK_HBDIA(hbdia_stream_synchronize(sD2H));
-  hbdia_cpu_coo_spmv<T>(matrix.getCpuRowIndices(), matrix.getCpuColIndices(), matrix.getCpuValues(),
-                        inputVector.getHostLocalPtr(), numRows, outputVector.getHostCOOResultsPtr());
+  hbdia_cpu_coo_spmv<T>(rows, cols, vals, inputVector.getHostLocalPtr(), numRows,
+                        outputVector.getHostCOOResultsPtr());
   CHECK_HBDIA(hbdia_memcpy_h2d(outputVector.getDeviceCOOResultsPtr(), outputVector.getHostCOOResultsPtr(),
                                static_cast<size_t>(numRows) * sizeof(T), sH2D));
   CHECK_HBDIA(hbdia_event_record(cooEvent, sH2D));
@@ -146,11 +164,11 @@ void executeCOOOnCPU(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector, HBDIAV
 template <typename T>
 void executeCOOOnGPU(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector, HBDIAVector<T>& outputVector,
                      int numRows, hbdia_stream_t sCOO, hbdia_event_t cooEvent) {
-  if (matrix.getCpuRowIndices().empty()) return;
+  if (matrix.numSpMVCOOEntries() == 0) return;
   if (!matrix.hasDeviceCOO())
     throw std::runtime_error("executeCOOOnGPU: COO fallback not on the device (convert with ExecutionMode::GPU_COO)");
   CHECK_HBDIA(hbdia_memset(outputVector.getDeviceCOOResultsPtr(), 0, static_cast<size_t>(numRows) * sizeof(T), sCOO));
-  CHECK_HBDIA(Dev<T>::coo(matrix, inputVector.getDeviceDataPtr(), outputVector.getDeviceCOOResultsPtr(), sCOO));
+  CHECK_HBDIA(Dev<T>::coo(matrix, device_x(matrix, inputVector), outputVector.getDeviceCOOResultsPtr(), sCOO));
   CHECK_HBDIA(hbdia_event_record(cooEvent, sCOO));
 }
 
@@ -161,8 +179,8 @@ bool hbdiaSpMV(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector, HBDIAVector<
   check_ready(matrix, inputVector, outputVector);
   const int numRows = matrix.getNumRows();
   hbdia_stream_t sBDIA = matrix.getBDIAStream();
-  const bool hasCOOWork = !matrix.getCpuValues().empty();
-  const T* x = inputVector.getDeviceDataPtr();
+  const bool hasCOOWork = matrix.numSpMVCOOEntries() > 0;
+  const T* x = device_x(matrix, inputVector);
   T* y = outputVector.getDeviceLocalPtr();
 
   if (hasCOOWork && !execCOOCPU && execCOOGPU) {
@@ -196,6 +214,8 @@ template <typename T>
 bool hbdiaSpMVMulti(HBDIA<T>& matrix, const T* X, int64_t ldx, T* Y, int64_t ldy, int nrhs, bool execCOOGPU) {
   if (!matrix.isHBDIAFormat() || !matrix.areStreamsInitialized())
     throw std::runtime_error("hbdiaSpMVMulti: matrix is not converted and on the device");
+  if (matrix.isPartialMatrix())
+    throw std::invalid_argument("hbdiaSpMVMulti: rank-local (partial) matrices take hbdiaSpMV with halo vectors");
   const bool hasCOOWork = !matrix.getCpuValues().empty();
   if (hasCOOWork && !execCOOGPU) throw std::runtime_error("Invalid COO execution flags");
   if (hasCOOWork && !matrix.hasDeviceCOO())

@@ -163,6 +163,22 @@ void HBDIAVector<T>::upload(const std::vector<T>& data) {
 }
 
 template <typename T>
+void HBDIAVector<T>::uploadHalo(const std::vector<T>& left, const std::vector<T>& right) {
+  if (left.size() != size_recv_left_ || right.size() != size_recv_right_)
+    throw std::invalid_argument("HBDIAVector::uploadHalo: halo sizes " + std::to_string(left.size()) + "+" +
+                                std::to_string(right.size()) + " != " + std::to_string(size_recv_left_) + "+" +
+                                std::to_string(size_recv_right_));
+  if (!left.empty()) {
+    if (left_ptr_h_) std::copy(left.begin(), left.end(), left_ptr_h_);
+    CHECK_HBDIA(hbdia_memcpy(left_ptr_d_, left.data(), left.size() * sizeof(T)));
+  }
+  if (!right.empty()) {
+    if (right_ptr_h_) std::copy(right.begin(), right.end(), right_ptr_h_);
+    CHECK_HBDIA(hbdia_memcpy(right_ptr_d_, right.data(), right.size() * sizeof(T)));
+  }
+}
+
+template <typename T>
 void HBDIAVector<T>::download(std::vector<T>& out) const {
   out.resize(size_local_);
   CHECK_HBDIA(hbdia_memcpy(out.data(), local_ptr_d_, size_local_ * sizeof(T)));

@@ -9,6 +9,7 @@
 #include <memory>
 #include <string>
 
+#include "DataExchange/BasicExtractor.hpp"
 #include "Format/HBDIA.hpp"
 #include "Format/HBDIAVector.hpp"
 #include "Operations/HBDIABatch.hpp"
@@ -153,6 +154,44 @@ template <typename T>
 std::unique_ptr<HBDIA<T>> from_gen(hbdia::CooMatrix<T>& g) {
   const int nr = g.numRows, nc = g.numCols;
   return std::make_unique<HBDIA<T>>(std::move(g.rows), std::move(g.cols), std::move(g.vals), nr, nc);
+}
+
+}  // namespace
+
+namespace {
+
+template <typename T>
+void local_info(const HBDIA<T>& m, int64_t* info) {
+  info[HBDIA_LOCAL_FIRST_ROW] = m.getLocalFirstRow();
+  info[HBDIA_LOCAL_HALO_LEFT] = m.getHaloLeftSize();
+  info[HBDIA_LOCAL_HALO_RIGHT] = m.getHaloRightSize();
+  info[HBDIA_LOCAL_COO] = static_cast<int64_t>(m.getLocalCOOValues().size());
+  info[HBDIA_LOCAL_FOLDED] = m.getLocalCOOFolded();
+  info[HBDIA_LOCAL_RANGES] = static_cast<int64_t>(m.getDataRanges().size());
+}
+
+template <typename T>
+void get_local(const HBDIA<T>& m, int32_t* halo, int32_t* cr, int32_t* cc, void* cv, int32_t* ranges) {
+  if (halo) std::copy(m.getHaloColumns().begin(), m.getHaloColumns().end(), halo);
+  if (cr) std::copy(m.getLocalCOORowIndices().begin(), m.getLocalCOORowIndices().end(), cr);
+  if (cc) std::copy(m.getLocalCOOColIndices().begin(), m.getLocalCOOColIndices().end(), cc);
+  if (cv) std::copy(m.getLocalCOOValues().begin(), m.getLocalCOOValues().end(), static_cast<T*>(cv));
+  if (ranges) {
+    const auto& dr = m.getDataRanges();
+    for (size_t i = 0; i < dr.size(); ++i) {
+      ranges[2 * i] = std::get<0>(dr[i]);
+      ranges[2 * i + 1] = std::get<1>(dr[i]);
+    }
+  }
+}
+
+template <typename T>
+void get_vector_offsets(const HBDIA<T>& m, int32_t* out) {
+  int64_t pos = 0;
+  for (const auto& vo : m.getVectorOffsets()) {
+    std::copy(vo.begin(), vo.end(), out + pos);
+    pos += static_cast<int64_t>(vo.size());
+  }
 }
 
 }  // namespace
@@ -428,6 +467,83 @@ int hbdia_spmv(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_co
     need(m->dtype == x->dtype && m->dtype == y->dtype, "dtype mismatch between matrix and vectors");
     if (m->dtype == HBDIA_DTYPE_F64) hbdiaSpMV<double>(*m->d, *x->d, *y->d, exec_coo_cpu != 0, exec_coo_gpu != 0);
     else hbdiaSpMV<float>(*m->f, *x->f, *y->f, exec_coo_cpu != 0, exec_coo_gpu != 0);
+  });
+}
+
+int hbdia_matrix_rank_local(hbdia_matrix_t* local, hbdia_matrix_t global, int rank, int size, int block_width,
+                            int threshold, int max_coo_entries, int gpu_coo) {
+  return guard("hbdia_matrix_rank_local", [&] {
+    need(local != nullptr && global != nullptr, "null handle");
+    const ExecutionMode em = gpu_coo ? ExecutionMode::GPU_COO : ExecutionMode::CPU_COO;
+    auto h = std::make_unique<hbdia_matrix_opaque>();
+    h->dtype = global->dtype;
+    if (global->dtype == HBDIA_DTYPE_F64)
+      h->d = std::make_unique<HBDIA<double>>(
+          createRankLocalMatrix<double>(*global->d, rank, size, block_width, threshold, max_coo_entries, em));
+    else
+      h->f = std::make_unique<HBDIA<float>>(
+          createRankLocalMatrix<float>(*global->f, rank, size, block_width, threshold, max_coo_entries, em));
+    *local = h.release();
+  });
+}
+
+int hbdia_matrix_build_local_coo(hbdia_matrix_t m) {
+  return guard("hbdia_matrix_build_local_coo", [&] { DISPATCH(m, M.buildLocalCOO(), M.buildLocalCOO()); });
+}
+
+
+int hbdia_matrix_local_info(hbdia_matrix_t m, int64_t* info) {
+  return guard("hbdia_matrix_local_info", [&] {
+    need(info != nullptr, "null info");
+    DISPATCH(m, local_info(M, info), local_info(M, info));
+  });
+}
+
+int hbdia_matrix_get_local(hbdia_matrix_t m, int32_t* halo_cols, int32_t* coo_rows, int32_t* coo_cols,
+                           void* coo_values, int32_t* ranges) {
+  return guard("hbdia_matrix_get_local", [&] {
+    DISPATCH(m, get_local(M, halo_cols, coo_rows, coo_cols, coo_values, ranges),
+             get_local(M, halo_cols, coo_rows, coo_cols, coo_values, ranges));
+  });
+}
+
+int hbdia_matrix_get_vector_offsets(hbdia_matrix_t m, int32_t* out) {
+  return guard("hbdia_matrix_get_vector_offsets", [&] {
+    need(out != nullptr, "null output");
+    DISPATCH(m, get_vector_offsets(M, out), get_vector_offsets(M, out));
+  });
+}
+
+int hbdia_vector_create_halo(hbdia_vector_t* v, int dtype, const void* data, int64_t n, hbdia_matrix_t m) {
+  return guard("hbdia_vector_create_halo", [&] {
+    need(v != nullptr && m != nullptr, "null handle");
+    need(n >= 0 && (n == 0 || data), "bad data");
+    need(dtype == m->dtype, "dtype mismatch between matrix and vector");
+    auto h = std::make_unique<hbdia_vector_opaque>();
+    h->dtype = dtype;
+    if (dtype == HBDIA_DTYPE_F64) {
+      std::vector<double> d(static_cast<const double*>(data), static_cast<const double*>(data) + n);
+      h->d = std::make_unique<HBDIAVector<double>>(d, *m->d, m->d->getRank(), m->d->getSize());
+    } else {
+      std::vector<float> d(static_cast<const float*>(data), static_cast<const float*>(data) + n);
+      h->f = std::make_unique<HBDIAVector<float>>(d, *m->f, m->f->getRank(), m->f->getSize());
+    }
+    *v = h.release();
+  });
+}
+
+int hbdia_vector_fill_halo(hbdia_vector_t v, hbdia_matrix_t m, const void* global, int64_t n) {
+  return guard("hbdia_vector_fill_halo", [&] {
+    need(v != nullptr && m != nullptr && n >= 0 && (n == 0 || global), "bad argument");
+    need(v->dtype == m->dtype, "dtype mismatch between matrix and vector");
+    bool ok;
+    if (m->dtype == HBDIA_DTYPE_F64)
+      ok = exchangeHaloFromGlobal<double>(
+          *m->d, *v->d, std::vector<double>(static_cast<const double*>(global), static_cast<const double*>(global) + n));
+    else
+      ok = exchangeHaloFromGlobal<float>(
+          *m->f, *v->f, std::vector<float>(static_cast<const float*>(global), static_cast<const float*>(global) + n));
+    if (!ok) throw std::invalid_argument("halo layout of the vector does not match the matrix");
   });
 }
 

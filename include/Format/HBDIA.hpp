@@ -162,9 +162,51 @@ class HBDIA {
   int getRank() const { return rank_; }
   int getSize() const { return size_; }
   const PartialMatrixMetadata<T>& getPartialMatrixMetadata() const { return partialMatrixMetadata_; }
+  PartialMatrixMetadata<T>& getPartialMatrixMetadataRef() { return partialMatrixMetadata_; }
+  void setPartialMatrixMetadata(const PartialMatrixMetadata<T>& metadata) { partialMatrixMetadata_ = metadata; }
   bool hasPartialMatrixMetadata() const {
     return partialMatrix && partialMatrixMetadata_.totalBufferSize > 0;
   }
+
+  // The global columns a partial matrix reads, as sorted, merged [start, end)
+  // ranges (reference src/Format/HBDIA.cpp:1333-1412): every COO column, and
+  // for every stored (block, offset) the column of each lane whose row exists.
+  void analyzeDataRanges();
+  const std::vector<std::tuple<int, int>>& getDataRanges() const { return dataRanges; }
+  // Per (block, offset), the position of lane 0's x entry in the
+  // [left halo | local | right halo] vector (reference HBDIA.cpp:1034-1178);
+  // INT_MIN where the column is in no range.  A host table: the kernels here
+  // read the halo through the rank-local COO part instead.
+  void calculateVectorOffsets(int rank, int size);
+  const std::vector<std::vector<int>>& getVectorOffsets() const { return vectorOffsets; }
+
+  // Rank-local COO part of a partial matrix (one GPU, no halo exchange).  The
+  // local rows are the contiguous global rows [r0, r0 + numRows), and x is
+  // [left halo | local | right halo] with the halo columns of the partial
+  // metadata: processDataRanges of the ranks below this one, then above.
+  // The blocked part is kept as converted and reads only local columns (its
+  // bound check skips the others); every stored non-zero whose column lies
+  // outside the local rows is added to the COO part instead.  The list is the
+  // fallback entries in their order, then those blocked entries in
+  // (block, offset, lane) order, stably sorted by row; columns are indices
+  // relative to the local segment (negative in the left halo, numRows and up
+  // in the right one).  Built by prepareForGPU for partial matrices; needs
+  // the partial metadata (analyzeDataRanges + extractPartialMatrixMetadata).
+  void buildLocalCOO();
+  bool hasLocalCOO() const { return localCOOBuilt_; }
+  int getLocalFirstRow() const { return localFirstRow_; }
+  int getHaloLeftSize() const { return haloLeft_; }
+  int getHaloRightSize() const { return haloRight_; }
+  // Global column of each halo entry: left halo first, then right halo.
+  const std::vector<int>& getHaloColumns() const { return haloColumns_; }
+  const std::vector<int>& getLocalCOORowIndices() const { return localCooRows_; }
+  const std::vector<int>& getLocalCOOColIndices() const { return localCooCols_; }
+  const std::vector<T>& getLocalCOOValues() const { return localCooValues_; }
+  // Entries of the local COO part that come from the blocked part.
+  int64_t getLocalCOOFolded() const { return localCooFolded_; }
+  // Fallback entries the SpMV runs: the rank-local list for partial
+  // matrices, the conversion's fallback otherwise.
+  size_t numSpMVCOOEntries() const { return partialMatrix ? localCooValues_.size() : cpuValues.size(); }
 
   // format flags
   bool getHasCOO() const { return hasCOO; }
@@ -256,6 +298,17 @@ class HBDIA {
   PartialMatrixMetadata<T> partialMatrixMetadata_;
   int rank_ = -1;
   int size_ = -1;
+  std::vector<std::tuple<int, int>> dataRanges;
+  std::vector<std::vector<int>> vectorOffsets;
+  bool localCOOBuilt_ = false;
+  int localFirstRow_ = 0;
+  int haloLeft_ = 0;
+  int haloRight_ = 0;
+  std::vector<int> haloColumns_;
+  std::vector<int> localCooRows_;
+  std::vector<int> localCooCols_;
+  std::vector<T> localCooValues_;
+  int64_t localCooFolded_ = 0;
 
   bool hasCOO = false;
   bool hasHBDIA = false;

@@ -62,8 +62,14 @@ class HBDIAVector {
   bool isUnifiedMemoryManagedMalloc() const { return unifiedMemoryManagedMalloc_; }
   bool isUnifiedNumaAllocOnNode() const { return unifiedNumaAllocOnNode_; }
 
+  size_t getLeftSize() const { return size_recv_left_; }
+  size_t getRightSize() const { return size_recv_right_; }
+
   // Synchronous copies of the local segment.
   void upload(const std::vector<T>& data);
+  // Synchronous copy of both halo segments (host mirror and device); the
+  // sizes must equal getLeftSize() and getRightSize().
+  void uploadHalo(const std::vector<T>& left, const std::vector<T>& right);
   void download(std::vector<T>& out) const;
   std::vector<T> download() const;
 

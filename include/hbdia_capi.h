@@ -19,6 +19,14 @@
  *   hbdia_vector_create                -> HBDIAVector<T>(localData[, four memory flags]),
  *                                         include/Format/HBDIAVector.hpp:31-32
  *   hbdia_spmv                         -> hbdiaSpMV<T>, include/Operations/HBDIASpMV.hpp:8-10
+ *   hbdia_matrix_rank_local            -> one rank's matrix as the reference's distributor builds it:
+ *                                         BasicExtractor<T>::partitionRowWise (src/DataExchange/
+ *                                         BasicExtractor.cpp:125-234) and BasicDistributor.cpp:57-79
+ *   hbdia_matrix_get_vector_offsets    -> HBDIA<T>::calculateVectorOffsets, src/Format/HBDIA.cpp:1034-1178
+ *   hbdia_vector_create_halo           -> HBDIAVector<T>(localData, matrix, rank, size), HBDIAVector.hpp:33
+ *   hbdia_vector_fill_halo             -> stands in for MPICommunicator<T>::exchangeData on one GPU
+ *   hbdia_matrix_build_local_coo, _local_info, _get_local
+ *                                      -> no reference counterpart: the rank-local COO part
  *   hbdia_spmv_multi                   -> no reference counterpart: many right-hand sides in one launch
  *   hbdia_mixed_*                      -> no reference counterpart: FP32 / BF16 value storage
  *   hbdia_batch_*                      -> no reference counterpart: many matrices in one launch
@@ -149,6 +157,39 @@ int hbdia_vector_download_coo(hbdia_vector_t v, void* out, int64_t n);
 int hbdia_vector_device_ptrs(hbdia_vector_t v, void** local, void** coo);
 
 int hbdia_spmv(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_coo_cpu, int exec_coo_gpu);
+
+/* Rank-local form on one GPU (createRankLocalMatrix, include/DataExchange/BasicExtractor.hpp):
+ * rows [rank*q, rank*q + q) of the global matrix (q = rows / size, the last rank also the
+ * remainder), converted on the host with the global-row offsets, its data ranges and
+ * per-process request lists computed.  hbdia_matrix_prepare_gpu then builds the rank-local
+ * COO part (halo columns folded in) and uploads; hbdia_matrix_build_local_coo builds it
+ * without the upload. */
+int hbdia_matrix_rank_local(hbdia_matrix_t* local, hbdia_matrix_t global, int rank, int size, int block_width,
+                            int threshold, int max_coo_entries, int gpu_coo);
+int hbdia_matrix_build_local_coo(hbdia_matrix_t m);
+enum {
+  HBDIA_LOCAL_FIRST_ROW = 0,  /* global row of local row 0 */
+  HBDIA_LOCAL_HALO_LEFT = 1,  /* entries of x before the local segment */
+  HBDIA_LOCAL_HALO_RIGHT = 2, /* entries of x after it */
+  HBDIA_LOCAL_COO = 3,        /* entries of the rank-local COO part */
+  HBDIA_LOCAL_FOLDED = 4,     /* of which come from the blocked part */
+  HBDIA_LOCAL_RANGES = 5,     /* data ranges (analyzeDataRanges) */
+  HBDIA_LOCAL_COUNT = 6
+};
+int hbdia_matrix_local_info(hbdia_matrix_t m, int64_t* info /* [HBDIA_LOCAL_COUNT] */);
+/* Any output may be NULL.  halo_cols: LEFT + RIGHT global columns (left halo first);
+ * coo_*: LOCAL_COO entries, columns relative to the local segment; ranges: 2 * RANGES
+ * (start, end) pairs. */
+int hbdia_matrix_get_local(hbdia_matrix_t m, int32_t* halo_cols, int32_t* coo_rows, int32_t* coo_cols,
+                           void* coo_values, int32_t* ranges);
+/* calculateVectorOffsets' table, flattened like the offsets (NUM_OFFSETS entries). */
+int hbdia_matrix_get_vector_offsets(hbdia_matrix_t m, int32_t* out);
+/* HBDIAVector(localData, matrix, matrix rank, matrix size): halo segments sized from the
+ * matrix's partial metadata. */
+int hbdia_vector_create_halo(hbdia_vector_t* v, int dtype, const void* data, int64_t n, hbdia_matrix_t m);
+/* Fills the halo segments from the global vector (n entries) on the host and copies them to
+ * the device. */
+int hbdia_vector_fill_halo(hbdia_vector_t v, hbdia_matrix_t m, const void* global, int64_t n);
 /* hbdiaSpMVMulti: X, Y device arrays (column-major, leading dimensions ldx, ldy). */
 int hbdia_spmv_multi(hbdia_matrix_t m, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrhs, int exec_coo_gpu);
 

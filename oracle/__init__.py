@@ -35,6 +35,8 @@ def _load():
         "orc_coo_free_f64": (None, [P]), "orc_coo_free_f32": (None, [P]),
         "orc_convert_f64": (P, [P, P, P, c_int64, c_int, c_int, c_int, c_int, c_int]),
         "orc_convert_f32": (P, [P, P, P, c_int64, c_int, c_int, c_int, c_int, c_int]),
+        "orc_convert_g_f64": (P, [P, P, P, c_int64, c_int, c_int, c_int, c_int, c_int, P]),
+        "orc_convert_g_f32": (P, [P, P, P, c_int64, c_int, c_int, c_int, c_int, c_int, P]),
         "orc_hb_sizes_f64": (None, [P, P]), "orc_hb_sizes_f32": (None, [P, P]),
         "orc_hb_get_f64": (None, [P] * 10), "orc_hb_get_f32": (None, [P] * 10),
         "orc_hb_free_f64": (None, [P]), "orc_hb_free_f32": (None, [P]),
@@ -91,14 +93,22 @@ def stencil27(nx: int, ny: int, nz: int, noise: float = 0.0, iteration: int = 0,
 
 
 def convert(rows, cols, values, num_rows: int, num_cols: int, block_width: int, threshold: int,
-            max_coo_entries: int = 1_000_000_000) -> dict:
-    """Reference convertToHBDIAFormat + prepareForGPU flattening."""
+            max_coo_entries: int = 1_000_000_000, global_rows=None) -> dict:
+    """Reference convertToHBDIAFormat + prepareForGPU flattening; `global_rows` is a partial
+    matrix's global row mapping (offsets col - global row, blocks and lanes local)."""
     v = np.ascontiguousarray(values)
     s = _sfx(v.dtype)
     r = np.ascontiguousarray(rows, dtype=np.int32)
     c = np.ascontiguousarray(cols, dtype=np.int32)
-    h = getattr(_lib, "orc_convert_" + s)(_ptr(r), _ptr(c), _ptr(v), len(v), int(num_rows), int(num_cols),
-                                         int(block_width), int(threshold), int(max_coo_entries))
+    if global_rows is None:
+        h = getattr(_lib, "orc_convert_" + s)(_ptr(r), _ptr(c), _ptr(v), len(v), int(num_rows), int(num_cols),
+                                             int(block_width), int(threshold), int(max_coo_entries))
+    else:
+        g = np.ascontiguousarray(global_rows, dtype=np.int32)
+        if len(g) != num_rows:
+            raise ValueError("global_rows needs one entry per local row")
+        h = getattr(_lib, "orc_convert_g_" + s)(_ptr(r), _ptr(c), _ptr(v), len(v), int(num_rows), int(num_cols),
+                                               int(block_width), int(threshold), int(max_coo_entries), _ptr(g))
     try:
         sz = np.zeros(7, np.int64)
         getattr(_lib, "orc_hb_sizes_" + s)(h, sz.ctypes.data)
@@ -191,3 +201,188 @@ def storage_report(hb: dict, num_rows: int, num_cols: int, nnz: int, nnz_per_row
     return {"hbdia_bytes": hbdia, "coo_bytes": coo, "dia_bytes": dia, "sell_bytes": sell, "csr_bytes": csr,
             "stored_offsets": offsets, "number_diagonals": nd, "num_blocks": nb, "coo_fallback_entries": ncoo,
             "hbdia_data_size": len(hb["values"]), "sell_applies": threshold == 1, "histogram": hist}
+
+
+# ---- one rank's matrix on one GPU ------------------------------------------------------------
+
+def partition_row_wise(rows, cols, values, num_rows: int, size: int) -> list[dict]:
+    """Restates BasicExtractor<T>::partitionRowWise (src/DataExchange/BasicExtractor.cpp:125-234):
+    partition p owns rows [p*q, (p+1)*q - 1] (q = num_rows // size; the last one also the remainder,
+    :136-165), found by the first range that holds the row (:196-205); entries keep their input order
+    with local rows and global columns (:212-217)."""
+    rows = np.asarray(rows, dtype=np.int64)
+    q = num_rows // size
+    rem = num_rows % size
+    if q == 0:
+        raise ValueError("fewer rows than partitions")
+    starts = [p * q for p in range(size)]
+    ends = [starts[p] + q + (rem if p == size - 1 else 0) - 1 for p in range(size)]  # inclusive
+    owner = np.full(len(rows), -1, np.int64)
+    for p in range(size - 1, -1, -1):
+        owner[(rows >= starts[p]) & (rows <= ends[p])] = p
+    out = []
+    for p in range(size):
+        m = owner == p
+        out.append({"rows": (rows[m] - starts[p]).astype(np.int32), "cols": np.asarray(cols, np.int32)[m],
+                    "values": np.asarray(values)[m], "first_row": starts[p], "num_rows": ends[p] - starts[p] + 1})
+    return out
+
+
+def data_ranges(hb: dict, cols, global_rows, num_rows: int, num_global_cols: int) -> list[tuple[int, int]]:
+    """Restates HBDIA<T>::analyzeDataRanges (src/Format/HBDIA.cpp:1333-1412): every COO column
+    (:1357-1360) and, for every stored (block, offset), the global column of each lane (:1366-1384),
+    kept when inside [0, numGlobalCols); sorted and merged into [start, end) runs (:1392-1411).  Lanes
+    past the last row are skipped: the reference reads past its row mapping there."""
+    g = np.asarray(global_rows, np.int64)
+    bw = int(hb["block_width"])
+    bs = hb["block_start"].astype(np.int64)
+    pair_block = np.repeat(np.arange(len(bs) - 1), np.diff(bs))
+    lane_rows = pair_block[:, None] * bw + np.arange(bw)[None, :]
+    ok = lane_rows < num_rows
+    c = g[np.minimum(lane_rows, num_rows - 1)] + hb["offsets"].astype(np.int64)[:, None]
+    ok &= (c >= 0) & (c < num_global_cols)
+    u = np.unique(np.concatenate([np.asarray(cols, np.int64), c[ok]]))
+    if len(u) == 0:
+        return []
+    brk = np.nonzero(np.diff(u) != 1)[0]
+    first = np.concatenate([[0], brk + 1])
+    last = np.concatenate([brk, [len(u) - 1]])
+    return [(int(u[a]), int(u[b]) + 1) for a, b in zip(first, last)]
+
+
+def process_data_ranges(ranges, num_global_rows: int, size: int) -> tuple[list[list[tuple[int, int]]], int]:
+    """Restates BasicExtractor<T>::extractPartialMatrixMetadataRowWise (BasicExtractor.cpp:269-333):
+    each range is cut at the ownership boundaries (p+1)*q, the last process taking the rest; returns
+    the per-process lists and totalBufferSize."""
+    q = num_global_rows // size
+    pdr = [[] for _ in range(size)]
+    total = 0
+    for rs, re in ranges:
+        total += re - rs
+        p0 = max(0, min(rs // q, size - 1))
+        p1 = max(0, min((re - 1) // q, size - 1))
+        at = rs
+        p = p0
+        while p <= p1 and at < re:
+            boundary = re if p == size - 1 else (p + 1) * q
+            end = min(re, boundary)
+            if at < end:
+                pdr[p].append((at, end))
+                at = end
+            p += 1
+    return pdr, total
+
+
+def vector_offsets(hb: dict, pdr, rank: int, size: int, num_global_rows: int, num_rows: int) -> np.ndarray:
+    """Restates HBDIA<T>::calculateVectorOffsets (src/Format/HBDIA.cpp:1034-1122) with
+    findMemoryOffsetForGlobalIndex (:1124-1178): per (block, offset), where lane 0's x entry
+    (rank*q + block*bw + offset) sits in [left | local | right]; INT_MIN when in no range."""
+    left = sum(e - s for p in range(rank) for s, e in pdr[p])
+    q = num_global_rows // size
+    g_start = rank * q
+    g_end = g_start + q + (num_global_rows % q if rank == size - 1 else 0)
+    bw = int(hb["block_width"])
+    bs = hb["block_start"]
+    out = np.empty(len(hb["offsets"]), np.int64)
+
+    def where(g):
+        if g_start <= g < g_end:
+            return left + (g - g_start)
+        if g < 0:
+            return g
+        off = 0
+        if g < g_start:
+            for p in range(rank):
+                for s, e in pdr[p]:
+                    if s <= g < e:
+                        return off + (g - s)
+                    off += e - s
+        else:
+            off = left + num_rows
+            for p in range(rank + 1, len(pdr)):
+                for s, e in pdr[p]:
+                    if s <= g < e:
+                        return off + (g - s)
+                    off += e - s
+        return -(2 ** 31)
+
+    for b in range(len(bs) - 1):
+        for k in range(bs[b], bs[b + 1]):
+            out[k] = where(g_start + b * bw + int(hb["offsets"][k]))
+    return out.astype(np.int32)
+
+
+def rank_local(rows, cols, values, num_rows: int, num_cols: int, rank: int, size: int, block_width: int,
+               threshold: int, max_coo_entries: int = 1_000_000_000) -> dict:
+    """One rank of a row-wise partition set up as the reference's distributor does
+    (src/DataExchange/BasicDistributor.cpp:57-79) and laid out for one GPU: the rank's conversion with
+    global-row offsets, its data ranges and request lists, the halo columns (requests to lower ranks,
+    then to higher ranks) and the rank-local COO part.  The last has no reference counterpart (the
+    reference's rank kernel would read halo entries through vector offsets): the fallback entries in
+    their order, then every stored non-zero whose column is outside the local rows in
+    (block, offset, lane) order, stably sorted by row, columns relative to the local segment."""
+    part = partition_row_wise(rows, cols, values, num_rows, size)[rank]
+    n = part["num_rows"]
+    r0 = part["first_row"]
+    grm = np.arange(r0, r0 + n, dtype=np.int32)
+    hb = convert(part["rows"], part["cols"], part["values"], n, num_cols, block_width, threshold,
+                 max_coo_entries, global_rows=grm)
+    ranges = data_ranges(hb, part["cols"], grm, n, num_cols)
+    pdr, total = process_data_ranges(ranges, num_rows, size)
+    seg = lambda ps: np.concatenate([np.arange(s, e) for p in ps for s, e in pdr[p]] or [np.zeros(0, np.int64)])
+    left = seg(range(rank)).astype(np.int64)
+    right = seg(range(rank + 1, size)).astype(np.int64)
+    L = len(left)
+
+    def local_col(g):
+        g = np.asarray(g, np.int64)
+        out = np.empty(len(g), np.int64)
+        loc = (g >= r0) & (g < r0 + n)
+        out[loc] = g[loc] - r0
+        lo = g < r0
+        il = np.searchsorted(left, g[lo])
+        if np.any(il >= len(left)) or np.any(left[np.minimum(il, len(left) - 1)] != g[lo]):
+            raise ValueError("column in no halo range")
+        out[lo] = il - L
+        hi = g >= r0 + n
+        ir = np.searchsorted(right, g[hi])
+        if np.any(ir >= len(right)) or np.any(right[np.minimum(ir, len(right) - 1)] != g[hi]):
+            raise ValueError("column in no halo range")
+        out[hi] = n + ir
+        return out
+
+    bw = int(block_width)
+    bs = hb["block_start"].astype(np.int64)
+    pair_block = np.repeat(np.arange(len(bs) - 1), np.diff(bs))
+    lane_rows = pair_block[:, None] * bw + np.arange(bw)[None, :]
+    c_local = lane_rows + hb["offsets"].astype(np.int64)[:, None]
+    v = hb["values"].reshape(-1, bw) if len(hb["values"]) else np.zeros((0, bw), hb["values"].dtype)
+    fold = (lane_rows < n) & (v != 0) & ((c_local < 0) | (c_local >= n))
+    fold &= (c_local + r0 >= 0) & (c_local + r0 < num_cols)
+    f_rows = lane_rows[fold]
+    f_cols = local_col(c_local[fold] + r0)
+    f_vals = v[fold]
+    rows_all = np.concatenate([hb["coo_rows"].astype(np.int64), f_rows])
+    cols_all = np.concatenate([local_col(hb["coo_cols"]), f_cols])
+    vals_all = np.concatenate([hb["coo_values"], f_vals])
+    order = np.argsort(rows_all, kind="stable")
+    return {"hb": hb, "first_row": r0, "num_rows": n, "ranges": ranges, "process_data_ranges": pdr,
+            "total_buffer_size": total, "halo_cols": np.concatenate([left, right]).astype(np.int32),
+            "halo_left": L, "halo_right": len(right), "coo_rows": rows_all[order].astype(np.int32),
+            "coo_cols": cols_all[order].astype(np.int32), "coo_values": vals_all[order],
+            "folded": int(fold.sum()), "part": part}
+
+
+def rank_local_spmv(loc: dict, x_ext: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """y (local rows, in/out) for x_ext = [left halo | local | right halo]: the blocked part over the
+    local segment with the kernel's bound check, the rank-local COO part into a zeroed buffer, then
+    y += buffer when the part is non-empty -- the order of hbdia_spmv above."""
+    n = loc["num_rows"]
+    L = loc["halo_left"]
+    x_local = np.ascontiguousarray(x_ext[L:L + n])
+    bdia_spmv(loc["hb"], x_local, y, n)
+    if len(loc["coo_values"]):
+        buf = np.zeros(n, y.dtype)
+        cpu_coo_spmv(loc["coo_rows"], loc["coo_cols"].astype(np.int64) + L, loc["coo_values"], x_ext, n, buf)
+        y += buf
+    return y

@@ -146,15 +146,18 @@ template <typename T> struct Hb {
   std::vector<int> histBlocks, histNnz;
 };
 
-// Restates HBDIA<T>::convertToHBDIAFormat (src/Format/HBDIA.cpp:507-718) for a
-// full (non-partial) matrix, followed by the flattening of prepareForGPU
-// (:736-765).  std::map per block (:540-560), threshold / max_coo_entries rule
+// Restates HBDIA<T>::convertToHBDIAFormat (src/Format/HBDIA.cpp:507-718),
+// followed by the flattening of prepareForGPU (:736-765).  G, when given, is
+// the global row mapping of a partial matrix: offsets are col - G[row] (:543-553,
+// :648-660) while block and lane stay local.  std::map per block (:540-560), threshold / max_coo_entries rule
 // (:571-583), zero-filled contiguous storage (:600-621), histograms (:623-640),
 // placement by assignment or COO fallback in input order with the cpuRowPtr
 // gap-filling rule (:643-708).
 template <typename T>
 Hb<T>* convert(const int* rowIndices, const int* colIndices, const T* values, int64_t nnz,
-               int numRows, int numCols, int blockWidth, int threshold, int max_coo_entries) {
+               int numRows, int numCols, int blockWidth, int threshold, int max_coo_entries,
+               const int* G = nullptr) {
+  auto grow = [&](int r) { return G ? G[r] : r; };
   auto* h = new Hb<T>();
   h->bw = blockWidth;
   const int maxNumBlocks = (std::min(numRows, numCols) + blockWidth - 1) / blockWidth;
@@ -162,7 +165,7 @@ Hb<T>* convert(const int* rowIndices, const int* colIndices, const T* values, in
   std::vector<std::map<int, int>> cnt(maxNumBlocks);
   for (int64_t i = 0; i < nnz; ++i) {
     int block = rowIndices[i] / blockWidth;
-    int offset = colIndices[i] - rowIndices[i];
+    int offset = colIndices[i] - grow(rowIndices[i]);
     if (block < maxNumBlocks) cnt[block][offset]++;
   }
   std::set<int> uniq;
@@ -207,7 +210,7 @@ Hb<T>* convert(const int* rowIndices, const int* colIndices, const T* values, in
   for (int64_t i = 0; i < nnz; ++i) {
     int r = rowIndices[i], c = colIndices[i];
     T v = values[i];
-    int block = r / blockWidth, offset = c - r, lane = r % blockWidth;
+    int block = r / blockWidth, offset = c - grow(r), lane = r % blockWidth;
     if (block < maxNumBlocks && !h->offs[block].empty()) {
       auto& ob = h->offs[block];
       auto it = std::find(ob.begin(), ob.end(), offset);
@@ -345,6 +348,15 @@ void* orc_convert_f64(const int* r, const int* c, const double* v, int64_t nnz, 
 void* orc_convert_f32(const int* r, const int* c, const float* v, int64_t nnz, int nrows,
                       int ncols, int bw, int thr, int max_coo) {
   return convert<float>(r, c, v, nnz, nrows, ncols, bw, thr, max_coo);
+}
+
+void* orc_convert_g_f64(const int* r, const int* c, const double* v, int64_t nnz, int nrows, int ncols, int bw,
+                        int thr, int max_coo, const int* G) {
+  return convert<double>(r, c, v, nnz, nrows, ncols, bw, thr, max_coo, G);
+}
+void* orc_convert_g_f32(const int* r, const int* c, const float* v, int64_t nnz, int nrows, int ncols, int bw,
+                        int thr, int max_coo, const int* G) {
+  return convert<float>(r, c, v, nnz, nrows, ncols, bw, thr, max_coo, G);
 }
 // sizes[0]=num_blocks [1]=num_values [2]=num_offsets [3]=num_coo [4]=numberDiagonals
 // [5]=hist_len [6]=rowptr_len
