@@ -61,6 +61,7 @@ constexpr int kWave = 64;
 constexpr int kWavesPerWG = 4;
 constexpr int kWG = kWave * kWavesPerWG;
 constexpr int kChunk = 9;
+constexpr int kRow2Chunk = 9;
 template <typename T> __device__ __forceinline__ T fma_t(T a, T b, T c);
 template <> __device__ __forceinline__ double fma_t<double>(double a, double b, double c) {
   return __builtin_fma(a, b, c);
@@ -133,11 +134,49 @@ __device__ __forceinline__ T coo_group_sum_from(const int* __restrict__ cr, cons
   return cs;
 }
 
+// The same sums for a group with many entries (more than one 64-entry
+// chunk): each lane finds its row's entries [lo, hi) by a lower-bound search
+// over the group's row-sorted rows, then sums them in entry order, four loads
+// of each kind in flight at a time.  Same FMA sequence as the chunked loop, so
+// the same result bit for bit, without a wave barrier per chunk.  Entries of
+// rows outside the group match no lane and are skipped.
+template <typename T>
+__device__ __forceinline__ T coo_group_sum_direct(const int* __restrict__ cr, const int* __restrict__ cc,
+                                                  const T* __restrict__ cv, const T* __restrict__ x,
+                                                  int p0, int p1, int64_t row0, int lane, bool* any) {
+  const int target = static_cast<int>(row0) + lane;
+  // lower bound of target in cr[p0, p1): p0 + #entries with row < target
+  int lo = p0;
+  for (int step = 1 << (31 - __builtin_clz(p1 - p0)); step > 0; step >>= 1) {
+    const int probe = lo + step;
+    if (probe <= p1 && cr[probe - 1] < target) lo = probe;
+  }
+  // the next lane's lower bound ends this lane's rows; the last lane's is p1
+  const int nxt = __shfl_down(lo, 1);
+  int hi = lane == kWave - 1 ? p1 : nxt;
+  if (hi < lo) hi = lo;
+  T cs = T(0);
+  int i = lo;
+  for (; i + 4 <= hi; i += 4) {
+    const int c0 = cc[i], c1 = cc[i + 1], c2 = cc[i + 2], c3 = cc[i + 3];
+    const T v0 = cv[i], v1 = cv[i + 1], v2 = cv[i + 2], v3 = cv[i + 3];
+    const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    cs = fma_t<T>(v0, x0, cs);
+    cs = fma_t<T>(v1, x1, cs);
+    cs = fma_t<T>(v2, x2, cs);
+    cs = fma_t<T>(v3, x3, cs);
+  }
+  for (; i < hi; ++i) cs = fma_t<T>(cv[i], x[cc[i]], cs);
+  *any = hi > lo;
+  return cs;
+}
+
 template <typename T>
 __device__ __forceinline__ T coo_group_sum(const int* __restrict__ cr, const int* __restrict__ cc,
                                            const T* __restrict__ cv, const T* __restrict__ x,
                                            int p0, int p1, int64_t row0, int lane, T* s_v,
                                            T* s_x, int* s_lo, int* s_hi, bool* any) {
+  if (p1 - p0 > kWave) return coo_group_sum_direct<T>(cr, cc, cv, x, p0, p1, row0, lane, any);
   return coo_group_sum_from<T>(cr, cc, cv, x, p0, p1, row0, lane, s_v, s_x, s_lo, s_hi, T(0), any);
 }
 
@@ -456,6 +495,7 @@ typedef double dbl2a8 __attribute__((ext_vector_type(2), aligned(8)));
 // one offset list of runs of three.  vb + vlane addresses the lane's two values
 // of the list's first offset; offset o of them is at vb + vlane + o * bw.  The x
 // values, skips and FMA order per row are exactly those of bdia_sum_runs3.
+template <int CH>
 __device__ __forceinline__ void bdia_sum_runs3_row2(const double* __restrict__ vb, int vlane,
                                                     const int* __restrict__ op, int cnt, int bw,
                                                     const double* __restrict__ x, int64_t row0, int64_t row,
@@ -464,52 +504,147 @@ __device__ __forceinline__ void bdia_sum_runs3_row2(const double* __restrict__ v
   const int64_t last = nrows - 1;
   sa = 0.0;
   sb = 0.0;
-  for (int o0 = 0; o0 < cnt; o0 += kChunk) {
-    int off[kChunk];
+  for (int o0 = 0; o0 < cnt; o0 += CH) {
+    int off[CH];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
-    dbl2a8 v[kChunk];
+    for (int k = 0; k < CH; ++k) off[k] = op[o0 + k];  // padded array
+    dbl2a8 v[CH];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k)
+    for (int k = 0; k < CH; ++k)
       v[k] = __builtin_nontemporal_load(
           reinterpret_cast<const dbl2a8*>(vb + static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + vlane)));
-    double xa[kChunk], xb[kChunk];
+    // every run's centre pair (one 16-byte load) and edge values, before any FMA
+    dbl2a8 m[CH / 3];
+    double first[CH / 3], after[CH / 3];
 #pragma unroll
-    for (int w = 0; w < kChunk / 3; ++w) {
+    for (int w = 0; w < CH / 3; ++w) {
       // runs past the block's count read padding: clamped, their products unused
       const int64_t o = off[3 * w + 1];
-      dbl2a8 m;
       if (row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last) {  // wave-uniform: the window lies inside x
-        m = *reinterpret_cast<const dbl2a8*>(x + (row + o));
+        m[w] = *reinterpret_cast<const dbl2a8*>(x + (row + o));
       } else {
         int64_t c0 = row + o, c1 = row + 1 + o;
         c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
         c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
-        m.x = x[c0];
-        m.y = x[c1];
+        m[w].x = x[c0];
+        m[w].y = x[c1];
       }
+    }
+#pragma unroll
+    for (int w = 0; w < CH / 3; ++w) {
+      const int64_t o = off[3 * w + 1];
       int64_t cf = row0 + o - 1, ce = row0 + o + 2 * kWave;
       cf = cf < 0 ? 0 : (cf > last ? last : cf);
       ce = ce < 0 ? 0 : (ce > last ? last : ce);
-      const double first = x[cf], after = x[ce];      // wave-uniform: scalar loads
-      const double left = wave_shr1(m.y, first);       // x[row + o - 1]
-      const double right = wave_shl1(m.x, after);      // x[row + o + 2]
-      xa[3 * w] = left;
-      xa[3 * w + 1] = m.x;
-      xa[3 * w + 2] = m.y;
-      xb[3 * w] = m.x;
-      xb[3 * w + 1] = m.y;
-      xb[3 * w + 2] = right;
+      first[w] = x[cf];  // wave-uniform: scalar loads
+      after[w] = x[ce];
     }
+    // the FMAs run by run, in offset order
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) {
-      const int64_t ca = row + off[k], cb = ca + 1;
-      const bool oka = (o0 + k < cnt) & act_a & (v[k].x != 0.0) & (ca >= 0) & (ca < nrows);
-      const bool okb = (o0 + k < cnt) & act_b & (v[k].y != 0.0) & (cb >= 0) & (cb < nrows);
-      sa = masked_fma(oka, v[k].x, xa[k], sa);
-      sb = masked_fma(okb, v[k].y, xb[k], sb);
+    for (int w = 0; w < CH / 3; ++w) {
+      const double left = wave_shr1(m[w].y, first[w]);   // x[row + o - 1]
+      const double right = wave_shl1(m[w].x, after[w]);  // x[row + o + 2]
+      const double xa[3] = {left, m[w].x, m[w].y};
+      const double xb[3] = {m[w].x, m[w].y, right};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int k = 3 * w + j;
+        const int64_t ca = row + off[k], cb = ca + 1;
+        const bool oka = (o0 + k < cnt) & act_a & (v[k].x != 0.0) & (ca >= 0) & (ca < nrows);
+        const bool okb = (o0 + k < cnt) & act_b & (v[k].y != 0.0) & (cb >= 0) & (cb < nrows);
+        sa = masked_fma(oka, v[k].x, xa[j], sa);
+        sb = masked_fma(okb, v[k].y, xb[j], sb);
+      }
     }
   }
+}
+
+// One nine-offset chunk of the two-row runs-of-three sums, split into its
+// loads and its FMAs so that the next chunk's loads can be issued first.
+struct Row2Chunk {
+  dbl2a8 v[kChunk];
+  dbl2a8 m[kChunk / 3];
+  double first[kChunk / 3], after[kChunk / 3];
+};
+
+__device__ __forceinline__ void row2_chunk_load(Row2Chunk& q, const double* __restrict__ vb, int vlane,
+                                                const int* off, int o0, int cnt, int bw,
+                                                const double* __restrict__ x, int64_t row0, int64_t row,
+                                                int64_t last) {
+#pragma unroll
+  for (int k = 0; k < kChunk; ++k)
+    q.v[k] = __builtin_nontemporal_load(
+        reinterpret_cast<const dbl2a8*>(vb + static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + vlane)));
+#pragma unroll
+  for (int w = 0; w < kChunk / 3; ++w) {
+    // runs past the block's count read padding: clamped, their products unused
+    const int64_t o = off[o0 + 3 * w + 1];
+    if (row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last) {  // wave-uniform: the window lies inside x
+      q.m[w] = *reinterpret_cast<const dbl2a8*>(x + (row + o));
+    } else {
+      int64_t c0 = row + o, c1 = row + 1 + o;
+      c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+      c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+      q.m[w].x = x[c0];
+      q.m[w].y = x[c1];
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < kChunk / 3; ++w) {
+    const int64_t o = off[o0 + 3 * w + 1];
+    int64_t cf = row0 + o - 1, ce = row0 + o + 2 * kWave;
+    cf = cf < 0 ? 0 : (cf > last ? last : cf);
+    ce = ce < 0 ? 0 : (ce > last ? last : ce);
+    q.first[w] = x[cf];  // wave-uniform: scalar loads
+    q.after[w] = x[ce];
+  }
+}
+
+// Row and column indices are below 2^31, so row + offset taken modulo 2^32
+// lies in [0, nrows) exactly when the column does: one unsigned compare.
+__device__ __forceinline__ void row2_chunk_fma(const Row2Chunk& q, const int* off, int o0, int cnt, int64_t row,
+                                               int64_t nrows, bool act_a, bool act_b, double& sa, double& sb) {
+  const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
+#pragma unroll
+  for (int w = 0; w < kChunk / 3; ++w) {
+    const double left = wave_shr1(q.m[w].y, q.first[w]);   // x[row + o - 1]
+    const double right = wave_shl1(q.m[w].x, q.after[w]);  // x[row + o + 2]
+    const double xa[3] = {left, q.m[w].x, q.m[w].y};
+    const double xb[3] = {q.m[w].x, q.m[w].y, right};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int k = 3 * w + j;
+      const uint32_t ca = row32 + static_cast<uint32_t>(off[o0 + k]), cb = ca + 1u;
+      const bool oka = (o0 + k < cnt) & act_a & (q.v[k].x != 0.0) & (ca < n32);
+      const bool okb = (o0 + k < cnt) & act_b & (q.v[k].y != 0.0) & (cb < n32);
+      sa = masked_fma(oka, q.v[k].x, xa[j], sa);
+      sb = masked_fma(okb, q.v[k].y, xb[j], sb);
+    }
+  }
+}
+
+// bdia_sum_runs3_row2 for a block of at most 27 offsets: its three chunks
+// unrolled, each chunk's loads issued before the previous chunk's FMAs, so a
+// wave keeps the next nine value lines and x windows in flight while it
+// computes.  Same arithmetic and order.
+__device__ __forceinline__ void bdia_sum_runs3_row2_27(const double* __restrict__ vb, int vlane,
+                                                       const int* __restrict__ op, int cnt, int bw,
+                                                       const double* __restrict__ x, int64_t row0, int64_t row,
+                                                       int64_t nrows, bool act_a, bool act_b, double& sa,
+                                                       double& sb) {
+  const int64_t last = nrows - 1;
+  int off[3 * kChunk];
+#pragma unroll
+  for (int k = 0; k < 3 * kChunk; ++k) off[k] = op[k];  // padded array
+  sa = 0.0;
+  sb = 0.0;
+  Row2Chunk c0, c1;
+  row2_chunk_load(c0, vb, vlane, off, 0, cnt, bw, x, row0, row, last);
+  if (cnt > kChunk) row2_chunk_load(c1, vb, vlane, off, kChunk, cnt, bw, x, row0, row, last);
+  row2_chunk_fma(c0, off, 0, cnt, row, nrows, act_a, act_b, sa, sb);
+  if (cnt > 2 * kChunk) row2_chunk_load(c0, vb, vlane, off, 2 * kChunk, cnt, bw, x, row0, row, last);
+  if (cnt > kChunk) row2_chunk_fma(c1, off, kChunk, cnt, row, nrows, act_a, act_b, sa, sb);
+  if (cnt > 2 * kChunk) row2_chunk_fma(c0, off, 2 * kChunk, cnt, row, nrows, act_a, act_b, sa, sb);
 }
 
 // The blocked part of one 64-row half (rows r0 + lane) of a block width that is
@@ -582,7 +717,11 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     // lanes of half B read block bB's lines, which follow block bA's
     const int vlane = 2 * lane + ((lane >= kWave / 2 && bB != bA) ? (cA - 1) * bw : 0);
     double sa, sb;
-    bdia_sum_runs3_row2(vb, vlane, offs + sA, cA, bw, x, row0, row, nrows, row < nrows, row + 1 < nrows, sa, sb);
+    if (cA <= 3 * kChunk)
+      bdia_sum_runs3_row2_27(vb, vlane, offs + sA, cA, bw, x, row0, row, nrows, row < nrows, row + 1 < nrows, sa, sb);
+    else
+      bdia_sum_runs3_row2<kRow2Chunk>(vb, vlane, offs + sA, cA, bw, x, row0, row, nrows, row < nrows,
+                                      row + 1 < nrows, sa, sb);
     // back to the halves' row-per-lane order
     double* sy = s_y[wv];
     sy[2 * lane] = sa;
@@ -1581,8 +1720,8 @@ int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   const bool runs3 = (uniform || bw == 32) && runs3_blocks != nullptr;
   const int kshift = xcd_chunk_shift(wgs);
   if constexpr (std::is_same<T, double>::value) {
-    if (runs3 && uniform) {
-      // 128-row waves, two rows per lane
+    if (runs3 && uniform && nrows <= 0x7fffffffLL) {
+      // 128-row waves, two rows per lane (32-bit column bound checks)
       const int64_t wgs2 = (nrows + 2 * kWG - 1) / (2 * kWG);
       const dim3 grid2(static_cast<unsigned>(wgs2));
       const int ks2 = xcd_chunk_shift(wgs2);
