@@ -826,11 +826,16 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
   __shared__ int s_hi[kWavesPerWG][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const unsigned work = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  // one workgroup per (row block, column group): a row block's column groups are
+  // consecutive work items, which the XCD runs keep on one XCD, so the later
+  // groups read the block's values from that XCD's L2 instead of HBM
+  const unsigned passes = static_cast<unsigned>((nrhs + KB - 1) / KB);
+  const unsigned q = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  const unsigned work = q / passes;
   const int64_t grp = static_cast<int64_t>(work) * kWavesPerWG + wv;
   const int64_t row0 = grp * kWave;
   if (row0 >= nrows) return;
-  const int j0 = static_cast<int>(blockIdx.y) * KB;
+  const int j0 = static_cast<int>(q - work * passes) * KB;
   const int kb = min(KB, nrhs - j0);
   const T* Xj = X + static_cast<int64_t>(j0) * ldx;
   T* Yj = Y + static_cast<int64_t>(j0) * ldy;
@@ -865,16 +870,48 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
         v[k] = small ? __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(kk * bw + lane)))
                      : __builtin_nontemporal_load(vb + static_cast<int64_t>(kk) * bw + lane);
       }
+      // every column's x windows for the chunk before any FMA
+      T mid[KB][RR], first[KB][RR], after[KB][RR];
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        if (j < kb) {
+          const T* xj = Xj + static_cast<int64_t>(j) * ldx;
+#pragma unroll
+          for (int w = 0; w < RR; ++w) {
+            int64_t c = row + off[3 * w + 1];
+            c = c < 0 ? 0 : (c > last ? last : c);
+            mid[j][w] = load_x<T, X32>(xj, c);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < KB; ++j) {
+        if (j < kb) {
+          const T* xj = Xj + static_cast<int64_t>(j) * ldx;
+#pragma unroll
+          for (int w = 0; w < RR; ++w) {
+            const int64_t o = off[3 * w + 1];
+            int64_t cf = row0 + o - 1, ce = row0 + o + kWave;
+            cf = cf < 0 ? 0 : (cf > last ? last : cf);
+            ce = ce < 0 ? 0 : (ce > last ? last : ce);
+            first[j][w] = load_x<T, X32>(xj, cf);
+            after[j][w] = load_x<T, X32>(xj, ce);
+          }
+        }
+      }
 #pragma unroll
       for (int j = 0; j < KB; ++j) {
         if (j >= kb) break;
-        T xl[3 * RR];
-        runs3_x<T, X32, RR>(Xj + static_cast<int64_t>(j) * ldx, off, row0, row, last, xl);
 #pragma unroll
-        for (int k = 0; k < 3 * RR; ++k) {
-          const int64_t c = row + off[k];
-          const bool ok = (o0 + k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
-          sum[j] = masked_fma(ok, v[k], xl[k], sum[j]);
+        for (int w = 0; w < RR; ++w) {
+          const T xl[3] = {wave_shr1(mid[j][w], first[j][w]), mid[j][w], wave_shl1(mid[j][w], after[j][w])};
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const int k = 3 * w + i;
+            const int64_t c = row + off[k];
+            const bool ok = (o0 + k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
+            sum[j] = masked_fma(ok, v[k], xl[i], sum[j]);
+          }
         }
       }
     }
@@ -1796,9 +1833,11 @@ int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   // 8 columns per pass: measured faster than 16 (whose one-offset chunks lose
   // memory-level parallelism); more columns take more passes (grid.y).
   const int kb = nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1));
-  const dim3 grid(static_cast<unsigned>(wgs), static_cast<unsigned>((nrhs + kb - 1) / kb));
+  const int64_t items = wgs * ((nrhs + kb - 1) / kb);
+  if (items > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows x columns");
+  const dim3 grid(static_cast<unsigned>(items));
   hipStream_t s = hs(stream);
-  const int ks = xcd_chunk_shift(wgs);
+  const int ks = xcd_chunk_shift(items);
   switch (kb) {
     case 8: launch_multi_kb<T, 8>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks, grid, s); break;
     case 4: launch_multi_kb<T, 4>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks, grid, s); break;
