@@ -141,6 +141,7 @@ _CAPI_SIGS = {
     "hbdia_matrix_get_vector_offsets": (_i, [_p, _p]),
     "hbdia_vector_create_halo": (_i, [POINTER(c_void_p), c_int, _p, c_int64, _p]),
     "hbdia_vector_fill_halo": (_i, [_p, _p, _p, c_int64]),
+    "hbdia_vector_verify_halo": (_i, [_p, _p, POINTER(c_int)]),
     "hbdia_spmv_multi": (_i, [_p, _p, c_int64, _p, c_int64, c_int, c_int]),
     "hbdia_mixed_create": (_i, [POINTER(c_void_p), _p, c_int, c_double]),
     "hbdia_mixed_destroy": (_i, [_p]),

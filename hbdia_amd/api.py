@@ -374,6 +374,12 @@ class HBDIAVector:
         g = np.ascontiguousarray(globalVector, dtype=self.dtype)
         check_capi(capi.hbdia_vector_fill_halo(self._h, matrix._h, g.ctypes.data, g.size))
 
+    def verifyHalo(self, matrix: HBDIA) -> bool:
+        """The reference's verifyDataExchange for a vector filled from x[g] = g."""
+        ok = ctypes.c_int(0)
+        check_capi(capi.hbdia_vector_verify_halo(self._h, matrix._h, ctypes.byref(ok)))
+        return bool(ok.value)
+
     def getLocalSize(self) -> int:
         return self.size
 

@@ -227,9 +227,46 @@ bool exchangeHaloFromGlobal(const HBDIA<T>& matrix, HBDIAVector<T>& vector, cons
   return true;
 }
 
+template <typename T>
+bool verifyHaloExchange(const HBDIA<T>& matrix, const HBDIAVector<T>& vector) {
+  if (!matrix.isPartialMatrix() || !matrix.hasPartialMatrixMetadata()) {
+    std::cerr << "Matrix must be a partial matrix with metadata for verification" << std::endl;
+    return false;
+  }
+  const auto& pdr = matrix.getPartialMatrixMetadata().processDataRanges;
+  const auto& grm = matrix.getGlobalRowMapping();
+  const std::vector<T> all = vector.downloadAll();
+  int64_t at = 0, errors = 0;
+  auto check = [&](int64_t pos, int64_t global) {
+    if (pos >= static_cast<int64_t>(all.size()) || all[pos] != static_cast<T>(global)) {
+      if (errors < 8)
+        std::cerr << "Verification ERROR: rank " << matrix.getRank() << " expected " << global << " at buffer position "
+                  << pos << std::endl;
+      ++errors;
+    }
+  };
+  for (int p = 0; p < matrix.getSize() && p < static_cast<int>(pdr.size()); ++p) {
+    if (p == matrix.getRank()) {
+      for (size_t i = 0; i < grm.size(); ++i) check(at + static_cast<int64_t>(i), grm[i]);
+      at += static_cast<int64_t>(grm.size());
+      continue;
+    }
+    for (const auto& rg : pdr[p])
+      for (int g = std::get<0>(rg); g < std::get<1>(rg); ++g) check(at++, g);
+  }
+  if (at != static_cast<int64_t>(all.size())) ++errors;
+  if (errors == 0) std::cout << "Data exchange verification PASSED for rank " << matrix.getRank() << std::endl;
+  else
+    std::cerr << "Data exchange verification FAILED for rank " << matrix.getRank() << " with " << errors << " errors"
+              << std::endl;
+  return errors == 0;
+}
+
 template class BasicExtractor<double>;
 template class BasicExtractor<float>;
 template HBDIA<double> createRankLocalMatrix<double>(const HBDIA<double>&, int, int, int, int, int, ExecutionMode);
 template HBDIA<float> createRankLocalMatrix<float>(const HBDIA<float>&, int, int, int, int, int, ExecutionMode);
 template bool exchangeHaloFromGlobal<double>(const HBDIA<double>&, HBDIAVector<double>&, const std::vector<double>&);
 template bool exchangeHaloFromGlobal<float>(const HBDIA<float>&, HBDIAVector<float>&, const std::vector<float>&);
+template bool verifyHaloExchange<double>(const HBDIA<double>&, const HBDIAVector<double>&);
+template bool verifyHaloExchange<float>(const HBDIA<float>&, const HBDIAVector<float>&);

@@ -185,6 +185,13 @@ void HBDIAVector<T>::download(std::vector<T>& out) const {
 }
 
 template <typename T>
+std::vector<T> HBDIAVector<T>::downloadAll() const {
+  std::vector<T> out(getTotalSize());
+  if (!out.empty()) CHECK_HBDIA(hbdia_memcpy(out.data(), data_ptr_d_, out.size() * sizeof(T)));
+  return out;
+}
+
+template <typename T>
 std::vector<T> HBDIAVector<T>::download() const {
   std::vector<T> out;
   download(out);

@@ -547,6 +547,17 @@ int hbdia_vector_fill_halo(hbdia_vector_t v, hbdia_matrix_t m, const void* globa
   });
 }
 
+int hbdia_vector_verify_halo(hbdia_vector_t v, hbdia_matrix_t m, int* ok) {
+  return guard("hbdia_vector_verify_halo", [&] {
+    need(v != nullptr && m != nullptr && ok != nullptr, "bad argument");
+    need(v->dtype == m->dtype, "dtype mismatch between matrix and vector");
+    *ok = (m->dtype == HBDIA_DTYPE_F64 ? verifyHaloExchange<double>(*m->d, *v->d)
+                                       : verifyHaloExchange<float>(*m->f, *v->f))
+              ? 1
+              : 0;
+  });
+}
+
 int hbdia_spmv_multi(hbdia_matrix_t m, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrhs, int exec_coo_gpu) {
   return guard("hbdia_spmv_multi", [&] {
     need(m != nullptr && X != nullptr && Y != nullptr, "null argument");

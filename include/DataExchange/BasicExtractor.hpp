@@ -64,4 +64,13 @@ HBDIA<T> createRankLocalMatrix(const HBDIA<T>& globalMatrix, int rank, int size,
 template <typename T>
 bool exchangeHaloFromGlobal(const HBDIA<T>& matrix, HBDIAVector<T>& vector, const std::vector<T>& globalVector);
 
+// The check of the reference's MPICommunicator<T>::verifyDataExchange
+// (src/DataExchange/MPICommunicator.cpp:415-498) for a vector whose global
+// values were x[g] = g: every halo entry, walked through the request lists of
+// the other ranks in order, must hold its global index, and every local entry
+// its global row.  Reads the device buffer back; prints one PASSED / FAILED
+// line like the reference and returns whether all entries matched.
+template <typename T>
+bool verifyHaloExchange(const HBDIA<T>& matrix, const HBDIAVector<T>& vector);
+
 #endif  // BASIC_EXTRACTOR_HPP

@@ -72,6 +72,8 @@ class HBDIAVector {
   void uploadHalo(const std::vector<T>& left, const std::vector<T>& right);
   void download(std::vector<T>& out) const;
   std::vector<T> download() const;
+  // The whole device buffer, [left halo | local | right halo].
+  std::vector<T> downloadAll() const;
 
   void print(const std::string& vectorName = "HBDIAVector") const;
 

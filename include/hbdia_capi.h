@@ -25,6 +25,7 @@
  *   hbdia_matrix_get_vector_offsets    -> HBDIA<T>::calculateVectorOffsets, src/Format/HBDIA.cpp:1034-1178
  *   hbdia_vector_create_halo           -> HBDIAVector<T>(localData, matrix, rank, size), HBDIAVector.hpp:33
  *   hbdia_vector_fill_halo             -> stands in for MPICommunicator<T>::exchangeData on one GPU
+ *   hbdia_vector_verify_halo           -> MPICommunicator<T>::verifyDataExchange, MPICommunicator.cpp:415-498
  *   hbdia_matrix_build_local_coo, _local_info, _get_local
  *                                      -> no reference counterpart: the rank-local COO part
  *   hbdia_spmv_multi                   -> no reference counterpart: many right-hand sides in one launch
@@ -190,6 +191,10 @@ int hbdia_vector_create_halo(hbdia_vector_t* v, int dtype, const void* data, int
 /* Fills the halo segments from the global vector (n entries) on the host and copies them to
  * the device. */
 int hbdia_vector_fill_halo(hbdia_vector_t v, hbdia_matrix_t m, const void* global, int64_t n);
+/* verifyHaloExchange: *ok = 1 when the vector, filled from x[g] = g, holds every halo entry's
+ * global index and every local entry's global row (MPICommunicator::verifyDataExchange,
+ * src/DataExchange/MPICommunicator.cpp:415-498). */
+int hbdia_vector_verify_halo(hbdia_vector_t v, hbdia_matrix_t m, int* ok);
 /* hbdiaSpMVMulti: X, Y device arrays (column-major, leading dimensions ldx, ldy). */
 int hbdia_spmv_multi(hbdia_matrix_t m, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrhs, int exec_coo_gpu);
 

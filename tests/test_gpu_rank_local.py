@@ -104,3 +104,21 @@ def test_rank_local_rejects_vector_without_halo(gpu):
     Y = HBDIAVector(np.zeros(n))
     with pytest.raises((ValueError, HBDIAError)):
         hbdiaSpMV(A, X, Y, False, True)
+
+
+@pytest.mark.parametrize("size,rank", [(4, 0), (4, 2), (4, 3), (3, 1)])
+def test_halo_exchange_verifies(gpu, size, rank):
+    """The reference's self-check of its exchange (x[g] = g, every received entry its global index)
+    on the vector filled from the global x."""
+    print(f"case: halo verify 16^3 noise 0.02 size {size} rank {rank}", flush=True)
+    r, c, v, N = oracle.stencil27(16, 16, 16, 0.02, 1)
+    G = HBDIA(r, c, v, N, N)
+    A = HBDIA.createRankLocalMatrix(G, rank, size, 32, 2)
+    A.prepareForGPU()
+    lay = A.getLocalLayout()
+    x = np.arange(N, dtype=np.float64)
+    r0, n = lay["first_row"], A.getNumRows()
+    X = HBDIAVector.withHalo(x[r0:r0 + n], A)
+    assert not X.verifyHalo(A) or lay["halo_left"] + lay["halo_right"] == 0  # halo still zero
+    X.fillHalo(A, x)
+    assert X.verifyHalo(A)
