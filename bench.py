@@ -586,8 +586,9 @@ def bench_convert(args) -> int:
     """COO -> HBDIA conversion of the 256^3 stencil + scatter matrix (453M entries, block width 64,
     threshold 2): the device conversion with the COO already in HBM (hbdia_convert_coo, every result
     array left in HBM), the whole convertToHBDIAFormatOnDevice (COO upload, device conversion, host
-    copies of the result), and the host conversion + upload (convertToHBDIAFormat); the two methods
-    must leave the same host arrays.  value = entries per second of the device conversion."""
+    copies of the result; and with hostValues=False, the blocked values left on the device only),
+    and the host conversion + upload (convertToHBDIAFormat); the two methods must leave the same
+    host arrays.  value = entries per second of the device conversion."""
     import ctypes
 
     import hbdia_amd
@@ -632,6 +633,13 @@ def bench_convert(args) -> int:
         method_s.append(time.perf_counter() - t)
         log(f"convertToHBDIAFormatOnDevice {method_s[-1]:.2f} s")
     dev_state = A.getHBDIA()
+    donly_s = []
+    for k in range(reps):
+        A.deleteHBDIAFormat()
+        t = time.perf_counter()
+        A.convertToHBDIAFormatOnDevice(bw, thr, hostValues=False)
+        donly_s.append(time.perf_counter() - t)
+        log(f"convertToHBDIAFormatOnDevice(hostValues=False) {donly_s[-1]:.2f} s")
     for k in range(reps):
         A.deleteHBDIAFormat()
         t = time.perf_counter()
@@ -666,7 +674,8 @@ def bench_convert(args) -> int:
             "config": {"workload": f"convert_stencil27_scatter_{n}^3", "entries": nnz, "block_width": bw,
                        "threshold": thr, "pairs": int(pairs)},
             "ms_min": round(min(dev_s) * 1e3, 3), "ms_max": round(max(dev_s) * 1e3, 3),
-            "method_s": round(min(method_s), 3), "host_convert_upload_s": round(min(host_s), 3),
+            "method_s": round(min(method_s), 3), "method_device_only_s": round(min(donly_s), 3),
+            "host_convert_upload_s": round(min(host_s), 3),
             "host_threads": hbdia_amd.api.capi.hbdia_get_num_threads(),
             "host_over_method": round(min(host_s) / min(method_s), 2), "identical": same, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)

@@ -117,6 +117,7 @@ _CAPI_SIGS = {
     "hbdia_matrix_convert": (_i, [_p, c_int, c_int, c_int, c_int, c_int]),
     "hbdia_matrix_convert_host": (_i, [_p, c_int, c_int, c_int, c_int, c_int]),
     "hbdia_matrix_convert_device": (_i, [_p, c_int, c_int, c_int, c_int, c_int]),
+    "hbdia_matrix_convert_device_ex": (_i, [_p, c_int, c_int, c_int, c_int, c_int, c_int]),
     "hbdia_matrix_remove_duplicates": (_i, [_p]),
     "hbdia_matrix_delete_hbdia": (_i, [_p]),
     "hbdia_matrix_prepare_gpu": (_i, [_p]),

@@ -30,7 +30,7 @@ RAND_SEED = 42
 F64, F32 = 0, 1
 _INFO = ["rows", "cols", "nnz", "has_coo", "has_hbdia", "block_width", "threshold", "num_blocks", "num_values",
          "num_offsets", "num_coo", "num_diagonals", "hist_len", "on_device", "device_coo", "stored_bytes",
-         "runs3_blocks"]
+         "runs3_blocks", "host_values"]
 
 
 _REPORT = ["rows", "cols", "nnz", "number_diagonals", "num_blocks", "block_width", "threshold", "max_coo_entries",
@@ -159,13 +159,14 @@ class HBDIA:
     def convertToHBDIAFormatOnDevice(self, blockWidth: int = DEFAULT_BLOCK_WIDTH,
                                      threshold: int = DEFAULT_THRESHOLD, max_coo_entries: int = MAX_COO_ENTRIES,
                                      COOisUnique: bool = False,
-                                     execMode: ExecutionMode = ExecutionMode.GPU_COO) -> None:
+                                     execMode: ExecutionMode = ExecutionMode.GPU_COO, hostValues: bool = True) -> None:
         """The conversion computed on the GPU: the host state of convertToHBDIAFormatOnHost, bit for bit,
-        and the device arrays of prepareForGPU built in place."""
+        and the device arrays of prepareForGPU built in place.  hostValues=False leaves the blocked values
+        on the device until a host access reads them back."""
         try:
-            check_capi(capi.hbdia_matrix_convert_device(self._h, int(blockWidth), int(threshold),
-                                                        int(max_coo_entries), int(bool(COOisUnique)),
-                                                        int(execMode == ExecutionMode.GPU_COO)))
+            check_capi(capi.hbdia_matrix_convert_device_ex(self._h, int(blockWidth), int(threshold),
+                                                           int(max_coo_entries), int(bool(COOisUnique)),
+                                                           int(execMode == ExecutionMode.GPU_COO), int(bool(hostValues))))
         except HBDIAError as e:
             raise _map_error(e, "convert") from None
 

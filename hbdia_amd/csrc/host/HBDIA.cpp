@@ -201,6 +201,8 @@ void HBDIA<T>::moveFrom(HBDIA& o) noexcept {
   colIndices = std::move(o.colIndices);
   hbdiaData = std::move(o.hbdiaData);
   ptrToBlock = std::move(o.ptrToBlock);
+  hostValuesPending_ = std::exchange(o.hostValuesPending_, false);
+  pendingValues_ = std::exchange(o.pendingValues_, 0);
   offsetsPerBlock = std::move(o.offsetsPerBlock);
   cpuRowIndices = std::move(o.cpuRowIndices);
   cpuColIndices = std::move(o.cpuColIndices);
@@ -655,7 +657,7 @@ bool HBDIA<T>::convertToHBDIAFormatOnHost(int bw, int thr, int maxCoo, bool /*CO
 
 template <typename T>
 bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*COOisUnique*/,
-                                            ExecutionMode execMode) {
+                                            ExecutionMode execMode, bool hostValues) {
   static_assert(sizeof(T) == 4 || sizeof(T) == 8, "values are moved as 4- or 8-byte words");
   if (hasHBDIA) {
     std::cout << "Matrix is already in HBDIA format" << std::endl;
@@ -722,7 +724,9 @@ bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*
   dev_download(placed, r.block_placed, static_cast<size_t>(nb));
   std::vector<T> data, cooV;
   std::vector<int> cooR, cooC;
-  dev_download(data, r.values, static_cast<size_t>(r.num_pairs) * static_cast<size_t>(bw));
+  const size_t nvals = static_cast<size_t>(r.num_pairs) * static_cast<size_t>(bw);
+  const bool deferValues = !hostValues && !partialMatrix;
+  if (!deferValues) dev_download(data, r.values, nvals);
   dev_download(cooR, r.coo_rows, static_cast<size_t>(r.num_coo));
   dev_download(cooC, r.coo_cols, static_cast<size_t>(r.num_coo));
   dev_download(cooV, r.coo_values, static_cast<size_t>(r.num_coo));
@@ -734,12 +738,15 @@ bool HBDIA<T>::convertToHBDIAFormatOnDevice(int bw, int thr, int maxCoo, bool /*
   executionMode = execMode;
   numberDiagonals = r.number_diagonals;
   hbdiaData = std::move(data);
+  hostValuesPending_ = deferValues;
+  pendingValues_ = deferValues ? nvals : 0;
   offsetsPerBlock.assign(nb, {});
   ptrToBlock.assign(nb, nullptr);
   int maxDiag = 0;
   for (int b = 0; b < nb; ++b) {
     offsetsPerBlock[b].assign(flat.begin() + starts[b], flat.begin() + starts[b + 1]);
-    if (starts[b + 1] > starts[b]) ptrToBlock[b] = hbdiaData.data() + static_cast<size_t>(starts[b]) * bw;
+    if (!deferValues && starts[b + 1] > starts[b])
+      ptrToBlock[b] = hbdiaData.data() + static_cast<size_t>(starts[b]) * bw;
     maxDiag = std::max(maxDiag, starts[b + 1] - starts[b]);
   }
   histogramBlocks.assign(maxDiag + 1, 0);
@@ -1004,6 +1011,8 @@ void HBDIA<T>::deleteHBDIAFormat() {
   histogramNnz.clear();
   numberDiagonals = 0;
   hasHBDIA = false;
+  hostValuesPending_ = false;
+  pendingValues_ = 0;
   dataRanges.clear();
   vectorOffsets.clear();
   localCOOBuilt_ = false;
@@ -1027,8 +1036,26 @@ void HBDIA<T>::deleteMatrix() {
 // device upload (reference src/Format/HBDIA.cpp:720-816) and streams (:858-876)
 
 template <typename T>
+void HBDIA<T>::materializeHostValues() const {
+  if (!hostValuesPending_) return;
+  if (!hbdiaData_d_ && pendingValues_ > 0)
+    throw std::logic_error("HBDIA: blocked values are neither on the host nor on the device");
+  std::vector<T> data;
+  dev_download(data, hbdiaData_d_, pendingValues_);
+  hbdiaData = std::move(data);
+  ptrToBlock.assign(offsetsPerBlock.size(), nullptr);
+  size_t pos = 0;
+  for (size_t b = 0; b < offsetsPerBlock.size(); ++b) {
+    if (!offsetsPerBlock[b].empty()) ptrToBlock[b] = hbdiaData.data() + pos;
+    pos += offsetsPerBlock[b].size() * static_cast<size_t>(blockWidth);
+  }
+  hostValuesPending_ = false;
+}
+
+template <typename T>
 void HBDIA<T>::prepareForGPU() {
   if (!hasHBDIA) return;
+  materializeHostValues();  // the upload below starts from the host values
   cleanupGPUData();
   // a partial matrix runs rank-local: halo columns go through the COO part
   if (partialMatrix) buildLocalCOO();
@@ -1146,6 +1173,7 @@ void HBDIA<T>::releaseDevice() noexcept {
 
 template <typename T>
 void HBDIA<T>::cleanupGPUData() {
+  materializeHostValues();  // device-only values must survive the release
   releaseDevice();
 }
 
@@ -1196,7 +1224,7 @@ template <typename T>
 size_t HBDIA<T>::storedBytes() const {
   size_t nOff = 0;
   for (const auto& o : offsetsPerBlock) nOff += o.size();
-  return hbdiaData.size() * sizeof(T) + nOff * sizeof(int) + (offsetsPerBlock.size() + 1) * sizeof(int) +
+  return getNumStoredValues() * sizeof(T) + nOff * sizeof(int) + (offsetsPerBlock.size() + 1) * sizeof(int) +
          cpuValues.size() * (sizeof(T) + 2 * sizeof(int));
 }
 
@@ -1209,7 +1237,7 @@ void HBDIA<T>::print() const {
             << numNonZeros << (hasCOO ? ", COO" : "") << (hasHBDIA ? ", HBDIA" : "");
   if (hasHBDIA)
     std::cout << " (bw " << blockWidth << ", thr " << threshold << ", " << offsetsPerBlock.size()
-              << " blocks, " << hbdiaData.size() << " stored values, " << cpuValues.size()
+              << " blocks, " << getNumStoredValues() << " stored values, " << cpuValues.size()
               << " fallback entries)";
   std::cout << std::endl;
 }
@@ -1229,6 +1257,7 @@ void HBDIA<T>::printHBDIA() const {
     std::cout << "Matrix is not in HBDIA format" << std::endl;
     return;
   }
+  materializeHostValues();
   const size_t nbShow = std::min<size_t>(offsetsPerBlock.size(), 64);
   for (size_t b = 0; b < nbShow; ++b) {
     std::cout << "block " << b << " offsets:";

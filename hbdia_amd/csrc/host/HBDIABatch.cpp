@@ -82,7 +82,7 @@ HBDIABatch<T>::HBDIABatch(const std::vector<const HBDIA<T>*>& ms) {
     d.num_rows = n;
     d.num_blocks = A->getNumBlocks();
     d.block_width = bw;
-    nvals += static_cast<int64_t>(A->getHBDIAData().size());
+    nvals += static_cast<int64_t>(A->getNumStoredValues());
     noffs += pairs;
     nblk += d.num_blocks + 1;
     ncoo += d.coo_nnz;

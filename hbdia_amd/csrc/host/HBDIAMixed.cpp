@@ -22,7 +22,7 @@ HBDIAMixed::HBDIAMixed(const HBDIA<double>& matrix, StoragePrecision precision, 
   if (hasCOO && !matrix.hasDeviceCOO())
     throw std::runtime_error("HBDIAMixed: the fallback is not on the device (convert with ExecutionMode::GPU_COO)");
   const size_t es = elem_bytes(precision);
-  const size_t nv = matrix.getHBDIAData().size(), nc = matrix.getCpuValues().size();
+  const size_t nv = matrix.getNumStoredValues(), nc = matrix.getCpuValues().size();
   hbdia_stream_t s = matrix.getBDIAStream();
   try {
     if (nv) {

@@ -166,6 +166,7 @@ void HBDIA<T>::buildLocalCOO() {
       for (int c = std::get<0>(rg); c < std::get<1>(rg); ++c) halo.push_back(c);
   }
   const int64_t right = static_cast<int64_t>(halo.size()) - left;
+  materializeHostValues();
   if (left + numRows + right > static_cast<int64_t>(INT32_MAX))
     throw std::runtime_error("buildLocalCOO: the vector with its halo exceeds 2^31-1 entries");
   for (size_t i = 0; i < halo.size(); ++i) {

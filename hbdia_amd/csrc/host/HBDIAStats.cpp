@@ -20,7 +20,7 @@ HBDIAStorageReport hbdiaStorageReport(const HBDIA<T>& m) {
   r.threshold = m.getThreshold();
   r.maxCooEntries = m.getMaxCooEntries();
   r.cooFallbackEntries = static_cast<int64_t>(m.getCpuValues().size());
-  r.hbdiaDataSize = static_cast<int64_t>(m.getHBDIAData().size());
+  r.hbdiaDataSize = static_cast<int64_t>(m.getNumStoredValues());
 
   const auto& hb = m.getHistogramBlocks();
   const auto& hn = m.getHistogramNnz();

@@ -92,7 +92,7 @@ void info_of(const HBDIA<T>& m, int64_t* info) {
   info[HBDIA_INFO_BLOCK_WIDTH] = m.getBlockWidth();
   info[HBDIA_INFO_THRESHOLD] = m.getThreshold();
   info[HBDIA_INFO_NUM_BLOCKS] = m.getNumBlocks();
-  info[HBDIA_INFO_NUM_VALUES] = static_cast<int64_t>(m.getHBDIAData().size());
+  info[HBDIA_INFO_NUM_VALUES] = static_cast<int64_t>(m.getNumStoredValues());
   int64_t noff = 0;
   for (const auto& o : m.getOffsetsPerBlock()) noff += static_cast<int64_t>(o.size());
   info[HBDIA_INFO_NUM_OFFSETS] = noff;
@@ -103,6 +103,7 @@ void info_of(const HBDIA<T>& m, int64_t* info) {
   info[HBDIA_INFO_DEVICE_COO] = m.hasDeviceCOO();
   info[HBDIA_INFO_STORED_BYTES] = static_cast<int64_t>(m.storedBytes());
   info[HBDIA_INFO_RUNS3_BLOCKS] = m.getRuns3BlockCount();
+  info[HBDIA_INFO_HOST_VALUES] = m.hasHostValues() ? 1 : 0;
 }
 
 template <typename T>
@@ -274,10 +275,15 @@ int hbdia_matrix_convert_host(hbdia_matrix_t m, int bw, int thr, int max_coo, in
 }
 
 int hbdia_matrix_convert_device(hbdia_matrix_t m, int bw, int thr, int max_coo, int coo_unique, int gpu_coo) {
+  return hbdia_matrix_convert_device_ex(m, bw, thr, max_coo, coo_unique, gpu_coo, 1);
+}
+
+int hbdia_matrix_convert_device_ex(hbdia_matrix_t m, int bw, int thr, int max_coo, int coo_unique, int gpu_coo,
+                                   int host_values) {
   return guard("hbdia_matrix_convert_device", [&] {
     const ExecutionMode em = gpu_coo ? ExecutionMode::GPU_COO : ExecutionMode::CPU_COO;
-    DISPATCH(m, M.convertToHBDIAFormatOnDevice(bw, thr, max_coo, coo_unique != 0, em),
-             M.convertToHBDIAFormatOnDevice(bw, thr, max_coo, coo_unique != 0, em));
+    DISPATCH(m, M.convertToHBDIAFormatOnDevice(bw, thr, max_coo, coo_unique != 0, em, host_values != 0),
+             M.convertToHBDIAFormatOnDevice(bw, thr, max_coo, coo_unique != 0, em, host_values != 0));
   });
 }
 

@@ -85,9 +85,12 @@ class HBDIA {
   // partial -- the device arrays of prepareForGPU built in place on the GPU
   // rather than uploaded.  The COO is uploaded once.  Same return value and
   // messages as the host conversion; throws hbdia::DeviceError on device failures.
+  // hostValues = false leaves the blocked values on the device only: the first
+  // getHBDIAData() / getPtrToBlock() (or prepareForGPU, cleanupGPUData) reads
+  // them back.  Not for partial matrices, whose host state is always built.
   bool convertToHBDIAFormatOnDevice(int blockWidth = DEFAULT_BLOCK_WIDTH, int threshold = DEFAULT_THRESHOLD,
                                     int max_coo_entries = MAX_COO_ENTRIES, bool COOisUnique = false,
-                                    ExecutionMode execMode = ExecutionMode::GPU_COO);
+                                    ExecutionMode execMode = ExecutionMode::GPU_COO, bool hostValues = true);
 
   // print wrappers (short textual dumps)
   void print() const;
@@ -126,8 +129,20 @@ class HBDIA {
   const std::vector<int>& getNnzPerRow() const { return nnzPerRow; }
 
   // HBDIA data
-  const std::vector<T>& getHBDIAData() const { return hbdiaData; }
-  const std::vector<T*>& getPtrToBlock() const { return ptrToBlock; }
+  // The blocked values on the host.  After a device-only conversion the first
+  // call reads them back from the device (not safe against concurrent callers).
+  const std::vector<T>& getHBDIAData() const {
+    materializeHostValues();
+    return hbdiaData;
+  }
+  const std::vector<T*>& getPtrToBlock() const {
+    materializeHostValues();
+    return ptrToBlock;
+  }
+  // Stored blocked-DIA values, padding included, without reading them back.
+  size_t getNumStoredValues() const { return hostValuesPending_ ? pendingValues_ : hbdiaData.size(); }
+  // False between a device-only conversion and the first host access.
+  bool hasHostValues() const { return !hostValuesPending_; }
   const std::vector<std::vector<int>>& getOffsetsPerBlock() const { return offsetsPerBlock; }
   const std::vector<int>& getCpuRowIndices() const { return cpuRowIndices; }
   const std::vector<int>& getCpuColIndices() const { return cpuColIndices; }
@@ -262,15 +277,18 @@ class HBDIA {
  private:
   void releaseDevice() noexcept;
   void moveFrom(HBDIA& o) noexcept;
+  void materializeHostValues() const;
 
   // COO
   std::vector<T> values;
   std::vector<int> rowIndices;
   std::vector<int> colIndices;
 
-  // HBDIA
-  std::vector<T> hbdiaData;
-  std::vector<T*> ptrToBlock;
+  // HBDIA (the values may be read back lazily: see materializeHostValues)
+  mutable std::vector<T> hbdiaData;
+  mutable std::vector<T*> ptrToBlock;
+  mutable bool hostValuesPending_ = false;
+  size_t pendingValues_ = 0;
   std::vector<std::vector<int>> offsetsPerBlock;
   std::vector<int> cpuRowIndices;
   std::vector<int> cpuColIndices;

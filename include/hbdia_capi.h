@@ -65,7 +65,8 @@ enum {
   HBDIA_INFO_DEVICE_COO = 14,
   HBDIA_INFO_STORED_BYTES = 15,
   HBDIA_INFO_RUNS3_BLOCKS = 16, /* blocks given the runs-of-three hint (0 before prepareForGPU) */
-  HBDIA_INFO_COUNT = 17
+  HBDIA_INFO_HOST_VALUES = 17,  /* 0 between a device-only conversion and the first host access */
+  HBDIA_INFO_COUNT = 18
 };
 
 typedef struct hbdia_matrix_opaque* hbdia_matrix_t;
@@ -93,6 +94,11 @@ int hbdia_matrix_convert_host(hbdia_matrix_t m, int block_width, int threshold, 
  * device arrays of hbdia_matrix_prepare_gpu. */
 int hbdia_matrix_convert_device(hbdia_matrix_t m, int block_width, int threshold, int max_coo_entries,
                                 int coo_is_unique, int gpu_coo);
+/* The same with host_values = 0: the blocked values stay on the device until a host access
+ * (hbdia_matrix_get_hbdia, prepare_gpu) reads them back -- the admission path of a matrix that
+ * only runs on the GPU.  host_values = 1 is hbdia_matrix_convert_device. */
+int hbdia_matrix_convert_device_ex(hbdia_matrix_t m, int block_width, int threshold, int max_coo_entries,
+                                   int coo_is_unique, int gpu_coo, int host_values);
 /* deleteHBDIAFormat: drops the blocked format (host and device); the COO stays. */
 int hbdia_matrix_delete_hbdia(hbdia_matrix_t m);
 int hbdia_matrix_remove_duplicates(hbdia_matrix_t m);

@@ -230,3 +230,66 @@ def test_c_abi_runs3_bits_and_errors(gpu):
     from hbdia_amd import HBDIAError
     with pytest.raises(HBDIAError, match="block_width"):
         K.convert_coo(DeviceArray.from_host(r), DeviceArray.from_host(c), DeviceArray.from_host(v), 8000, 8000, 0, 1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_device_only_conversion_defers_host_values(gpu, dtype):
+    """hostValues=False: the device arrays are those of the full device conversion, the SpMV is bit-identical,
+    size-only queries (info, storage report) leave the values on the device, and the first host access reads
+    back exactly the host conversion's arrays."""
+    print(f"case: device-only conversion 20^3 noise 0.05 {np.dtype(dtype).name}", flush=True)
+    r, c, v, N = oracle.stencil27(20, 20, 20, 0.05, 3, dtype=dtype)
+    A = HBDIA(r, c, v, N, N)
+    A.convertToHBDIAFormatOnDevice(64, 2)
+    B = HBDIA(r, c, v, N, N)
+    B.convertToHBDIAFormatOnDevice(64, 2, hostValues=False)
+    assert B.info()["host_values"] == 0 and A.info()["host_values"] == 1
+    da, db = device_state(A), device_state(B)
+    for k in da:
+        np.testing.assert_array_equal(db[k], da[k], err_msg="device " + k)
+    x = np.random.default_rng(5).uniform(-1, 1, N).astype(dtype)
+    ya, yb = HBDIAVector(np.zeros(N, dtype)), HBDIAVector(np.zeros(N, dtype))
+    X = HBDIAVector(x)
+    hbdiaSpMV(A, X, ya, False, True)
+    hbdiaSpMV(B, X, yb, False, True)
+    A.synchronize()
+    B.synchronize()
+    assert yb.download().tobytes() == ya.download().tobytes()
+    ra, rb = A.storage_report(), B.storage_report()
+    assert ra["hbdia_data_size"] == rb["hbdia_data_size"] and ra["hbdia_bytes"] == rb["hbdia_bytes"]
+    assert B.info()["num_values"] == A.info()["num_values"]
+    assert B.info()["host_values"] == 0
+    ha, hb = A.getHBDIA(), B.getHBDIA()  # the first host access reads the values back
+    assert B.info()["host_values"] == 1
+    for k in HOST_KEYS:
+        np.testing.assert_array_equal(hb[k], ha[k], err_msg=k)
+    assert hb["values"].tobytes() == ha["values"].tobytes()
+
+
+def test_device_only_conversion_then_prepare_and_mixed(gpu):
+    print("case: device-only conversion, prepareForGPU and FP32 storage", flush=True)
+    from hbdia_amd import HBDIAMixed
+    r, c, v, N = oracle.stencil27(16, 16, 16, 0.02, 1)
+    A = HBDIA(r, c, v, N, N)
+    A.convertToHBDIAFormatOnDevice(64, 2)
+    B = HBDIA(r, c, v, N, N)
+    B.convertToHBDIAFormatOnDevice(64, 2, hostValues=False)
+    x = np.random.default_rng(6).uniform(-1, 1, N)
+    X = HBDIAVector(x)
+    ya, yb = HBDIAVector(np.zeros(N)), HBDIAVector(np.zeros(N))
+    ma, mb = HBDIAMixed(A, "fp32"), HBDIAMixed(B, "fp32")
+    assert B.info()["host_values"] == 0  # the reduced-precision copy is made on the device
+    ma.spmv(X, ya)
+    mb.spmv(X, yb)
+    A.synchronize()
+    B.synchronize()
+    assert yb.download().tobytes() == ya.download().tobytes()
+    ma.close()
+    mb.close()
+    B.prepareForGPU()  # re-uploads from the host values it reads back first
+    assert B.info()["host_values"] == 1
+    hbdiaSpMV(A, X, ya, False, True)
+    hbdiaSpMV(B, X, yb, False, True)
+    A.synchronize()
+    B.synchronize()
+    assert yb.download().tobytes() == ya.download().tobytes()
