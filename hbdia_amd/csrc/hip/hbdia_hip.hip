@@ -70,21 +70,22 @@ template <> __device__ __forceinline__ float fma_t<float>(float a, float b, floa
   return __builtin_fmaf(a, b, c);
 }
 
-// ok ? fma(a, b, c) : c, computed as fma(a & m, b & m, c) with m = all ones or
-// zero.  For !ok that is fma(+0, +0, c) = c + (+0), which is c for every c
-// except -0; the FMA chains here start at +0 and round to nearest, so they
-// never hold -0.  Masking the operands instead of selecting the result keeps
+// ok ? fma(a, b, c) : c, computed as fma(a', b', c) with a' = -0 and b' = +0
+// when !ok: c + (-0) is c for every c, -0 and NaN included (a sum that
+// underflowed to -0 keeps its sign across a skipped entry, as the reference's
+// skip does).  Masking the operands instead of selecting the result keeps
 // every load of a chunk unconditional: with a select the compiler sinks the x
 // load behind the test of the loaded value, a second memory latency per chunk.
 __device__ __forceinline__ double masked_fma(bool ok, double a, double b, double c) {
   const uint64_t m = ok ? ~0ull : 0ull;
-  const double am = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, a) & m);
+  const uint64_t sign = 0x8000000000000000ull;
+  const double am = __builtin_bit_cast(double, (__builtin_bit_cast(uint64_t, a) & m) | (~m & sign));
   const double bm = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, b) & m);
   return __builtin_fma(am, bm, c);
 }
 __device__ __forceinline__ float masked_fma(bool ok, float a, float b, float c) {
   const uint32_t m = ok ? ~0u : 0u;
-  const float am = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, a) & m);
+  const float am = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, a) & m) | (~m & 0x80000000u));
   const float bm = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, b) & m);
   return __builtin_fmaf(am, bm, c);
 }

@@ -474,3 +474,28 @@ def test_runs3_hint_on_mixed_layouts(gpu, seed):
     hb = {"values": values2, "offsets": offsets2, "block_start": block_start2, "num_blocks": nb, "block_width": bw,
           "coo_rows": coo[0], "coo_cols": coo[1], "coo_values": coo[2]}
     np.testing.assert_array_equal(outs[1], oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N))
+
+
+@pytest.mark.parametrize("dtype,tiny", [(np.float64, 1e-200), (np.float32, 1e-30)])
+@pytest.mark.parametrize("offsets", [(0, 1), (-1, 0, 1), (-1, 0, 1, 5)])
+def test_negative_zero_sum_survives_skipped_entries(gpu, dtype, tiny, offsets):
+    """A row sum that underflows to -0 keeps its sign across later zero-valued entries, which the
+    reference skips (src/Operations/HBDIASpMV.cu:50-53): the first offset's product -tiny * tiny rounds
+    to -0, the other offsets hold explicit zeros.  Covers the plain, runs-of-three and two-row paths."""
+    print(f"case: -0 across skipped entries {np.dtype(dtype).name} offsets {offsets}", flush=True)
+    n = 512
+    rows, cols, vals = [], [], []
+    for r in range(n):
+        for k, o in enumerate(offsets):
+            if 0 <= r + o < n:
+                rows.append(r)
+                cols.append(r + o)
+                vals.append(-tiny if k == 0 else 0.0)
+    A = HBDIA(np.array(rows), np.array(cols), np.array(vals, dtype=dtype), n, n)
+    A.convertToHBDIAFormat(64, 1)
+    assert A.info()["num_coo"] == 0
+    x = np.full(n, tiny, dtype)
+    want = oracle_y(A, x, np.zeros(n, dtype))
+    assert np.signbit(want).sum() > n // 2  # the case is exercised: most rows end at -0
+    y = run_spmv(A, x, np.zeros(n, dtype), False, True)
+    assert y.tobytes() == want.tobytes()
