@@ -1537,6 +1537,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
     const TV* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
     const int vlane = 2 * lane + ((lane >= kWave / 2 && bB != bA) ? (cA - 1) * bw : 0);
     const int* op = offs + sA;
+    const uint32_t n32 = static_cast<uint32_t>(nrows);
     float sa = 0.0f, aa = 0.0f, sb = 0.0f, ab = 0.0f;
     for (int o0 = 0; o0 < cA; o0 += kChunk) {
       int off[kChunk];
@@ -1574,9 +1575,12 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
       }
 #pragma unroll
       for (int k = 0; k < kChunk; ++k) {
-        const int64_t ca = row + off[k], cb = ca + 1;
-        const bool oka = (o0 + k < cA) & act_a & (va[k] != 0.0f) & (ca >= 0) & (ca < nrows);
-        const bool okb = (o0 + k < cA) & act_b & (vbv[k] != 0.0f) & (cb >= 0) & (cb < nrows);
+        // rows and columns are below 2^31: row + offset modulo 2^32 is below nrows
+        // exactly when the column is in [0, nrows); rows past the end are cleared
+        // after the loop instead of per entry
+        const uint32_t ca = static_cast<uint32_t>(row) + static_cast<uint32_t>(off[k]), cb = ca + 1u;
+        const bool oka = (o0 + k < cA) & (va[k] != 0.0f) & (ca < n32);
+        const bool okb = (o0 + k < cA) & (vbv[k] != 0.0f) & (cb < n32);
         const float xfa = static_cast<float>(xa[k]), xfb = static_cast<float>(xb[k]);
         sa = oka ? __builtin_fmaf(va[k], xfa, sa) : sa;
         aa = oka ? __builtin_fmaf(__builtin_fabsf(va[k]), __builtin_fabsf(xfa), aa) : aa;
@@ -1584,6 +1588,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
         ab = okb ? __builtin_fmaf(__builtin_fabsf(vbv[k]), __builtin_fabsf(xfb), ab) : ab;
       }
     }
+    if (!act_a) sa = aa = 0.0f;
+    if (!act_b) sb = ab = 0.0f;
     float* sy = s_y[wv];
     sy[2 * lane] = sa;
     sy[2 * lane + 1] = sb;
