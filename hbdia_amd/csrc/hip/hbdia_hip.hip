@@ -1102,7 +1102,17 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
       const bool active = row < nrows;
       const T* x = xp + rows;
       T* y = yp + rows;
-      const int64_t last = nrows - 1;
+      // A matrix's rows are below 2^31: row + offset modulo 2^32 is below n32
+      // exactly when the column lies in [0, nrows), and an out-of-range column
+      // lies below 0 for a negative offset and past the last row otherwise.
+      // Rows past the end compute unused sums (they are not stored).
+      const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(num_rows);
+      const uint32_t last32 = n32 - 1u;
+      auto col_of = [&](int o, bool& in) -> uint32_t {
+        const uint32_t c = row32 + static_cast<uint32_t>(o);
+        in = c < n32;
+        return in ? c : (o < 0 ? 0u : last32);
+      };
 
       // ---- issue the tile's loads: fallback entries, first chunk of values and x
       int r_ = 0, c_ = 0;
@@ -1119,12 +1129,9 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
 #pragma unroll
       for (int k = 0; k < kChunk; ++k)
         v[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(k, lastk) * bw + lane)));
+      bool in1[kChunk];
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        int64_t c = row + off[k];
-        c = c < 0 ? 0 : (c > last ? last : c);
-        xl[k] = load_x<T, X32>(x, c);
-      }
+      for (int k = 0; k < kChunk; ++k) xl[k] = load_x<T, X32>(x, col_of(off[k], in1[k]));
       // every load above is issued before anything waits for one of them
       __builtin_amdgcn_sched_barrier(0);
       T ex = T(0);
@@ -1139,11 +1146,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
       // ---- blocked part: the FMA chain over the first chunk, then any further chunks
       T sum = T(0);
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        const int64_t c = row + off[k];
-        const bool ok = (k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
-        sum = masked_fma(ok, v[k], xl[k], sum);
-      }
+      for (int k = 0; k < kChunk; ++k) sum = masked_fma((k < cnt) & (v[k] != T(0)) & in1[k], v[k], xl[k], sum);
       if (cnt > kChunk) {
         const int* op = offs + offs_at;
         for (int o0 = kChunk; o0 < cnt; o0 += kChunk) {
@@ -1154,18 +1157,12 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
 #pragma unroll
           for (int k = 0; k < kChunk; ++k)
             v2[k] = __builtin_nontemporal_load(at_u32(vb, static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + lane)));
+          bool in2[kChunk];
 #pragma unroll
-          for (int k = 0; k < kChunk; ++k) {
-            int64_t c = row + of2[k];
-            c = c < 0 ? 0 : (c > last ? last : c);
-            x2[k] = load_x<T, X32>(x, c);
-          }
+          for (int k = 0; k < kChunk; ++k) x2[k] = load_x<T, X32>(x, col_of(of2[k], in2[k]));
 #pragma unroll
-          for (int k = 0; k < kChunk; ++k) {
-            const int64_t c = row + of2[k];
-            const bool ok = (o0 + k < cnt) & active & (v2[k] != T(0)) & (c >= 0) & (c < nrows);
-            sum = masked_fma(ok, v2[k], x2[k], sum);
-          }
+          for (int k = 0; k < kChunk; ++k)
+            sum = masked_fma((o0 + k < cnt) & (v2[k] != T(0)) & in2[k], v2[k], x2[k], sum);
         }
       }
 
