@@ -859,6 +859,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
     // x load per run (runs3_x); same x values, skips and FMA order as below
     const T* vb = vals + static_cast<int64_t>(start) * bw + (row0 - b * bw);
     const int* op = offs + start;
+    const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
     const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(T)) < (1ll << 32);
     for (int o0 = 0; o0 < cnt; o0 += 3 * RR) {
       int off[3 * RR];
@@ -873,16 +874,21 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
       }
       // every column's x windows for the chunk before any FMA
       T mid[KB][RR], first[KB][RR], after[KB][RR];
+      uint32_t cm[RR];
+#pragma unroll
+      for (int w = 0; w < RR; ++w) {
+        // rows are below 2^31: row + offset modulo 2^32 is below n32 exactly when
+        // the column is in range; out of range, the offset's sign picks the clamp
+        const int o = off[3 * w + 1];
+        const uint32_t c = row32 + static_cast<uint32_t>(o);
+        cm[w] = c < n32 ? c : (o < 0 ? 0u : n32 - 1u);
+      }
 #pragma unroll
       for (int j = 0; j < KB; ++j) {
         if (j < kb) {
           const T* xj = Xj + static_cast<int64_t>(j) * ldx;
 #pragma unroll
-          for (int w = 0; w < RR; ++w) {
-            int64_t c = row + off[3 * w + 1];
-            c = c < 0 ? 0 : (c > last ? last : c);
-            mid[j][w] = load_x<T, X32>(xj, c);
-          }
+          for (int w = 0; w < RR; ++w) mid[j][w] = load_x<T, X32>(xj, cm[w]);
         }
       }
 #pragma unroll
@@ -900,6 +906,11 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
           }
         }
       }
+      // rows past the end compute sums that are never stored
+      bool ok[3 * RR];
+#pragma unroll
+      for (int k = 0; k < 3 * RR; ++k)
+        ok[k] = (o0 + k < cnt) & (v[k] != T(0)) & (row32 + static_cast<uint32_t>(off[k]) < n32);
 #pragma unroll
       for (int j = 0; j < KB; ++j) {
         if (j >= kb) break;
@@ -907,12 +918,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
         for (int w = 0; w < RR; ++w) {
           const T xl[3] = {wave_shr1(mid[j][w], first[j][w]), mid[j][w], wave_shl1(mid[j][w], after[j][w])};
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            const int k = 3 * w + i;
-            const int64_t c = row + off[k];
-            const bool ok = (o0 + k < cnt) & active & (v[k] != T(0)) & (c >= 0) & (c < nrows);
-            sum[j] = masked_fma(ok, v[k], xl[i], sum[j]);
-          }
+          for (int i = 0; i < 3; ++i) sum[j] = masked_fma(ok[3 * w + i], v[3 * w + i], xl[i], sum[j]);
         }
       }
     }
@@ -1830,9 +1836,9 @@ int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   const bool with_coo = gptr != nullptr;
   if (with_coo && (!cr || !cc || !cv))
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null COO pointer");
+  if (nrows > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows");
   const int64_t groups = num_groups(nrows);
   const int64_t wgs = (groups + kWavesPerWG - 1) / kWavesPerWG;
-  if (wgs > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows");
   const bool x32 = static_cast<uint64_t>(nrows) * sizeof(T) < (1ull << 32);
   // 8 columns per pass; more columns take more passes, a row block's passes
   // adjacent.  Passes of 16 columns measured slower (k = 16: 0.679 against
