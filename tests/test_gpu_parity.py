@@ -499,3 +499,34 @@ def test_negative_zero_sum_survives_skipped_entries(gpu, dtype, tiny, offsets):
     assert np.signbit(want).sum() > n // 2  # the case is exercised: most rows end at -0
     y = run_spmv(A, x, np.zeros(n, dtype), False, True)
     assert y.tobytes() == want.tobytes()
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_matrix_market_file_through_the_gpu_path(gpu, tmp_path, symmetric):
+    """loadMTX -> convertToHBDIAFormat -> hbdiaSpMV on a Matrix Market file written here (the reference's
+    route for SuiteSparse inputs, run_HBDIA_test.cpp:19-83): bit-exact against the oracle on the loaded
+    COO, and within the stated tolerance of a long-double SpMV of the file's entries."""
+    print(f"case: Matrix Market 14^3 stencil + noise, symmetric={symmetric}", flush=True)
+    r, c, v, N = oracle.stencil27(14, 14, 14, 0.05, 2)
+    lines = []
+    if symmetric:  # the lower triangle, pattern only (value 1 each), mirrored by the loader
+        keep = r >= c
+        lines = [f"{a + 1} {b + 1}" for a, b in zip(r[keep], c[keep])]
+        header = "%%MatrixMarket matrix coordinate pattern symmetric\n"
+    else:
+        lines = [f"{a + 1} {b + 1} {val!r}" for a, b, val in zip(r, c, v.tolist())]
+        header = "%%MatrixMarket matrix coordinate real general\n"
+    f = tmp_path / "m.mtx"
+    f.write_text(header + f"{N} {N} {len(lines)}\n" + "\n".join(lines) + "\n")
+    A = HBDIA()
+    assert A.loadMTX(str(f))
+    R, C, V = A.getCOO()
+    A.convertToHBDIAFormat(64, 2)
+    x = np.random.default_rng(9).uniform(-1, 1, N)
+    y = run_spmv(A, x, np.zeros(N), False, True)
+    np.testing.assert_array_equal(y, oracle_y(A, x, np.zeros(N)))
+    ref = oracle.coo_spmv_longdouble(R, C, V, x, N)
+    scale = oracle.coo_spmv_longdouble(R, C, np.abs(V), np.abs(x), N)
+    assert np.max(np.abs(y - ref) / np.maximum(scale, 1e-300)) < 1e-12
+    if symmetric:
+        assert np.all(V == 1.0) and len(V) > len(lines)
