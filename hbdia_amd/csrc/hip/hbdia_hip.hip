@@ -1550,13 +1550,15 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
 #pragma unroll
       for (int k = 0; k < kChunk; ++k)
         lp_pair(vb + static_cast<int64_t>(min(o0 + k, cA - 1)) * bw + vlane, va[k], vbv[k]);
-      double xa[kChunk], xb[kChunk];
+      // x in FP32: each run's centre pair converted once, its neighbours shifted in
+      // as FP32 (the same values as converting the shifted FP64 ones)
+      float xa[kChunk], xb[kChunk];
 #pragma unroll
       for (int w = 0; w < kChunk / 3; ++w) {
         // runs past the block's count read padding: clamped, their products unused
         const int64_t o = off[3 * w + 1];
         dbl2a8 m;
-        if (row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last) {
+        if (row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last) {  // wave-uniform
           m = *reinterpret_cast<const dbl2a8*>(x + (row + o));
         } else {
           int64_t c0 = row + o, c1 = row + 1 + o;
@@ -1568,13 +1570,14 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
         int64_t cf = row0 + o - 1, ce = row0 + o + 2 * kWave;
         cf = cf < 0 ? 0 : (cf > last ? last : cf);
         ce = ce < 0 ? 0 : (ce > last ? last : ce);
-        const double first = x[cf], after = x[ce];
-        xa[3 * w] = wave_shr1(m.y, first);
-        xa[3 * w + 1] = m.x;
-        xa[3 * w + 2] = m.y;
-        xb[3 * w] = m.x;
-        xb[3 * w + 1] = m.y;
-        xb[3 * w + 2] = wave_shl1(m.x, after);
+        const float first = static_cast<float>(x[cf]), after = static_cast<float>(x[ce]);
+        const float mx = static_cast<float>(m.x), my = static_cast<float>(m.y);
+        xa[3 * w] = wave_shr1(my, first);
+        xa[3 * w + 1] = mx;
+        xa[3 * w + 2] = my;
+        xb[3 * w] = mx;
+        xb[3 * w + 1] = my;
+        xb[3 * w + 2] = wave_shl1(mx, after);
       }
 #pragma unroll
       for (int k = 0; k < kChunk; ++k) {
@@ -1584,11 +1587,10 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
         const uint32_t ca = static_cast<uint32_t>(row) + static_cast<uint32_t>(off[k]), cb = ca + 1u;
         const bool oka = (o0 + k < cA) & (va[k] != 0.0f) & (ca < n32);
         const bool okb = (o0 + k < cA) & (vbv[k] != 0.0f) & (cb < n32);
-        const float xfa = static_cast<float>(xa[k]), xfb = static_cast<float>(xb[k]);
-        sa = oka ? __builtin_fmaf(va[k], xfa, sa) : sa;
-        aa = oka ? __builtin_fmaf(__builtin_fabsf(va[k]), __builtin_fabsf(xfa), aa) : aa;
-        sb = okb ? __builtin_fmaf(vbv[k], xfb, sb) : sb;
-        ab = okb ? __builtin_fmaf(__builtin_fabsf(vbv[k]), __builtin_fabsf(xfb), ab) : ab;
+        sa = oka ? __builtin_fmaf(va[k], xa[k], sa) : sa;
+        aa = oka ? __builtin_fmaf(__builtin_fabsf(va[k]), __builtin_fabsf(xa[k]), aa) : aa;
+        sb = okb ? __builtin_fmaf(vbv[k], xb[k], sb) : sb;
+        ab = okb ? __builtin_fmaf(__builtin_fabsf(vbv[k]), __builtin_fabsf(xb[k]), ab) : ab;
       }
     }
     if (!act_a) sa = aa = 0.0f;
