@@ -387,6 +387,53 @@ int xcd_chunk_shift(int64_t nwg) {
   return k;
 }
 
+// The blocked part of one 64-row group (rows row0 + lane) for a block width
+// that is not a multiple of 64; start and cnt are the calling lane's block's
+// (zero past the last row).
+template <typename T, bool X32, bool RUNS3>
+__device__ __forceinline__ T group_bdia_nonuniform(const T* __restrict__ vals, const int* __restrict__ offs,
+                                                   const int* __restrict__ blk, const T* __restrict__ x,
+                                                   int64_t row0, int lane, int64_t nrows, int nb, int bw, int64_t b,
+                                                   int start, int cnt, const uint32_t* __restrict__ runs3_blocks) {
+  const int64_t row = row0 + lane;
+  const bool active = row < nrows;
+  T sum = T(0);
+  // Block width 32 (the reference's default): the wave's rows are blocks
+  // b0 = row0 / 32 and b0 + 1.  When both exist and hold the same offsets,
+  // the wave runs the uniform sums with a per-lane value index (block
+  // b0 + 1's lines follow block b0's: lane l >= 32 reads line
+  // pcnt + o at l - 32); otherwise each lane runs its own block.
+  bool pair = false;
+  int pstart = 0, pcnt = 0;
+  int64_t b0 = 0;
+  if (bw == 32) {
+    b0 = row0 / 32;
+    if (b0 + 1 < nb) {
+      const int s0 = blk[b0], s1 = blk[b0 + 1], e1 = blk[b0 + 2];
+      if (s1 - s0 == e1 - s1 && s1 > s0 && s1 - s0 <= kWave) {
+        const int c0 = s1 - s0;
+        const int oa = lane < c0 ? offs[s0 + lane] : 0;
+        const int ob = lane < c0 ? offs[s1 + lane] : 0;
+        pair = __ballot(oa != ob) == 0;
+        pstart = s0;
+        pcnt = c0;
+      }
+    }
+  }
+  if (pair) {
+    const T* vb = vals + static_cast<int64_t>(pstart) * 32;
+    const int vlane = lane + (lane >= 32 ? (pcnt - 1) * 32 : 0);
+    bool runs = false;
+    if constexpr (RUNS3) runs = (runs3_blocks[b0 >> 5] >> (b0 & 31)) & 1u;
+    if (runs) sum = bdia_sum_runs3<T, X32>(vb, offs + pstart, pcnt, 32, vlane, x, row0, row, nrows, active);
+    else sum = bdia_sum_uniform<T, X32>(vb, offs + pstart, pcnt, 32, vlane, x, row, nrows, active);
+  } else if (cnt > 0) {
+    const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
+    sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
+  }
+  return sum;
+}
+
 // Fused HBDIA SpMV.  UNIFORM: block_width % 64 == 0, so a wave's 64 rows lie
 // in one block.  HAS_COO: the matrix has a non-empty COO fallback, and then
 // every row receives y = (bdia or old y) + coo, as the reference's
@@ -434,39 +481,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia(
       sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
     }
   } else {
-    // Block width 32 (the reference's default): the wave's rows are blocks
-    // b0 = row0 / 32 and b0 + 1.  When both exist and hold the same offsets,
-    // the wave runs the uniform sums with a per-lane value index (block
-    // b0 + 1's lines follow block b0's: lane l >= 32 reads line
-    // pcnt + o at l - 32); otherwise each lane runs its own block.
-    bool pair = false;
-    int pstart = 0, pcnt = 0;
-    int64_t b0 = 0;
-    if (bw == 32) {
-      b0 = row0 / 32;
-      if (b0 + 1 < nb) {
-        const int s0 = blk[b0], s1 = blk[b0 + 1], e1 = blk[b0 + 2];
-        if (s1 - s0 == e1 - s1 && s1 > s0 && s1 - s0 <= kWave) {
-          const int c0 = s1 - s0;
-          const int oa = lane < c0 ? offs[s0 + lane] : 0;
-          const int ob = lane < c0 ? offs[s1 + lane] : 0;
-          pair = __ballot(oa != ob) == 0;
-          pstart = s0;
-          pcnt = c0;
-        }
-      }
-    }
-    if (pair) {
-      const T* vb = vals + static_cast<int64_t>(pstart) * 32;
-      const int vlane = lane + (lane >= 32 ? (pcnt - 1) * 32 : 0);
-      bool runs = false;
-      if constexpr (RUNS3) runs = (runs3_blocks[b0 >> 5] >> (b0 & 31)) & 1u;
-      if (runs) sum = bdia_sum_runs3<T, X32>(vb, offs + pstart, pcnt, 32, vlane, x, row0, row, nrows, active);
-      else sum = bdia_sum_uniform<T, X32>(vb, offs + pstart, pcnt, 32, vlane, x, row, nrows, active);
-    } else if (cnt > 0) {
-      const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
-      sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
-    }
+    sum = group_bdia_nonuniform<T, X32, RUNS3>(vals, offs, blk, x, row0, lane, nrows, nb, bw, b, start, cnt,
+                                               runs3_blocks);
   }
   if constexpr (HAS_COO) {
     const int p0 = gptr[grp], p1 = gptr[grp + 1];
@@ -742,6 +758,118 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     const int64_t row = r0 + lane;
     const bool active = row < nrows;
     const int cnt = h ? cB : cA;
+    const double sum = h ? yB : yA;
+    if constexpr (HAS_COO) {
+      const int64_t g = 2 * w2 + h;
+      const int p0 = gptr[g], p1 = gptr[g + 1];
+      double cs = 0.0;
+      bool any = false;
+      if (p0 < p1)
+        cs = coo_group_sum<double>(cr, cc, cv, x, p0, p1, r0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
+      if (active) {
+        const double base = cnt > 0 ? sum : y[row];
+        __builtin_nontemporal_store(base + cs, y + row);
+      }
+    } else {
+      if (active && cnt > 0) __builtin_nontemporal_store(sum, y + row);
+    }
+  }
+}
+
+// FP64, block width 32 (the reference's default), runs-of-three hints: the
+// two-row waves of k_hbdia_row2 over four consecutive width-32 blocks.  When
+// the four blocks exist, hold the same number of offsets and the same offsets
+// (nearly every wave of a stencil), lane l's rows row0 + 2l and row0 + 2l + 1
+// lie in block b0 + l / 16, whose lines follow block b0's by (l / 16) * cnt
+// lines: the runs-of-three two-row sums run with that per-lane value index.
+// Otherwise each 64-row half takes k_hbdia's width-32 path.  Values, skips and
+// FMA order per row are those of k_hbdia<double, false, ...>.
+template <bool HAS_COO, bool X32>
+__global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
+    const double* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb,
+    const int* __restrict__ cr, const int* __restrict__ cc, const double* __restrict__ cv,
+    const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
+  __shared__ double s_v[kWavesPerWG][kWave];
+  __shared__ double s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  __shared__ double s_y[kWavesPerWG][2 * kWave];
+  constexpr int bw = 32;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned work = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  const int64_t w2 = static_cast<int64_t>(work) * kWavesPerWG + wv;  // 128-row wave
+  const int64_t row0 = w2 * 2 * kWave;
+  if (row0 >= nrows) return;
+  const bool hasB = row0 + kWave < nrows;
+  const int64_t b0 = row0 / bw;
+  bool quad = false;
+  int s0 = 0, c = 0;
+  if (b0 + 3 < nb) {
+    s0 = blk[b0];
+    c = blk[b0 + 1] - s0;
+    if (c > 0 && c <= kWave && blk[b0 + 2] == s0 + 2 * c && blk[b0 + 3] == s0 + 3 * c &&
+        blk[b0 + 4] == s0 + 4 * c && ((runs3_blocks[b0 >> 5] >> (b0 & 31)) & 1u)) {
+      const int oa = lane < c ? offs[s0 + lane] : 0;
+      bool diff = false;
+#pragma unroll
+      for (int j = 1; j < 4; ++j) diff = diff | (oa != (lane < c ? offs[s0 + j * c + lane] : 0));
+      quad = __ballot(diff) == 0;
+    }
+  }
+  double yA = 0.0, yB = 0.0;
+  int cntA = 0, cntB = 0;  // the lane's block's offset count, per half
+  if (quad) {
+    const int64_t row = row0 + 2 * lane;
+    const double* vb = vals + static_cast<int64_t>(s0) * bw;
+    const int vlane = 2 * lane + (lane >> 4) * (c - 1) * bw;
+    double sa, sb;
+    if (c <= 3 * kChunk)
+      bdia_sum_runs3_row2_27(vb, vlane, offs + s0, c, bw, x, row0, row, nrows, row < nrows, row + 1 < nrows, sa, sb);
+    else
+      bdia_sum_runs3_row2<kRow2Chunk>(vb, vlane, offs + s0, c, bw, x, row0, row, nrows, row < nrows,
+                                      row + 1 < nrows, sa, sb);
+    double* sy = s_y[wv];
+    sy[2 * lane] = sa;
+    sy[2 * lane + 1] = sb;
+    wave_sync();
+    yA = sy[lane];
+    yB = sy[kWave + lane];
+    wave_sync();
+    cntA = row0 + lane < nrows ? c : 0;
+    cntB = row0 + kWave + lane < nrows ? c : 0;
+  } else {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && !hasB) break;
+      const int64_t r0 = row0 + h * kWave;
+      const int64_t row = r0 + lane;
+      const int64_t b = row / bw;
+      int start = 0, cnt = 0;
+      if (b < nb && row < nrows) {
+        start = blk[b];
+        cnt = blk[b + 1] - start;
+      }
+      const double sum =
+          group_bdia_nonuniform<double, X32, true>(vals, offs, blk, x, r0, lane, nrows, nb, bw, b, start, cnt,
+                                                   runs3_blocks);
+      if (h == 0) {
+        yA = sum;
+        cntA = cnt;
+      } else {
+        yB = sum;
+        cntB = cnt;
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t r0 = row0 + h * kWave;
+    if (h == 1 && !hasB) break;
+    const int64_t row = r0 + lane;
+    const bool active = row < nrows;
+    const int cnt = h ? cntB : cntA;
     const double sum = h ? yB : yA;
     if constexpr (HAS_COO) {
       const int64_t g = 2 * w2 + h;
@@ -1769,6 +1897,19 @@ int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   const bool runs3 = (uniform || bw == 32) && runs3_blocks != nullptr;
   const int kshift = xcd_chunk_shift(wgs);
   if constexpr (std::is_same<T, double>::value) {
+    if (runs3 && bw == 32 && nrows <= 0x7fffffffLL) {
+      // 128-row waves over four width-32 blocks, two rows per lane
+      const int64_t wgs2 = (nrows + 2 * kWG - 1) / (2 * kWG);
+      const dim3 grid2(static_cast<unsigned>(wgs2));
+      const int ks2 = xcd_chunk_shift(wgs2);
+#define HBDIA_LAUNCH_W32(C, X)                                                                                 \
+  hipLaunchKernelGGL((k_hbdia_row2_w32<C, X>), grid2, block, 0, s, vals, offs, blk, x, y, nrows, nb, cr, cc, cv, \
+                     gptr, runs3_blocks, ks2)
+      if (with_coo) { if (x32) HBDIA_LAUNCH_W32(true, true); else HBDIA_LAUNCH_W32(true, false); }
+      else { if (x32) HBDIA_LAUNCH_W32(false, true); else HBDIA_LAUNCH_W32(false, false); }
+#undef HBDIA_LAUNCH_W32
+      return check_hip(hipGetLastError(), name);
+    }
     if (runs3 && uniform && nrows <= 0x7fffffffLL) {
       // 128-row waves, two rows per lane (32-bit column bound checks)
       const int64_t wgs2 = (nrows + 2 * kWG - 1) / (2 * kWG);

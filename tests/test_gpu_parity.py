@@ -350,19 +350,20 @@ def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k):
                                       err_msg=f"column {j}")
 
 
-def _width32_case(N, seed, with_coo, share):
-    """Block width 32: pairs of blocks (2k, 2k+1) that with probability `share` hold the same offset
-    list (then one wave runs them together), otherwise their own; lists mix runs of three, single
-    offsets (some out of range) and empty blocks; zero values over NaN x entries."""
+def _width32_case(N, seed, with_coo, share, group=2, kmax=5):
+    """Block width 32: blocks after the first of each `group` (2: pairs 2k, 2k+1; 4: quads 4k..4k+3)
+    that with probability `share` repeat the previous block's offset list (then one wave runs them
+    together), otherwise their own; lists of up to kmax - 1 runs of three or single offsets (some out
+    of range) and empty blocks; zero values over NaN x entries."""
     rng = np.random.default_rng(seed)
     bw = 32
     nb = -(-N // bw)
     offs_per = []
     for b in range(nb):
-        if b % 2 == 1 and rng.random() < share:
+        if b % group != 0 and rng.random() < share:
             offs_per.append(list(offs_per[-1]))
             continue
-        k = int(rng.integers(0, 5))
+        k = int(rng.integers(0, kmax))
         if rng.random() < 0.6:
             centres = 3 * np.sort(rng.choice(np.arange(-N // 2, N // 2), size=k, replace=False)) + 1
             offs_per.append([int(c + d) for c in centres for d in (-1, 0, 1)])
@@ -383,13 +384,22 @@ def _width32_case(N, seed, with_coo, share):
     return values, offsets, block_start, x, coo
 
 
-@pytest.mark.parametrize("N,with_coo,share", [(1000, False, 0.8), (1000, True, 0.8), (4096, True, 1.0),
-                                             (777, False, 0.5), (64, True, 1.0), (96, False, 1.0)])
-def test_width32_shared_offsets_are_bit_identical(gpu, N, with_coo, share):
-    """Width-32 waves whose two blocks share their offsets run the uniform sums (with the runs-of-three
-    hint where it applies): results equal the hint-free launch and the oracle."""
-    print(f"case: width32 N={N} coo={with_coo} share={share}", flush=True)
-    values, offsets, block_start, x, coo = _width32_case(N, 90 + N, with_coo, share)
+@pytest.mark.parametrize("N,with_coo,share,group,kmax,seed", [
+    (1000, False, 0.8, 2, 5, None), (1000, True, 0.8, 2, 5, None), (4096, True, 1.0, 2, 5, None),
+    (777, False, 0.5, 2, 5, None), (64, True, 1.0, 2, 5, None), (96, False, 1.0, 2, 5, None),
+    # quads of blocks with one offset list: the two-row width-32 waves, their partial last wave and
+    # their fallback to 64-row halves; kmax 13 reaches lists of more than 27 offsets; seed 0 gives
+    # the small cases one quad of runs of three
+    (1000, True, 0.9, 4, 5, None), (4096, False, 1.0, 4, 5, None), (4096, True, 1.0, 4, 13, None),
+    (777, True, 0.7, 4, 13, None), (130, True, 1.0, 4, 5, 0), (224, False, 1.0, 4, 5, 0), (128, True, 1.0, 4, 5, 0)])
+def test_width32_shared_offsets_are_bit_identical(gpu, N, with_coo, share, group, kmax, seed):
+    """Width-32 waves whose blocks share their offsets run the uniform sums (with the runs-of-three
+    hint where it applies; four blocks with one list run two rows per lane): results equal the
+    hint-free launch and the oracle."""
+    print(f"case: width32 N={N} coo={with_coo} share={share} group={group} kmax={kmax}", flush=True)
+    if seed is None:
+        seed = 90 + N + (0 if group == 2 else 7 * group + kmax)
+    values, offsets, block_start, x, coo = _width32_case(N, seed, with_coo, share, group, kmax)
     L = _layout(values, offsets, block_start, N, 32, coo)
     y0 = np.linspace(-1.0, 1.0, N)
     X = DeviceArray.from_host(x)
