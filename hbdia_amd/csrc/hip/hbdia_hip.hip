@@ -181,6 +181,51 @@ __device__ __forceinline__ T coo_group_sum(const int* __restrict__ cr, const int
   return coo_group_sum_from<T>(cr, cc, cv, x, p0, p1, row0, lane, s_v, s_x, s_lo, s_hi, T(0), any);
 }
 
+// The fallback sums of a 128-row wave's two groups when their entries
+// [p0, p2) fit one 64-entry pass (group A rows [row0, row0 + 64) holds
+// [p0, p1), group B the next 64 rows [p1, p2)): one load of rows, columns,
+// values and x for both groups, then each lane sums its row of each group in
+// entry order (FMA).  The sums equal coo_group_sum's per group bit for bit.
+// s_lo and s_hi hold 128 entries.
+template <typename T>
+__device__ __forceinline__ void coo_pair_sum(const int* __restrict__ cr, const int* __restrict__ cc,
+                                             const T* __restrict__ cv, const T* __restrict__ x, int p0, int p1,
+                                             int p2, int64_t row0, int lane, T* s_v, T* s_x, int* s_lo, int* s_hi,
+                                             T& csA, T& csB) {
+  const int i = p0 + lane;
+  const bool ok = i < p2;
+  int r = -1;
+  T vv = T(0), xx = T(0);
+  if (ok) {
+    r = cr[i] - static_cast<int>(row0);
+    vv = cv[i];
+    xx = x[cc[i]];
+    // an entry outside its own group's rows (unsorted input) is dropped, as the
+    // per-group sums drop it, rather than written outside LDS
+    if (r < 0 || r >= 2 * kWave || ((i < p1) != (r < kWave))) r = -1;
+  }
+  const int rp = __shfl_up(r, 1);
+  const int rn = __shfl_down(r, 1);
+  s_lo[lane] = 0;
+  s_lo[kWave + lane] = 0;
+  s_hi[lane] = 0;
+  s_hi[kWave + lane] = 0;
+  s_v[lane] = vv;
+  s_x[lane] = xx;
+  wave_sync();
+  if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
+  if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p2 || rn != r)) s_hi[r] = lane + 1;
+  wave_sync();
+  T a_sum = T(0), b_sum = T(0);
+  const int a0 = s_lo[lane], e0 = s_hi[lane];
+  for (int k = a0; k < e0; ++k) a_sum = fma_t<T>(s_v[k], s_x[k], a_sum);
+  const int a1 = s_lo[kWave + lane], e1 = s_hi[kWave + lane];
+  for (int k = a1; k < e1; ++k) b_sum = fma_t<T>(s_v[k], s_x[k], b_sum);
+  wave_sync();
+  csA = a_sum;
+  csB = b_sum;
+}
+
 // Element `i` of a wave-uniform base as base + 32-bit byte offset, so that the
 // load uses the scalar-base addressing form (one VGPR per address).
 template <typename T>
@@ -693,8 +738,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
   __shared__ double s_v[kWavesPerWG][kWave];
   __shared__ double s_x[kWavesPerWG][kWave];
-  __shared__ int s_lo[kWavesPerWG][kWave];
-  __shared__ int s_hi[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][2 * kWave];
+  __shared__ int s_hi[kWavesPerWG][2 * kWave];
   __shared__ double s_y[kWavesPerWG][2 * kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -751,6 +796,23 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     yA = half_bdia<X32>(vals, offs, x, row0, bA, sA, cA, bw, lane, nrows, runsA);
     if (hasB) yB = half_bdia<X32>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB);
   }
+  // fallback sums of both 64-row groups: one pass when their entries fit one
+  double csA = 0.0, csB = 0.0;
+  if constexpr (HAS_COO) {
+    const int p0 = gptr[2 * w2], p1 = gptr[2 * w2 + 1];
+    const int p2 = hasB ? gptr[2 * w2 + 2] : p1;
+    bool any = false;
+    if (p2 - p0 <= kWave) {
+      if (p0 < p2) coo_pair_sum<double>(cr, cc, cv, x, p0, p1, p2, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
+                                        csA, csB);
+    } else {
+      if (p0 < p1)
+        csA = coo_group_sum<double>(cr, cc, cv, x, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
+      if (p1 < p2)
+        csB = coo_group_sum<double>(cr, cc, cv, x, p1, p2, row0 + kWave, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
+                                    &any);
+    }
+  }
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int64_t r0 = row0 + h * kWave;
@@ -760,12 +822,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     const int cnt = h ? cB : cA;
     const double sum = h ? yB : yA;
     if constexpr (HAS_COO) {
-      const int64_t g = 2 * w2 + h;
-      const int p0 = gptr[g], p1 = gptr[g + 1];
-      double cs = 0.0;
-      bool any = false;
-      if (p0 < p1)
-        cs = coo_group_sum<double>(cr, cc, cv, x, p0, p1, r0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
+      const double cs = h ? csB : csA;
       if (active) {
         const double base = cnt > 0 ? sum : y[row];
         __builtin_nontemporal_store(base + cs, y + row);
@@ -792,8 +849,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
     const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
   __shared__ double s_v[kWavesPerWG][kWave];
   __shared__ double s_x[kWavesPerWG][kWave];
-  __shared__ int s_lo[kWavesPerWG][kWave];
-  __shared__ int s_hi[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][2 * kWave];
+  __shared__ int s_hi[kWavesPerWG][2 * kWave];
   __shared__ double s_y[kWavesPerWG][2 * kWave];
   constexpr int bw = 32;
   const int lane = threadIdx.x & (kWave - 1);
@@ -863,6 +920,23 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
       }
     }
   }
+  // fallback sums of both 64-row groups: one pass when their entries fit one
+  double csA = 0.0, csB = 0.0;
+  if constexpr (HAS_COO) {
+    const int p0 = gptr[2 * w2], p1 = gptr[2 * w2 + 1];
+    const int p2 = hasB ? gptr[2 * w2 + 2] : p1;
+    bool any = false;
+    if (p2 - p0 <= kWave) {
+      if (p0 < p2) coo_pair_sum<double>(cr, cc, cv, x, p0, p1, p2, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
+                                        csA, csB);
+    } else {
+      if (p0 < p1)
+        csA = coo_group_sum<double>(cr, cc, cv, x, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
+      if (p1 < p2)
+        csB = coo_group_sum<double>(cr, cc, cv, x, p1, p2, row0 + kWave, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
+                                    &any);
+    }
+  }
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int64_t r0 = row0 + h * kWave;
@@ -872,12 +946,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
     const int cnt = h ? cntB : cntA;
     const double sum = h ? yB : yA;
     if constexpr (HAS_COO) {
-      const int64_t g = 2 * w2 + h;
-      const int p0 = gptr[g], p1 = gptr[g + 1];
-      double cs = 0.0;
-      bool any = false;
-      if (p0 < p1)
-        cs = coo_group_sum<double>(cr, cc, cv, x, p0, p1, r0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
+      const double cs = h ? csB : csA;
       if (active) {
         const double base = cnt > 0 ? sum : y[row];
         __builtin_nontemporal_store(base + cs, y + row);
