@@ -1626,51 +1626,101 @@ __device__ __forceinline__ void mixed_half(const TV* __restrict__ vals, const in
   }
 }
 
-// The fallback, the cancellation flag and the store of one 64-row group (rows
-// r0 + lane), exactly k_hbdia_mixed's
-template <typename TV, bool HAS_COO>
-__device__ __forceinline__ void mixed_finish(int64_t grp, int64_t r0, int lane, float sum, float sabs, int cnt,
-                                             const double* __restrict__ x, double* __restrict__ y, int64_t nrows,
-                                             const int* __restrict__ cr, const int* __restrict__ cc,
-                                             const TV* __restrict__ cv, const int* __restrict__ gptr, float ratio,
-                                             float u_store, uint64_t* __restrict__ flag_mask, float* s_v, float* s_x,
-                                             int* s_lo, int* s_hi) {
+// The fallback sums of one 64-row group (rows r0 + lane) for the calling
+// lane's row, in entry order: FP32 sum, FP32 sum of |a x| and the number of
+// terms added to nterms; exactly k_hbdia_mixed's.
+template <typename TV>
+__device__ __forceinline__ void mixed_coo_group(int p0, int p1, int64_t r0, int lane, const double* __restrict__ x,
+                                                const int* __restrict__ cr, const int* __restrict__ cc,
+                                                const TV* __restrict__ cv, float* s_v, float* s_x, int* s_lo,
+                                                int* s_hi, float& cs, float& cabs, int& nterms) {
+  for (int base = p0; base < p1; base += kWave) {
+    const int i = base + lane;
+    const bool ok = i < p1;
+    int r = -1;
+    float vv = 0.0f, xx = 0.0f;
+    if (ok) {
+      r = cr[i] - static_cast<int>(r0);
+      vv = lp_to_f32<TV>(cv[i]);
+      xx = static_cast<float>(x[cc[i]]);
+      if (r < 0 || r >= kWave) r = -1;
+    }
+    const int rp = __shfl_up(r, 1);
+    const int rn = __shfl_down(r, 1);
+    s_lo[lane] = 0;
+    s_hi[lane] = 0;
+    s_v[lane] = vv;
+    s_x[lane] = xx;
+    wave_sync();
+    if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
+    if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p1 || rn != r)) s_hi[r] = lane + 1;
+    wave_sync();
+    const int a = s_lo[lane], e = s_hi[lane];
+    nterms += e - a;
+    for (int k = a; k < e; ++k) {
+      cs = __builtin_fmaf(s_v[k], s_x[k], cs);
+      cabs = __builtin_fmaf(__builtin_fabsf(s_v[k]), __builtin_fabsf(s_x[k]), cabs);
+    }
+    wave_sync();
+  }
+}
+
+// mixed_coo_group for both 64-row groups of a 128-row wave whose entries
+// [p0, p2) fit one 64-entry pass (group A [p0, p1), group B [p1, p2)): one
+// load for both, then the same per-row sums in entry order.  s_lo and s_hi
+// hold 128 entries.
+template <typename TV>
+__device__ __forceinline__ void mixed_coo_pair(int p0, int p1, int p2, int64_t row0, int lane,
+                                               const double* __restrict__ x, const int* __restrict__ cr,
+                                               const int* __restrict__ cc, const TV* __restrict__ cv, float* s_v,
+                                               float* s_x, int* s_lo, int* s_hi, float* cs, float* cabs,
+                                               int* nterms) {
+  const int i = p0 + lane;
+  const bool ok = i < p2;
+  int r = -1;
+  float vv = 0.0f, xx = 0.0f;
+  if (ok) {
+    r = cr[i] - static_cast<int>(row0);
+    vv = lp_to_f32<TV>(cv[i]);
+    xx = static_cast<float>(x[cc[i]]);
+    // an entry outside its own group's rows (unsorted input) is dropped
+    if (r < 0 || r >= 2 * kWave || ((i < p1) != (r < kWave))) r = -1;
+  }
+  const int rp = __shfl_up(r, 1);
+  const int rn = __shfl_down(r, 1);
+  s_lo[lane] = 0;
+  s_lo[kWave + lane] = 0;
+  s_hi[lane] = 0;
+  s_hi[kWave + lane] = 0;
+  s_v[lane] = vv;
+  s_x[lane] = xx;
+  wave_sync();
+  if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
+  if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p2 || rn != r)) s_hi[r] = lane + 1;
+  wave_sync();
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int a = s_lo[h * kWave + lane], e = s_hi[h * kWave + lane];
+    nterms[h] += e - a;
+    float c = 0.0f, ca = 0.0f;
+    for (int k = a; k < e; ++k) {
+      c = __builtin_fmaf(s_v[k], s_x[k], c);
+      ca = __builtin_fmaf(__builtin_fabsf(s_v[k]), __builtin_fabsf(s_x[k]), ca);
+    }
+    cs[h] = c;
+    cabs[h] = ca;
+  }
+  wave_sync();
+}
+
+// The cancellation flag and the store of one 64-row group (rows r0 + lane)
+// from its blocked and fallback sums, exactly k_hbdia_mixed's
+template <bool HAS_COO>
+__device__ __forceinline__ void mixed_store(int64_t grp, int64_t r0, int lane, float sum, float sabs, int cnt,
+                                            float cs, float cabs, int nterms, double* __restrict__ y, int64_t nrows,
+                                            float ratio, float u_store, uint64_t* __restrict__ flag_mask) {
   const int64_t row = r0 + lane;
   const bool active = row < nrows;
-  float cs = 0.0f, cabs = 0.0f;
-  int nterms = cnt;
-  if constexpr (HAS_COO) {
-    const int p0 = gptr[grp], p1 = gptr[grp + 1];
-    for (int base = p0; base < p1; base += kWave) {
-      const int i = base + lane;
-      const bool ok = i < p1;
-      int r = -1;
-      float vv = 0.0f, xx = 0.0f;
-      if (ok) {
-        r = cr[i] - static_cast<int>(r0);
-        vv = lp_to_f32<TV>(cv[i]);
-        xx = static_cast<float>(x[cc[i]]);
-        if (r < 0 || r >= kWave) r = -1;
-      }
-      const int rp = __shfl_up(r, 1);
-      const int rn = __shfl_down(r, 1);
-      s_lo[lane] = 0;
-      s_hi[lane] = 0;
-      s_v[lane] = vv;
-      s_x[lane] = xx;
-      wave_sync();
-      if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
-      if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p1 || rn != r)) s_hi[r] = lane + 1;
-      wave_sync();
-      const int a = s_lo[lane], e = s_hi[lane];
-      nterms += e - a;
-      for (int k = a; k < e; ++k) {
-        cs = __builtin_fmaf(s_v[k], s_x[k], cs);
-        cabs = __builtin_fmaf(__builtin_fabsf(s_v[k]), __builtin_fabsf(s_x[k]), cabs);
-      }
-      wave_sync();
-    }
-  }
   const float tot = sum + cs;
   const float mag = sabs + cabs;
   const float u32 = 5.9604645e-08f;  // 2^-24
@@ -1697,8 +1747,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
     const uint32_t* __restrict__ runs3_blocks, int kshift) {
   __shared__ float s_v[kWavesPerWG][kWave];
   __shared__ float s_x[kWavesPerWG][kWave];
-  __shared__ int s_lo[kWavesPerWG][kWave];
-  __shared__ int s_hi[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][2 * kWave];
+  __shared__ int s_hi[kWavesPerWG][2 * kWave];
   __shared__ float s_y[kWavesPerWG][4 * kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1807,11 +1857,28 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
     mixed_half<TV>(vals, offs, x, row0, bA, sA, bA < nb ? cA : 0, bw, lane, nrows, runsA, sumA, absA);
     if (hasB) mixed_half<TV>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB, sumB, absB);
   }
-  mixed_finish<TV, HAS_COO>(2 * w2, row0, lane, sumA, absA, cA, x, y, nrows, cr, cc, cv, gptr, ratio, u_store,
-                            flag_mask, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv]);
+  // fallback sums of both groups: one pass when their entries fit one
+  float cs[2] = {0.0f, 0.0f}, cabs[2] = {0.0f, 0.0f};
+  int nterms[2] = {cA, cB};
+  if constexpr (HAS_COO) {
+    const int p0 = gptr[2 * w2], p1 = gptr[2 * w2 + 1];
+    const int p2 = hasB ? gptr[2 * w2 + 2] : p1;
+    if (p2 - p0 <= kWave) {
+      if (p0 < p2)
+        mixed_coo_pair<TV>(p0, p1, p2, row0, lane, x, cr, cc, cv, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], cs, cabs,
+                           nterms);
+    } else {
+      mixed_coo_group<TV>(p0, p1, row0, lane, x, cr, cc, cv, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], cs[0], cabs[0],
+                          nterms[0]);
+      mixed_coo_group<TV>(p1, p2, row0 + kWave, lane, x, cr, cc, cv, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], cs[1],
+                          cabs[1], nterms[1]);
+    }
+  }
+  mixed_store<HAS_COO>(2 * w2, row0, lane, sumA, absA, cA, cs[0], cabs[0], nterms[0], y, nrows, ratio, u_store,
+                       flag_mask);
   if (hasB)
-    mixed_finish<TV, HAS_COO>(2 * w2 + 1, row0 + kWave, lane, sumB, absB, cB, x, y, nrows, cr, cc, cv, gptr, ratio,
-                              u_store, flag_mask, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv]);
+    mixed_store<HAS_COO>(2 * w2 + 1, row0 + kWave, lane, sumB, absB, cB, cs[1], cabs[1], nterms[1], y, nrows, ratio,
+                         u_store, flag_mask);
 }
 
 // FP64 recomputation of one row with the FP64 kernel's arithmetic: blocked
