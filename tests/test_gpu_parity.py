@@ -320,6 +320,53 @@ def test_runs3_hint_is_bit_identical(gpu, N, bw, with_coo):
     np.testing.assert_array_equal(outs[1], want)
 
 
+def _dense_fallback(N, seed):
+    """Fallback entries, row-sorted, entry order within a row random: every third 64-row group is
+    dense (0-25 entries per row, so far more than 64 per group, some rows empty), the others
+    sparse (at most one entry per row, so that two groups often fit one 64-entry pass); duplicate
+    entries and zero values included."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for g in range(-(-N // 64)):
+        r = np.arange(64 * g, min(64 * g + 64, N))
+        per = rng.integers(0, 26, len(r)) if g % 3 == 0 else (rng.random(len(r)) < 0.15).astype(np.int64)
+        rows.append(np.repeat(r, per))
+    r = np.concatenate(rows).astype(np.int32)
+    c = rng.integers(0, N, len(r)).astype(np.int32)
+    c[1::17] = c[0::17][:len(c[1::17])]  # some duplicate columns
+    v = rng.standard_normal(len(r))
+    v[rng.random(len(r)) < 0.1] = 0.0
+    return r, c, v
+
+
+@pytest.mark.parametrize("N,bw,dtype", [(4099, 64, np.float64), (4096, 32, np.float64), (2000, 128, np.float64),
+                                        (4099, 64, np.float32)])
+def test_dense_fallback_groups_are_bit_identical(gpu, N, bw, dtype):
+    """Groups with more than 64 fallback entries (each lane sums its own row's segment) next to
+    sparse ones (two groups in one pass on the two-row kernels): results equal the hint-free launch
+    and the oracle bit for bit."""
+    print(f"case: dense fallback N={N} bw={bw} {np.dtype(dtype).name}", flush=True)
+    values, offsets, block_start, x, _ = _runs3_case(N, bw, 5 + N + bw, False)
+    coo = _dense_fallback(N, N + bw)
+    counts = np.bincount(coo[0] // 64, minlength=-(-N // 64))
+    assert counts.max() > 64 and np.any((counts[0::2][:len(counts[1::2])] + counts[1::2]) <= 64)
+    values, x = values.astype(dtype), x.astype(dtype)
+    coo = (coo[0], coo[1], coo[2].astype(dtype))
+    L = _layout(values, offsets, block_start, N, bw, coo)
+    y0 = np.linspace(-1.0, 1.0, N).astype(dtype)
+    X = DeviceArray.from_host(x)
+    outs = []
+    for hints in (False, True):
+        Y = DeviceArray.from_host(y0)
+        L.spmv(X, Y, hints=hints)
+        synchronize()
+        outs.append(Y.download())
+    np.testing.assert_array_equal(outs[1], outs[0])
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": len(block_start) - 1,
+          "block_width": bw, "coo_rows": coo[0], "coo_cols": coo[1], "coo_values": coo[2]}
+    np.testing.assert_array_equal(outs[1], oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N))
+
+
 @pytest.mark.parametrize("N,bw,with_coo,k", [(1000, 64, True, 3), (1000, 64, False, 8), (4099, 128, True, 5),
                                              (777, 256, True, 16)])
 def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k):
