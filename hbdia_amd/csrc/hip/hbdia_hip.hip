@@ -1010,31 +1010,19 @@ template <int KB> struct MrhsChunk { static constexpr int value = KB >= 16 ? 1 :
 // runs of three per chunk on the runs-of-three path
 template <int KB> struct MrhsRuns { static constexpr int value = KB >= 4 ? 1 : 2; };
 
+// One 64-row group (rows row0 + lane) of the multi-RHS product for columns
+// [j0, j0 + kb): the blocked part and the fused fallback, stored into Y.
 template <typename T, int KB, bool HAS_COO, bool X32, bool RUNS3>
-__global__ __launch_bounds__(kWG) void k_hbdia_multi(
-    const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
-    const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy, int nrhs, int64_t nrows, int nb,
-    int bw, const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
-    const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
+__device__ __forceinline__ void multi_group(const T* __restrict__ vals, const int* __restrict__ offs,
+                                            const int* __restrict__ blk, const T* __restrict__ X, int64_t ldx,
+                                            T* __restrict__ Y, int64_t ldy, int64_t nrows, int nb, int bw,
+                                            const int* __restrict__ cr, const int* __restrict__ cc,
+                                            const T* __restrict__ cv, const int* __restrict__ gptr,
+                                            const uint32_t* __restrict__ runs3_blocks, int64_t grp, int j0, int kb,
+                                            int lane, T* s_v_w, T* s_x_w, int* s_lo_w, int* s_hi_w) {
   constexpr int CH = MrhsChunk<KB>::value;
   constexpr int RR = MrhsRuns<KB>::value;
-  __shared__ T s_v[kWavesPerWG][kWave];
-  __shared__ T s_x[kWavesPerWG][kWave];
-  __shared__ int s_lo[kWavesPerWG][kWave];
-  __shared__ int s_hi[kWavesPerWG][kWave];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // one workgroup per (row block, column group): a row block's column groups are
-  // consecutive work items, which the XCD runs keep on one XCD, so the later
-  // groups read the block's values from that XCD's L2 instead of HBM
-  const unsigned passes = static_cast<unsigned>((nrhs + KB - 1) / KB);
-  const unsigned q = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
-  const unsigned work = q / passes;
-  const int64_t grp = static_cast<int64_t>(work) * kWavesPerWG + wv;
   const int64_t row0 = grp * kWave;
-  if (row0 >= nrows) return;
-  const int j0 = static_cast<int>(q - work * passes) * KB;
-  const int kb = min(KB, nrhs - j0);
   const T* Xj = X + static_cast<int64_t>(j0) * ldx;
   T* Yj = Y + static_cast<int64_t>(j0) * ldy;
   const int64_t row = row0 + lane;
@@ -1157,8 +1145,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
     const int p0 = gptr[grp], p1 = gptr[grp + 1];
     T cs[KB];
     if (p0 < p1) {
-      coo_group_sum_multi<T, KB>(cr, cc, cv, Xj, ldx, kb, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv],
-                                 s_hi[wv], cs);
+      coo_group_sum_multi<T, KB>(cr, cc, cv, Xj, ldx, kb, p0, p1, row0, lane, s_v_w, s_x_w, s_lo_w, s_hi_w, cs);
     } else {
 #pragma unroll
       for (int j = 0; j < KB; ++j) cs[j] = T(0);
@@ -1178,6 +1165,222 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
       for (int j = 0; j < KB; ++j) {
         if (j >= kb) break;
         __builtin_nontemporal_store(sum[j], Yj + static_cast<int64_t>(j) * ldy + row);
+      }
+    }
+  }
+}
+
+template <typename T, int KB, bool HAS_COO, bool X32, bool RUNS3>
+__global__ __launch_bounds__(kWG) void k_hbdia_multi(
+    const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy, int nrhs, int64_t nrows, int nb,
+    int bw, const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
+    const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
+  __shared__ T s_v[kWavesPerWG][kWave];
+  __shared__ T s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // one workgroup per (row block, column group): a row block's column groups are
+  // consecutive work items, which the XCD runs keep on one XCD, so the later
+  // groups read the block's values from that XCD's L2 instead of HBM
+  const unsigned passes = static_cast<unsigned>((nrhs + KB - 1) / KB);
+  const unsigned q = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  const unsigned work = q / passes;
+  const int64_t grp = static_cast<int64_t>(work) * kWavesPerWG + wv;
+  if (grp * kWave >= nrows) return;
+  const int j0 = static_cast<int>(q - work * passes) * KB;
+  const int kb = min(KB, nrhs - j0);
+  multi_group<T, KB, HAS_COO, X32, RUNS3>(vals, offs, blk, X, ldx, Y, ldy, nrows, nb, bw, cr, cc, cv, gptr,
+                                          runs3_blocks, grp, j0, kb, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv]);
+}
+
+// FP64 multi-RHS, block width a multiple of 64, runs-of-three hints: 128-row
+// waves with two adjacent rows per lane, as k_hbdia_row2.  A wave whose two
+// halves share one offset list loads both rows' values of an offset as one
+// 16-byte access (shared by the columns) and, per column, both rows' x of a
+// run's centre as one 16-byte access, with the neighbours shifted in by DPP
+// and the edge values as scalar loads.  Other waves run the 64-row code per
+// half.  Each column's values, skips and FMA order per row are the
+// single-vector kernel's, so every column equals a single-vector SpMV bit for
+// bit.
+template <int KB, bool HAS_COO, bool X32>
+__global__ __launch_bounds__(kWG) void k_hbdia_multi_row2(
+    const double* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const double* __restrict__ X, int64_t ldx, double* __restrict__ Y, int64_t ldy, int nrhs, int64_t nrows,
+    int nb, int bw, const int* __restrict__ cr, const int* __restrict__ cc, const double* __restrict__ cv,
+    const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
+  __shared__ double s_v[kWavesPerWG][kWave];
+  __shared__ double s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  __shared__ double s_y[kWavesPerWG][2 * kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned passes = static_cast<unsigned>((nrhs + KB - 1) / KB);
+  const unsigned q = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  const unsigned work = q / passes;
+  const int64_t w2 = static_cast<int64_t>(work) * kWavesPerWG + wv;  // 128-row wave
+  const int64_t row0 = w2 * 2 * kWave;
+  if (row0 >= nrows) return;
+  const int j0 = static_cast<int>(q - work * passes) * KB;
+  const int kb = min(KB, nrhs - j0);
+  const int64_t bA = row0 / bw, bB = (row0 + kWave) / bw;
+  const bool hasB = row0 + kWave < nrows;
+  int sA = 0, cA = 0, sB = 0, cB = 0;
+  if (bA < nb) {
+    sA = blk[bA];
+    cA = blk[bA + 1] - sA;
+  }
+  if (hasB && bB < nb) {
+    sB = blk[bB];
+    cB = blk[bB + 1] - sB;
+  }
+  const bool runsA = bA < nb && ((runs3_blocks[bA >> 5] >> (bA & 31)) & 1u);
+  bool same = false;
+  if (runsA && cA > 0 && cA <= kWave) {
+    if (bA == bB) {
+      same = true;
+    } else if (hasB && cB == cA) {
+      const int oa = lane < cA ? offs[sA + lane] : 0;
+      const int ob = lane < cA ? offs[sB + lane] : 0;
+      same = __ballot(oa != ob) == 0;
+    }
+  }
+  if (!same) {
+    multi_group<double, KB, HAS_COO, X32, true>(vals, offs, blk, X, ldx, Y, ldy, nrows, nb, bw, cr, cc, cv, gptr,
+                                                runs3_blocks, 2 * w2, j0, kb, lane, s_v[wv], s_x[wv], s_lo[wv],
+                                                s_hi[wv]);
+    if (hasB)
+      multi_group<double, KB, HAS_COO, X32, true>(vals, offs, blk, X, ldx, Y, ldy, nrows, nb, bw, cr, cc, cv, gptr,
+                                                  runs3_blocks, 2 * w2 + 1, j0, kb, lane, s_v[wv], s_x[wv], s_lo[wv],
+                                                  s_hi[wv]);
+    return;
+  }
+  const double* Xj = X + static_cast<int64_t>(j0) * ldx;
+  double* Yj = Y + static_cast<int64_t>(j0) * ldy;
+  const int64_t row = row0 + 2 * lane;
+  const bool act_a = row < nrows, act_b = row + 1 < nrows;
+  const int64_t last = nrows - 1;
+  const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
+  const double* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
+  // lanes of half B read block bB's lines, which follow block bA's
+  const int64_t vlane = 2 * lane + ((lane >= kWave / 2 && bB != bA) ? static_cast<int64_t>(cA - 1) * bw : 0);
+  const int* op = offs + sA;
+  double sa[KB], sb[KB];
+#pragma unroll
+  for (int j = 0; j < KB; ++j) sa[j] = sb[j] = 0.0;
+  for (int o0 = 0; o0 < cA; o0 += 3) {  // one run of three per step
+    int off[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) off[k] = op[o0 + k];  // padded array
+    dbl2a8 v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      v[k] = __builtin_nontemporal_load(
+          reinterpret_cast<const dbl2a8*>(vb + static_cast<int64_t>(min(o0 + k, cA - 1)) * bw + vlane));
+    // every column's centre pair and edge values before any FMA
+    const int64_t o = off[1];
+    const bool inside = row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last;  // wave-uniform
+    int64_t cf = row0 + o - 1, ce = row0 + o + 2 * kWave;
+    cf = cf < 0 ? 0 : (cf > last ? last : cf);
+    ce = ce < 0 ? 0 : (ce > last ? last : ce);
+    int64_t c0 = row + o, c1 = row + 1 + o;
+    c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+    c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+    dbl2a8 m[KB];
+    double first[KB], after[KB];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j < kb) {
+        const double* xj = Xj + static_cast<int64_t>(j) * ldx;
+        if (inside) {
+          m[j] = *reinterpret_cast<const dbl2a8*>(xj + (row + o));
+        } else {
+          m[j].x = xj[c0];
+          m[j].y = xj[c1];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j < kb) {
+        const double* xj = Xj + static_cast<int64_t>(j) * ldx;
+        first[j] = xj[cf];  // wave-uniform: scalar loads
+        after[j] = xj[ce];
+      }
+    }
+    // rows past the end compute sums that are never stored
+    bool oka[3], okb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t ca = row32 + static_cast<uint32_t>(off[k]), cb = ca + 1u;
+      oka[k] = (o0 + k < cA) & (v[k].x != 0.0) & (ca < n32);
+      okb[k] = (o0 + k < cA) & (v[k].y != 0.0) & (cb < n32);
+    }
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j >= kb) break;
+      const double xa[3] = {wave_shr1(m[j].y, first[j]), m[j].x, m[j].y};
+      const double xb[3] = {m[j].x, m[j].y, wave_shl1(m[j].x, after[j])};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sa[j] = masked_fma(oka[k], v[k].x, xa[k], sa[j]);
+        sb[j] = masked_fma(okb[k], v[k].y, xb[k], sb[j]);
+      }
+    }
+  }
+  if constexpr (!HAS_COO) {
+    // both rows of a lane are adjacent in every column: one 16-byte store
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j >= kb) break;
+      double* yp = Yj + static_cast<int64_t>(j) * ldy + row;
+      if (act_b) {
+        dbl2a8 t;
+        t.x = sa[j];
+        t.y = sb[j];
+        __builtin_nontemporal_store(t, reinterpret_cast<dbl2a8*>(yp));
+      } else if (act_a) {
+        __builtin_nontemporal_store(sa[j], yp);
+      }
+    }
+  } else {
+    // back to the halves' row-per-lane order, then each half's fallback sums
+    double yA[KB], yB[KB];
+    double* sy = s_y[wv];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j >= kb) break;
+      sy[2 * lane] = sa[j];
+      sy[2 * lane + 1] = sb[j];
+      wave_sync();
+      yA[j] = sy[lane];
+      yB[j] = sy[kWave + lane];
+      wave_sync();
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && !hasB) break;
+      const int64_t g = 2 * w2 + h;
+      const int64_t r0 = row0 + h * kWave;
+      const int p0 = gptr[g], p1 = gptr[g + 1];
+      double cs[KB];
+      if (p0 < p1) {
+        coo_group_sum_multi<double, KB>(cr, cc, cv, Xj, ldx, kb, p0, p1, r0, lane, s_v[wv], s_x[wv], s_lo[wv],
+                                        s_hi[wv], cs);
+      } else {
+#pragma unroll
+        for (int j = 0; j < KB; ++j) cs[j] = 0.0;
+      }
+      if (r0 + lane < nrows) {
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          if (j >= kb) break;
+          double* yp = Yj + static_cast<int64_t>(j) * ldy + r0 + lane;
+          __builtin_nontemporal_store((h ? yB[j] : yA[j]) + cs[j], yp);
+        }
       }
     }
   }
@@ -2098,6 +2301,19 @@ void launch_multi_kb(const T* vals, const int32_t* offs, const int32_t* blk, int
 #undef HBDIA_LAUNCH_M
 }
 
+template <int KB>
+void launch_multi_row2_kb(const double* vals, const int32_t* offs, const int32_t* blk, int32_t nb, int32_t bw,
+                          const int32_t* cr, const int32_t* cc, const double* cv, const int32_t* gptr,
+                          const double* X, int64_t ldx, double* Y, int64_t ldy, int nrhs, int64_t nrows,
+                          bool with_coo, bool x32, const uint32_t* runs3, int kshift, dim3 grid, hipStream_t s) {
+#define HBDIA_LAUNCH_M2(C, XX)                                                                                \
+  hipLaunchKernelGGL((k_hbdia_multi_row2<KB, C, XX>), grid, dim3(kWG), 0, s, vals, offs, blk, X, ldx, Y, ldy, \
+                     nrhs, nrows, nb, bw, cr, cc, cv, gptr, runs3, kshift)
+  if (with_coo) { if (x32) HBDIA_LAUNCH_M2(true, true); else HBDIA_LAUNCH_M2(true, false); }
+  else { if (x32) HBDIA_LAUNCH_M2(false, true); else HBDIA_LAUNCH_M2(false, false); }
+#undef HBDIA_LAUNCH_M2
+}
+
 template <typename T>
 int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t nb, int32_t bw,
                  const int32_t* cr, const int32_t* cc, const T* cv, const int32_t* gptr, const T* X, int64_t ldx,
@@ -2125,8 +2341,25 @@ int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   const int kb = nrhs >= 8 ? 8 : (nrhs >= 4 ? 4 : (nrhs >= 2 ? 2 : 1));
   const int64_t items = wgs * ((nrhs + kb - 1) / kb);
   if (items > 0x7fffffffLL) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": too many rows x columns");
-  const dim3 grid(static_cast<unsigned>(items));
   hipStream_t s = hs(stream);
+  if constexpr (std::is_same<T, double>::value) {
+    if (runs3) {
+      // 128-row waves, two rows per lane, at most 4 columns per pass (8 need
+      // about 128 VGPRs and spill scalars: slower); a row block's column passes
+      // are adjacent work items, so the later ones read its values from L2
+      const int kb2 = kb > 4 ? 4 : kb;
+      const int64_t items2 = (nrows + 2 * kWG - 1) / (2 * kWG) * ((nrhs + kb2 - 1) / kb2);
+      const dim3 grid2(static_cast<unsigned>(items2));
+      const int ks2 = xcd_chunk_shift(items2);
+      switch (kb2) {
+        case 4: launch_multi_row2_kb<4>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks2, grid2, s); break;
+        case 2: launch_multi_row2_kb<2>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks2, grid2, s); break;
+        default: launch_multi_row2_kb<1>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks2, grid2, s); break;
+      }
+      return check_hip(hipGetLastError(), name);
+    }
+  }
+  const dim3 grid(static_cast<unsigned>(items));
   const int ks = xcd_chunk_shift(items);
   switch (kb) {
     case 8: launch_multi_kb<T, 8>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks, grid, s); break;
