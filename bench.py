@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--count", type=int, default=None, help="override the number of batched matrices (testing only)")
     p.add_argument("--grid", type=int, default=None, help="override the grid edge (testing only)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full-size parity check")
+    p.add_argument("--dtype", choices=["f64", "f32"], default="f64",
+                   help="value, x and y type of the c2 and ref128 workloads (HBDIA<double> or HBDIA<float>)")
     p.add_argument("--cpu-reps", type=int, default=5)
     return p.parse_args()
 
@@ -87,23 +89,25 @@ def main() -> int:
     log(f"device {hbdia_amd.device_name()}, host threads {hbdia_amd.api.capi.hbdia_get_num_threads()}")
 
     t0 = time.time()
+    dt = np.float32 if args.dtype == "f32" else np.float64
+    s_v = np.dtype(dt).itemsize
     if args.workload == "c2":
         n = args.grid or 256
         bw, thr, seed = 64, 2, 0xC2
-        A = HBDIA.stencil(n, 27, 0.005, seed=seed)
+        A = HBDIA.stencil(n, 27, 0.005, seed=seed, dtype=dt)
         log(f"generated {n}^3 stencil + scatter in {time.time() - t0:.1f}s")
         N = n ** 3
-        x = gen_vector(N, seed + 1)
+        x = gen_vector(N, seed + 1).astype(dt)
         workload = {"workload": f"stencil27_scatter_{n}^3", "grid": n, "rows": N, "scatter": 0.005,
                     "block_width": bw, "threshold": thr, "seed": seed, "coo_exec": "gpu_fused"}
     else:
         n = args.grid or 128
         bw, thr = 32, 8
-        A = HBDIA()
+        A = HBDIA(dtype=dt)
         A.create3DStencil27Point(n, n, n, 0.0)
         log(f"generated {n}^3 reference stencil in {time.time() - t0:.1f}s")
         N = n ** 3
-        x = np.arange(N, dtype=np.float64)
+        x = np.arange(N, dtype=dt)
         workload = {"workload": f"reference_stencil27_{n}^3", "grid": n, "rows": N, "noise": 0.0,
                     "block_width": bw, "threshold": thr, "coo_exec": "gpu_fused"}
     t1 = time.time()
@@ -116,11 +120,11 @@ def main() -> int:
     nnz_coo = info["num_coo"]
     nnz_bdia = nnz - nnz_coo
     flop = 2.0 * nnz
-    alg_bytes = nnz_bdia * 8 + nnz_coo * (8 + 4 + 4) + N * 8 + N * 8
-    stored_bytes = info["stored_bytes"] + 2 * N * 8
+    alg_bytes = nnz_bdia * s_v + nnz_coo * (s_v + 4 + 4) + N * s_v + N * s_v
+    stored_bytes = info["stored_bytes"] + 2 * N * s_v
 
     X = HBDIAVector(x)
-    Y = HBDIAVector(np.zeros(N))
+    Y = HBDIAVector(np.zeros(N, dt))
     use_cpu = False
     gpu_coo = nnz_coo > 0
 
@@ -138,8 +142,8 @@ def main() -> int:
         hb = A.getHBDIA()
         hb["num_blocks"], hb["block_width"] = A.getNumBlocks(), bw
         cores = oracle.max_threads()
-        y_cpu = np.zeros(N)
-        coo_buf = np.zeros(N)
+        y_cpu = np.zeros(N, dt)
+        coo_buf = np.zeros(N, dt)
         oracle.hbdia_spmv(hb, x, y_cpu, N, coo_buf)
         diff = y_gpu != y_cpu
         rel = np.abs(y_gpu - y_cpu) / np.maximum(np.abs(y_cpu), 1e-300)
@@ -186,7 +190,7 @@ def main() -> int:
         f"{achieved:.1f} GB/s algorithmic")
 
     traffic = None
-    if PROFILE_TRAFFIC.exists() and args.workload == "c2" and (args.grid or 256) == 256:
+    if PROFILE_TRAFFIC.exists() and args.workload == "c2" and (args.grid or 256) == 256 and args.dtype == "f64":
         try:
             traffic = json.loads(PROFILE_TRAFFIC.read_text()).get("hbm_bytes_per_launch")
         except Exception:
@@ -194,10 +198,10 @@ def main() -> int:
 
     value = flop / (wall_ms * 1e-3) / 1e9
     vs = None
-    if args.workload == "ref128" and (args.grid or 128) == 128:
+    if args.workload == "ref128" and (args.grid or 128) == 128 and args.dtype == "f64":
         vs = round(value / (flop / 0.40e-3 / 1e9), 3)  # reference figure: 0.40 ms on a GH200-class GPU
     line = {
-        "metric": "SpMV GFLOP/s (FP64 hbdiaSpMV) + % HBM roofline",
+        "metric": f"SpMV GFLOP/s ({args.dtype.upper().replace('F', 'FP')} hbdiaSpMV) + % HBM roofline",
         "value": round(value, 2),
         "unit": "GFLOP/s",
         "n_gpus": 1,
@@ -207,7 +211,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": vs,
-        "dtype": "f64",
+        "dtype": args.dtype,
         "data": "synthetic (seeded generator)",
         "config": workload,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

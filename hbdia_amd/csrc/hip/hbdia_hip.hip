@@ -552,38 +552,45 @@ __global__ __launch_bounds__(kWG) void k_hbdia(
 // 16-byte accesses.  Measured at 256^3 (no fallback): 601 vs 620 us.
 
 typedef double dbl2a8 __attribute__((ext_vector_type(2), aligned(8)));
+// Two adjacent elements as one access: 16 bytes for FP64, 8 for FP32; the
+// alignment of one element, for loads at any x index.
+typedef float flt2a4 __attribute__((ext_vector_type(2), aligned(4)));
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef dbl2a8 type; };
+template <> struct Vec2<float> { typedef flt2a4 type; };
 
 // Rows row and row + 1 (row = row0 + 2 lane) of a 128-row wave whose rows share
 // one offset list of runs of three.  vb + vlane addresses the lane's two values
 // of the list's first offset; offset o of them is at vb + vlane + o * bw.  The x
 // values, skips and FMA order per row are exactly those of bdia_sum_runs3.
-template <int CH>
-__device__ __forceinline__ void bdia_sum_runs3_row2(const double* __restrict__ vb, int vlane,
+template <typename T, int CH>
+__device__ __forceinline__ void bdia_sum_runs3_row2(const T* __restrict__ vb, int vlane,
                                                     const int* __restrict__ op, int cnt, int bw,
-                                                    const double* __restrict__ x, int64_t row0, int64_t row,
-                                                    int64_t nrows, bool act_a, bool act_b, double& sa,
-                                                    double& sb) {
+                                                    const T* __restrict__ x, int64_t row0, int64_t row,
+                                                    int64_t nrows, bool act_a, bool act_b, T& sa,
+                                                    T& sb) {
+  using V2 = typename Vec2<T>::type;
   const int64_t last = nrows - 1;
-  sa = 0.0;
-  sb = 0.0;
+  sa = T(0);
+  sb = T(0);
   for (int o0 = 0; o0 < cnt; o0 += CH) {
     int off[CH];
 #pragma unroll
     for (int k = 0; k < CH; ++k) off[k] = op[o0 + k];  // padded array
-    dbl2a8 v[CH];
+    V2 v[CH];
 #pragma unroll
     for (int k = 0; k < CH; ++k)
       v[k] = __builtin_nontemporal_load(
-          reinterpret_cast<const dbl2a8*>(vb + static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + vlane)));
+          reinterpret_cast<const V2*>(vb + static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + vlane)));
     // every run's centre pair (one 16-byte load) and edge values, before any FMA
-    dbl2a8 m[CH / 3];
-    double first[CH / 3], after[CH / 3];
+    V2 m[CH / 3];
+    T first[CH / 3], after[CH / 3];
 #pragma unroll
     for (int w = 0; w < CH / 3; ++w) {
       // runs past the block's count read padding: clamped, their products unused
       const int64_t o = off[3 * w + 1];
       if (row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last) {  // wave-uniform: the window lies inside x
-        m[w] = *reinterpret_cast<const dbl2a8*>(x + (row + o));
+        m[w] = *reinterpret_cast<const V2*>(x + (row + o));
       } else {
         int64_t c0 = row + o, c1 = row + 1 + o;
         c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
@@ -604,16 +611,16 @@ __device__ __forceinline__ void bdia_sum_runs3_row2(const double* __restrict__ v
     // the FMAs run by run, in offset order
 #pragma unroll
     for (int w = 0; w < CH / 3; ++w) {
-      const double left = wave_shr1(m[w].y, first[w]);   // x[row + o - 1]
-      const double right = wave_shl1(m[w].x, after[w]);  // x[row + o + 2]
-      const double xa[3] = {left, m[w].x, m[w].y};
-      const double xb[3] = {m[w].x, m[w].y, right};
+      const T left = wave_shr1(m[w].y, first[w]);   // x[row + o - 1]
+      const T right = wave_shl1(m[w].x, after[w]);  // x[row + o + 2]
+      const T xa[3] = {left, m[w].x, m[w].y};
+      const T xb[3] = {m[w].x, m[w].y, right};
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const int k = 3 * w + j;
         const int64_t ca = row + off[k], cb = ca + 1;
-        const bool oka = (o0 + k < cnt) & act_a & (v[k].x != 0.0) & (ca >= 0) & (ca < nrows);
-        const bool okb = (o0 + k < cnt) & act_b & (v[k].y != 0.0) & (cb >= 0) & (cb < nrows);
+        const bool oka = (o0 + k < cnt) & act_a & (v[k].x != T(0)) & (ca >= 0) & (ca < nrows);
+        const bool okb = (o0 + k < cnt) & act_b & (v[k].y != T(0)) & (cb >= 0) & (cb < nrows);
         sa = masked_fma(oka, v[k].x, xa[j], sa);
         sb = masked_fma(okb, v[k].y, xb[j], sb);
       }
@@ -623,26 +630,30 @@ __device__ __forceinline__ void bdia_sum_runs3_row2(const double* __restrict__ v
 
 // One nine-offset chunk of the two-row runs-of-three sums, split into its
 // loads and its FMAs so that the next chunk's loads can be issued first.
+template <typename T>
 struct Row2Chunk {
-  dbl2a8 v[kChunk];
-  dbl2a8 m[kChunk / 3];
-  double first[kChunk / 3], after[kChunk / 3];
+  using V2 = typename Vec2<T>::type;
+  V2 v[kChunk];
+  V2 m[kChunk / 3];
+  T first[kChunk / 3], after[kChunk / 3];
 };
 
-__device__ __forceinline__ void row2_chunk_load(Row2Chunk& q, const double* __restrict__ vb, int vlane,
+template <typename T>
+__device__ __forceinline__ void row2_chunk_load(Row2Chunk<T>& q, const T* __restrict__ vb, int vlane,
                                                 const int* off, int o0, int cnt, int bw,
-                                                const double* __restrict__ x, int64_t row0, int64_t row,
+                                                const T* __restrict__ x, int64_t row0, int64_t row,
                                                 int64_t last) {
+  using V2 = typename Vec2<T>::type;
 #pragma unroll
   for (int k = 0; k < kChunk; ++k)
     q.v[k] = __builtin_nontemporal_load(
-        reinterpret_cast<const dbl2a8*>(vb + static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + vlane)));
+        reinterpret_cast<const V2*>(vb + static_cast<uint32_t>(min(o0 + k, cnt - 1) * bw + vlane)));
 #pragma unroll
   for (int w = 0; w < kChunk / 3; ++w) {
     // runs past the block's count read padding: clamped, their products unused
     const int64_t o = off[o0 + 3 * w + 1];
     if (row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last) {  // wave-uniform: the window lies inside x
-      q.m[w] = *reinterpret_cast<const dbl2a8*>(x + (row + o));
+      q.m[w] = *reinterpret_cast<const V2*>(x + (row + o));
     } else {
       int64_t c0 = row + o, c1 = row + 1 + o;
       c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
@@ -664,21 +675,22 @@ __device__ __forceinline__ void row2_chunk_load(Row2Chunk& q, const double* __re
 
 // Row and column indices are below 2^31, so row + offset taken modulo 2^32
 // lies in [0, nrows) exactly when the column does: one unsigned compare.
-__device__ __forceinline__ void row2_chunk_fma(const Row2Chunk& q, const int* off, int o0, int cnt, int64_t row,
-                                               int64_t nrows, bool act_a, bool act_b, double& sa, double& sb) {
+template <typename T>
+__device__ __forceinline__ void row2_chunk_fma(const Row2Chunk<T>& q, const int* off, int o0, int cnt, int64_t row,
+                                               int64_t nrows, bool act_a, bool act_b, T& sa, T& sb) {
   const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
 #pragma unroll
   for (int w = 0; w < kChunk / 3; ++w) {
-    const double left = wave_shr1(q.m[w].y, q.first[w]);   // x[row + o - 1]
-    const double right = wave_shl1(q.m[w].x, q.after[w]);  // x[row + o + 2]
-    const double xa[3] = {left, q.m[w].x, q.m[w].y};
-    const double xb[3] = {q.m[w].x, q.m[w].y, right};
+    const T left = wave_shr1(q.m[w].y, q.first[w]);   // x[row + o - 1]
+    const T right = wave_shl1(q.m[w].x, q.after[w]);  // x[row + o + 2]
+    const T xa[3] = {left, q.m[w].x, q.m[w].y};
+    const T xb[3] = {q.m[w].x, q.m[w].y, right};
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const int k = 3 * w + j;
       const uint32_t ca = row32 + static_cast<uint32_t>(off[o0 + k]), cb = ca + 1u;
-      const bool oka = (o0 + k < cnt) & act_a & (q.v[k].x != 0.0) & (ca < n32);
-      const bool okb = (o0 + k < cnt) & act_b & (q.v[k].y != 0.0) & (cb < n32);
+      const bool oka = (o0 + k < cnt) & act_a & (q.v[k].x != T(0)) & (ca < n32);
+      const bool okb = (o0 + k < cnt) & act_b & (q.v[k].y != T(0)) & (cb < n32);
       sa = masked_fma(oka, q.v[k].x, xa[j], sa);
       sb = masked_fma(okb, q.v[k].y, xb[j], sb);
     }
@@ -689,18 +701,19 @@ __device__ __forceinline__ void row2_chunk_fma(const Row2Chunk& q, const int* of
 // unrolled, each chunk's loads issued before the previous chunk's FMAs, so a
 // wave keeps the next nine value lines and x windows in flight while it
 // computes.  Same arithmetic and order.
-__device__ __forceinline__ void bdia_sum_runs3_row2_27(const double* __restrict__ vb, int vlane,
+template <typename T>
+__device__ __forceinline__ void bdia_sum_runs3_row2_27(const T* __restrict__ vb, int vlane,
                                                        const int* __restrict__ op, int cnt, int bw,
-                                                       const double* __restrict__ x, int64_t row0, int64_t row,
-                                                       int64_t nrows, bool act_a, bool act_b, double& sa,
-                                                       double& sb) {
+                                                       const T* __restrict__ x, int64_t row0, int64_t row,
+                                                       int64_t nrows, bool act_a, bool act_b, T& sa,
+                                                       T& sb) {
   const int64_t last = nrows - 1;
   int off[3 * kChunk];
 #pragma unroll
   for (int k = 0; k < 3 * kChunk; ++k) off[k] = op[k];  // padded array
-  sa = 0.0;
-  sb = 0.0;
-  Row2Chunk c0, c1;
+  sa = T(0);
+  sb = T(0);
+  Row2Chunk<T> c0, c1;
   row2_chunk_load(c0, vb, vlane, off, 0, cnt, bw, x, row0, row, last);
   if (cnt > kChunk) row2_chunk_load(c1, vb, vlane, off, kChunk, cnt, bw, x, row0, row, last);
   row2_chunk_fma(c0, off, 0, cnt, row, nrows, act_a, act_b, sa, sb);
@@ -710,37 +723,37 @@ __device__ __forceinline__ void bdia_sum_runs3_row2_27(const double* __restrict_
 }
 
 // The blocked part of one 64-row half (rows r0 + lane) of a block width that is
-// a multiple of 64: k_hbdia<double, true, ...>'s arithmetic.
-template <bool X32>
-__device__ __forceinline__ double half_bdia(const double* __restrict__ vals, const int* __restrict__ offs,
-                                            const double* __restrict__ x, int64_t r0, int64_t b, int start,
+// a multiple of 64: k_hbdia<T, true, ...>'s arithmetic.
+template <typename T, bool X32>
+__device__ __forceinline__ T half_bdia(const T* __restrict__ vals, const int* __restrict__ offs,
+                                            const T* __restrict__ x, int64_t r0, int64_t b, int start,
                                             int cnt, int bw, int lane, int64_t nrows, bool runs) {
   const int64_t row = r0 + lane;
   const bool active = row < nrows;
-  const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(double)) < (1ll << 32);
+  const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(T)) < (1ll << 32);
   if (cnt > 0 && small) {
-    const double* vb = vals + static_cast<int64_t>(start) * bw + (r0 - b * bw);
-    if (runs) return bdia_sum_runs3<double, X32>(vb, offs + start, cnt, bw, lane, x, r0, row, nrows, active);
-    return bdia_sum_uniform<double, X32>(vb, offs + start, cnt, bw, lane, x, row, nrows, active);
+    const T* vb = vals + static_cast<int64_t>(start) * bw + (r0 - b * bw);
+    if (runs) return bdia_sum_runs3<T, X32>(vb, offs + start, cnt, bw, lane, x, r0, row, nrows, active);
+    return bdia_sum_uniform<T, X32>(vb, offs + start, cnt, bw, lane, x, row, nrows, active);
   }
   if (cnt > 0 && active) {
-    const double* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
-    return bdia_sum_lane<double, X32>(vp, offs + start, cnt, bw, x, row, nrows);
+    const T* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
+    return bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
   }
   return 0.0;
 }
 
-template <bool HAS_COO, bool X32>
+template <typename T, bool HAS_COO, bool X32>
 __global__ __launch_bounds__(kWG) void k_hbdia_row2(
-    const double* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
-    const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb, int bw,
-    const int* __restrict__ cr, const int* __restrict__ cc, const double* __restrict__ cv,
+    const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const T* __restrict__ x, T* __restrict__ y, int64_t nrows, int nb, int bw,
+    const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
     const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
-  __shared__ double s_v[kWavesPerWG][kWave];
-  __shared__ double s_x[kWavesPerWG][kWave];
+  __shared__ T s_v[kWavesPerWG][kWave];
+  __shared__ T s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][2 * kWave];
   __shared__ int s_hi[kWavesPerWG][2 * kWave];
-  __shared__ double s_y[kWavesPerWG][2 * kWave];
+  __shared__ T s_y[kWavesPerWG][2 * kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned work = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
@@ -772,20 +785,20 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
       same = __ballot(oa != ob) == 0;
     }
   }
-  double yA = 0.0, yB = 0.0;
+  T yA = 0.0, yB = T(0);
   if (same) {
     const int64_t row = row0 + 2 * lane;
-    const double* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
+    const T* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
     // lanes of half B read block bB's lines, which follow block bA's
     const int vlane = 2 * lane + ((lane >= kWave / 2 && bB != bA) ? (cA - 1) * bw : 0);
-    double sa, sb;
+    T sa, sb;
     if (cA <= 3 * kChunk)
       bdia_sum_runs3_row2_27(vb, vlane, offs + sA, cA, bw, x, row0, row, nrows, row < nrows, row + 1 < nrows, sa, sb);
     else
-      bdia_sum_runs3_row2<kRow2Chunk>(vb, vlane, offs + sA, cA, bw, x, row0, row, nrows, row < nrows,
+      bdia_sum_runs3_row2<T, kRow2Chunk>(vb, vlane, offs + sA, cA, bw, x, row0, row, nrows, row < nrows,
                                       row + 1 < nrows, sa, sb);
     // back to the halves' row-per-lane order
-    double* sy = s_y[wv];
+    T* sy = s_y[wv];
     sy[2 * lane] = sa;
     sy[2 * lane + 1] = sb;
     wave_sync();
@@ -793,23 +806,23 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     yB = sy[kWave + lane];
     wave_sync();
   } else {
-    yA = half_bdia<X32>(vals, offs, x, row0, bA, sA, cA, bw, lane, nrows, runsA);
-    if (hasB) yB = half_bdia<X32>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB);
+    yA = half_bdia<T, X32>(vals, offs, x, row0, bA, sA, cA, bw, lane, nrows, runsA);
+    if (hasB) yB = half_bdia<T, X32>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB);
   }
   // fallback sums of both 64-row groups: one pass when their entries fit one
-  double csA = 0.0, csB = 0.0;
+  T csA = 0.0, csB = T(0);
   if constexpr (HAS_COO) {
     const int p0 = gptr[2 * w2], p1 = gptr[2 * w2 + 1];
     const int p2 = hasB ? gptr[2 * w2 + 2] : p1;
     bool any = false;
     if (p2 - p0 <= kWave) {
-      if (p0 < p2) coo_pair_sum<double>(cr, cc, cv, x, p0, p1, p2, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
+      if (p0 < p2) coo_pair_sum<T>(cr, cc, cv, x, p0, p1, p2, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
                                         csA, csB);
     } else {
       if (p0 < p1)
-        csA = coo_group_sum<double>(cr, cc, cv, x, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
+        csA = coo_group_sum<T>(cr, cc, cv, x, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
       if (p1 < p2)
-        csB = coo_group_sum<double>(cr, cc, cv, x, p1, p2, row0 + kWave, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
+        csB = coo_group_sum<T>(cr, cc, cv, x, p1, p2, row0 + kWave, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
                                     &any);
     }
   }
@@ -820,11 +833,11 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     const int64_t row = r0 + lane;
     const bool active = row < nrows;
     const int cnt = h ? cB : cA;
-    const double sum = h ? yB : yA;
+    const T sum = h ? yB : yA;
     if constexpr (HAS_COO) {
-      const double cs = h ? csB : csA;
+      const T cs = h ? csB : csA;
       if (active) {
-        const double base = cnt > 0 ? sum : y[row];
+        const T base = cnt > 0 ? sum : y[row];
         __builtin_nontemporal_store(base + cs, y + row);
       }
     } else {
@@ -840,18 +853,18 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
 // lie in block b0 + l / 16, whose lines follow block b0's by (l / 16) * cnt
 // lines: the runs-of-three two-row sums run with that per-lane value index.
 // Otherwise each 64-row half takes k_hbdia's width-32 path.  Values, skips and
-// FMA order per row are those of k_hbdia<double, false, ...>.
-template <bool HAS_COO, bool X32>
+// FMA order per row are those of k_hbdia<T, false, ...>.
+template <typename T, bool HAS_COO, bool X32>
 __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
-    const double* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
-    const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb,
-    const int* __restrict__ cr, const int* __restrict__ cc, const double* __restrict__ cv,
+    const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const T* __restrict__ x, T* __restrict__ y, int64_t nrows, int nb,
+    const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
     const int* __restrict__ gptr, const uint32_t* __restrict__ runs3_blocks, int kshift) {
-  __shared__ double s_v[kWavesPerWG][kWave];
-  __shared__ double s_x[kWavesPerWG][kWave];
+  __shared__ T s_v[kWavesPerWG][kWave];
+  __shared__ T s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][2 * kWave];
   __shared__ int s_hi[kWavesPerWG][2 * kWave];
-  __shared__ double s_y[kWavesPerWG][2 * kWave];
+  __shared__ T s_y[kWavesPerWG][2 * kWave];
   constexpr int bw = 32;
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -875,19 +888,19 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
       quad = __ballot(diff) == 0;
     }
   }
-  double yA = 0.0, yB = 0.0;
+  T yA = 0.0, yB = T(0);
   int cntA = 0, cntB = 0;  // the lane's block's offset count, per half
   if (quad) {
     const int64_t row = row0 + 2 * lane;
-    const double* vb = vals + static_cast<int64_t>(s0) * bw;
+    const T* vb = vals + static_cast<int64_t>(s0) * bw;
     const int vlane = 2 * lane + (lane >> 4) * (c - 1) * bw;
-    double sa, sb;
+    T sa, sb;
     if (c <= 3 * kChunk)
       bdia_sum_runs3_row2_27(vb, vlane, offs + s0, c, bw, x, row0, row, nrows, row < nrows, row + 1 < nrows, sa, sb);
     else
-      bdia_sum_runs3_row2<kRow2Chunk>(vb, vlane, offs + s0, c, bw, x, row0, row, nrows, row < nrows,
+      bdia_sum_runs3_row2<T, kRow2Chunk>(vb, vlane, offs + s0, c, bw, x, row0, row, nrows, row < nrows,
                                       row + 1 < nrows, sa, sb);
-    double* sy = s_y[wv];
+    T* sy = s_y[wv];
     sy[2 * lane] = sa;
     sy[2 * lane + 1] = sb;
     wave_sync();
@@ -908,8 +921,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
         start = blk[b];
         cnt = blk[b + 1] - start;
       }
-      const double sum =
-          group_bdia_nonuniform<double, X32, true>(vals, offs, blk, x, r0, lane, nrows, nb, bw, b, start, cnt,
+      const T sum =
+          group_bdia_nonuniform<T, X32, true>(vals, offs, blk, x, r0, lane, nrows, nb, bw, b, start, cnt,
                                                    runs3_blocks);
       if (h == 0) {
         yA = sum;
@@ -921,19 +934,19 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
     }
   }
   // fallback sums of both 64-row groups: one pass when their entries fit one
-  double csA = 0.0, csB = 0.0;
+  T csA = 0.0, csB = T(0);
   if constexpr (HAS_COO) {
     const int p0 = gptr[2 * w2], p1 = gptr[2 * w2 + 1];
     const int p2 = hasB ? gptr[2 * w2 + 2] : p1;
     bool any = false;
     if (p2 - p0 <= kWave) {
-      if (p0 < p2) coo_pair_sum<double>(cr, cc, cv, x, p0, p1, p2, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
+      if (p0 < p2) coo_pair_sum<T>(cr, cc, cv, x, p0, p1, p2, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
                                         csA, csB);
     } else {
       if (p0 < p1)
-        csA = coo_group_sum<double>(cr, cc, cv, x, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
+        csA = coo_group_sum<T>(cr, cc, cv, x, p0, p1, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], &any);
       if (p1 < p2)
-        csB = coo_group_sum<double>(cr, cc, cv, x, p1, p2, row0 + kWave, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
+        csB = coo_group_sum<T>(cr, cc, cv, x, p1, p2, row0 + kWave, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv],
                                     &any);
     }
   }
@@ -944,11 +957,11 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
     const int64_t row = r0 + lane;
     const bool active = row < nrows;
     const int cnt = h ? cntB : cntA;
-    const double sum = h ? yB : yA;
+    const T sum = h ? yB : yA;
     if constexpr (HAS_COO) {
-      const double cs = h ? csB : csA;
+      const T cs = h ? csB : csA;
       if (active) {
-        const double base = cnt > 0 ? sum : y[row];
+        const T base = cnt > 0 ? sum : y[row];
         __builtin_nontemporal_store(base + cs, y + row);
       }
     } else {
@@ -2235,14 +2248,14 @@ int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   hipStream_t s = hs(stream);
   const bool runs3 = (uniform || bw == 32) && runs3_blocks != nullptr;
   const int kshift = xcd_chunk_shift(wgs);
-  if constexpr (std::is_same<T, double>::value) {
+  {
     if (runs3 && bw == 32 && nrows <= 0x7fffffffLL) {
       // 128-row waves over four width-32 blocks, two rows per lane
       const int64_t wgs2 = (nrows + 2 * kWG - 1) / (2 * kWG);
       const dim3 grid2(static_cast<unsigned>(wgs2));
       const int ks2 = xcd_chunk_shift(wgs2);
 #define HBDIA_LAUNCH_W32(C, X)                                                                                 \
-  hipLaunchKernelGGL((k_hbdia_row2_w32<C, X>), grid2, block, 0, s, vals, offs, blk, x, y, nrows, nb, cr, cc, cv, \
+  hipLaunchKernelGGL((k_hbdia_row2_w32<T, C, X>), grid2, block, 0, s, vals, offs, blk, x, y, nrows, nb, cr, cc, cv, \
                      gptr, runs3_blocks, ks2)
       if (with_coo) { if (x32) HBDIA_LAUNCH_W32(true, true); else HBDIA_LAUNCH_W32(true, false); }
       else { if (x32) HBDIA_LAUNCH_W32(false, true); else HBDIA_LAUNCH_W32(false, false); }
@@ -2255,7 +2268,7 @@ int launch_hbdia(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
       const dim3 grid2(static_cast<unsigned>(wgs2));
       const int ks2 = xcd_chunk_shift(wgs2);
 #define HBDIA_LAUNCH_R2(C, X)                                                                                  \
-  hipLaunchKernelGGL((k_hbdia_row2<C, X>), grid2, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, \
+  hipLaunchKernelGGL((k_hbdia_row2<T, C, X>), grid2, block, 0, s, vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, \
                      gptr, runs3_blocks, ks2)
       if (with_coo) { if (x32) HBDIA_LAUNCH_R2(true, true); else HBDIA_LAUNCH_R2(true, false); }
       else { if (x32) HBDIA_LAUNCH_R2(false, true); else HBDIA_LAUNCH_R2(false, false); }

@@ -294,14 +294,21 @@ def _runs3_case(N, bw, seed, with_coo):
     return values, offsets, block_start, x, coo
 
 
-@pytest.mark.parametrize("N,bw,with_coo", [(1000, 64, False), (1000, 64, True), (4099, 128, True), (64, 64, False),
-                                           (777, 256, True)])
-def test_runs3_hint_is_bit_identical(gpu, N, bw, with_coo):
-    print(f"case: runs3 N={N} bw={bw} coo={with_coo}", flush=True)
+@pytest.mark.parametrize("N,bw,with_coo,dtype", [(1000, 64, False, np.float64), (1000, 64, True, np.float64),
+                                                 (4099, 128, True, np.float64), (64, 64, False, np.float64),
+                                                 (777, 256, True, np.float64), (1000, 64, True, np.float32),
+                                                 (4099, 128, False, np.float32), (777, 256, True, np.float32)])
+def test_runs3_hint_is_bit_identical(gpu, N, bw, with_coo, dtype):
+    """The runs-of-three paths (the two-row waves included, FP64 and FP32) equal the hint-free launch
+    and the oracle bit for bit."""
+    print(f"case: runs3 N={N} bw={bw} coo={with_coo} {np.dtype(dtype).name}", flush=True)
     values, offsets, block_start, x, coo = _runs3_case(N, bw, 31 + N + bw, with_coo)
+    values, x = values.astype(dtype), x.astype(dtype)
+    if coo is not None:
+        coo = (coo[0], coo[1], coo[2].astype(dtype))
     L = _layout(values, offsets, block_start, N, bw, coo)
     assert L.runs3 is not None
-    y0 = np.linspace(-1.0, 1.0, N)
+    y0 = np.linspace(-1.0, 1.0, N).astype(dtype)
     X = DeviceArray.from_host(x)
     outs = []
     for hints in (False, True):
@@ -315,7 +322,7 @@ def test_runs3_hint_is_bit_identical(gpu, N, bw, with_coo):
     if coo is not None:
         hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
     else:
-        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0))
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0, values.dtype))
     want = oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N)
     np.testing.assert_array_equal(outs[1], want)
 
@@ -391,7 +398,7 @@ def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k):
     if coo is not None:
         hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
     else:
-        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0))
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0, values.dtype))
     for j in range(k):
         np.testing.assert_array_equal(outs[1][j], oracle.hbdia_spmv(hb, Xh[j].copy(), Y0[j].copy(), N),
                                       err_msg=f"column {j}")
@@ -439,7 +446,8 @@ def _width32_case(N, seed, with_coo, share, group=2, kmax=5):
     # the small cases one quad of runs of three
     (1000, True, 0.9, 4, 5, None), (4096, False, 1.0, 4, 5, None), (4096, True, 1.0, 4, 13, None),
     (777, True, 0.7, 4, 13, None), (130, True, 1.0, 4, 5, 0), (224, False, 1.0, 4, 5, 0), (128, True, 1.0, 4, 5, 0)])
-def test_width32_shared_offsets_are_bit_identical(gpu, N, with_coo, share, group, kmax, seed):
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_width32_shared_offsets_are_bit_identical(gpu, N, with_coo, share, group, kmax, seed, dtype):
     """Width-32 waves whose blocks share their offsets run the uniform sums (with the runs-of-three
     hint where it applies; four blocks with one list run two rows per lane): results equal the
     hint-free launch and the oracle."""
@@ -447,8 +455,11 @@ def test_width32_shared_offsets_are_bit_identical(gpu, N, with_coo, share, group
     if seed is None:
         seed = 90 + N + (0 if group == 2 else 7 * group + kmax)
     values, offsets, block_start, x, coo = _width32_case(N, seed, with_coo, share, group, kmax)
+    values, x = values.astype(dtype), x.astype(dtype)
+    if coo is not None:
+        coo = (coo[0], coo[1], coo[2].astype(dtype))
     L = _layout(values, offsets, block_start, N, 32, coo)
-    y0 = np.linspace(-1.0, 1.0, N)
+    y0 = np.linspace(-1.0, 1.0, N).astype(dtype)
     X = DeviceArray.from_host(x)
     outs = []
     for hints in (False, True):
@@ -462,7 +473,7 @@ def test_width32_shared_offsets_are_bit_identical(gpu, N, with_coo, share, group
     if coo is not None:
         hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
     else:
-        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0))
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0, values.dtype))
     np.testing.assert_array_equal(outs[1], oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N))
 
 
