@@ -327,6 +327,47 @@ def test_runs3_hint_is_bit_identical(gpu, N, bw, with_coo, dtype):
     np.testing.assert_array_equal(outs[1], want)
 
 
+@pytest.mark.parametrize("N,cover,bw,with_coo,dtype", [(1000, 700, 64, True, np.float64), (1000, 700, 64, False, np.float64),
+                                                       (4096, 3000, 32, True, np.float64), (2000, 1100, 128, True, np.float64),
+                                                       (1000, 700, 64, True, np.float32), (4096, 3000, 32, False, np.float32)])
+def test_rows_past_the_last_block_keep_y(gpu, N, cover, bw, with_coo, dtype):
+    """Blocks that cover only the first `cover` rows (as for a matrix with fewer columns than rows):
+    on the runs-of-three paths (two-row waves included) the rows past the last block keep y, plus
+    their fallback sums when there is a fallback; equal to the hint-free launch and the oracle."""
+    print(f"case: rows past the last block N={N} cover={cover} bw={bw} coo={with_coo} {np.dtype(dtype).name}",
+          flush=True)
+    values, offsets, block_start, _, _ = _runs3_case(cover, bw, 17 + N + bw, False)
+    rng = np.random.default_rng(N + cover)
+    x = rng.standard_normal(N).astype(dtype)
+    values = values.astype(dtype)
+    coo = None
+    if with_coo:
+        r = np.sort(rng.integers(0, N, N)).astype(np.int32)
+        coo = (r, rng.integers(0, N, N).astype(np.int32), rng.standard_normal(N).astype(dtype))
+    L = _layout(values, offsets, block_start, N, bw, coo)
+    assert L.runs3 is not None
+    y0 = np.linspace(-1.0, 1.0, N).astype(dtype)
+    X = DeviceArray.from_host(x)
+    outs = []
+    for hints in (False, True):
+        Y = DeviceArray.from_host(y0)
+        L.spmv(X, Y, hints=hints)
+        synchronize()
+        outs.append(Y.download())
+    np.testing.assert_array_equal(outs[1], outs[0])
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": len(block_start) - 1,
+          "block_width": bw}
+    if coo is not None:
+        hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
+    else:
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0, dtype))
+    want = oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N)
+    np.testing.assert_array_equal(outs[1], want)
+    nb = len(block_start) - 1
+    if coo is None:
+        np.testing.assert_array_equal(outs[1][nb * bw:], y0[nb * bw:])
+
+
 def _dense_fallback(N, seed):
     """Fallback entries, row-sorted, entry order within a row random: every third 64-row group is
     dense (0-25 entries per row, so far more than 64 per group, some rows empty), the others
