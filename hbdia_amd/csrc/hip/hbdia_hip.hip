@@ -187,7 +187,7 @@ __device__ __forceinline__ T coo_group_sum(const int* __restrict__ cr, const int
 // values and x for both groups, then each lane sums its row of each group in
 // entry order (FMA).  The sums equal coo_group_sum's per group bit for bit.
 // s_lo and s_hi hold 128 entries.
-template <typename T>
+template <typename T, bool ADJ = false>
 __device__ __forceinline__ void coo_pair_sum(const int* __restrict__ cr, const int* __restrict__ cc,
                                              const T* __restrict__ cv, const T* __restrict__ x, int p0, int p1,
                                              int p2, int64_t row0, int lane, T* s_v, T* s_x, int* s_lo, int* s_hi,
@@ -217,9 +217,12 @@ __device__ __forceinline__ void coo_pair_sum(const int* __restrict__ cr, const i
   if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p2 || rn != r)) s_hi[r] = lane + 1;
   wave_sync();
   T a_sum = T(0), b_sum = T(0);
-  const int a0 = s_lo[lane], e0 = s_hi[lane];
+  // csA, csB: rows row0 + lane and row0 + 64 + lane, or with ADJ the two-row
+  // layout's rows row0 + 2 lane and row0 + 2 lane + 1
+  const int ia = ADJ ? 2 * lane : lane, ib = ADJ ? 2 * lane + 1 : kWave + lane;
+  const int a0 = s_lo[ia], e0 = s_hi[ia];
   for (int k = a0; k < e0; ++k) a_sum = fma_t<T>(s_v[k], s_x[k], a_sum);
-  const int a1 = s_lo[kWave + lane], e1 = s_hi[kWave + lane];
+  const int a1 = s_lo[ib], e1 = s_hi[ib];
   for (int k = a1; k < e1; ++k) b_sum = fma_t<T>(s_v[k], s_x[k], b_sum);
   wave_sync();
   csA = a_sum;
@@ -744,7 +747,7 @@ __device__ __forceinline__ T half_bdia(const T* __restrict__ vals, const int* __
 }
 
 template <typename T, bool HAS_COO, bool X32>
-__global__ __launch_bounds__(kWG) void k_hbdia_row2(
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_hbdia_row2(
     const T* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const T* __restrict__ x, T* __restrict__ y, int64_t nrows, int nb, int bw,
     const int* __restrict__ cr, const int* __restrict__ cc, const T* __restrict__ cv,
@@ -785,7 +788,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
       same = __ballot(oa != ob) == 0;
     }
   }
-  T yA = 0.0, yB = T(0);
+  T yA = T(0), yB = T(0);
   if (same) {
     const int64_t row = row0 + 2 * lane;
     const T* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
@@ -797,6 +800,37 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     else
       bdia_sum_runs3_row2<T, kRow2Chunk>(vb, vlane, offs + sA, cA, bw, x, row0, row, nrows, row < nrows,
                                       row + 1 < nrows, sa, sb);
+    // the fallback of both groups in the two-row layout when it fits one pass
+    // (or there is none): each lane stores its two adjacent rows as one access
+    {
+      int q0 = 0, q1 = 0, q2 = 0;
+      if constexpr (HAS_COO) {
+        q0 = gptr[2 * w2];
+        q1 = gptr[2 * w2 + 1];
+        q2 = hasB ? gptr[2 * w2 + 2] : q1;
+      }
+      if (!HAS_COO || q2 - q0 <= kWave) {
+        using V2 = typename Vec2<T>::type;
+        T ta = sa, tb = sb;
+        if constexpr (HAS_COO) {
+          T ca = T(0), cb = T(0);
+          if (q0 < q2)
+            coo_pair_sum<T, true>(cr, cc, cv, x, q0, q1, q2, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], ca,
+                                  cb);
+          ta = sa + ca;  // y = bdia + coo for every row, as below
+          tb = sb + cb;
+        }
+        if (row + 1 < nrows) {
+          V2 t;
+          t.x = ta;
+          t.y = tb;
+          __builtin_nontemporal_store(t, reinterpret_cast<V2*>(y + row));
+        } else if (row < nrows) {
+          __builtin_nontemporal_store(ta, y + row);
+        }
+        return;
+      }
+    }
     // back to the halves' row-per-lane order
     T* sy = s_y[wv];
     sy[2 * lane] = sa;
@@ -810,7 +844,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2(
     if (hasB) yB = half_bdia<T, X32>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB);
   }
   // fallback sums of both 64-row groups: one pass when their entries fit one
-  T csA = 0.0, csB = T(0);
+  T csA = T(0), csB = T(0);
   if constexpr (HAS_COO) {
     const int p0 = gptr[2 * w2], p1 = gptr[2 * w2 + 1];
     const int p2 = hasB ? gptr[2 * w2 + 2] : p1;
@@ -888,7 +922,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
       quad = __ballot(diff) == 0;
     }
   }
-  T yA = 0.0, yB = T(0);
+  T yA = T(0), yB = T(0);
   int cntA = 0, cntB = 0;  // the lane's block's offset count, per half
   if (quad) {
     const int64_t row = row0 + 2 * lane;
@@ -900,6 +934,37 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
     else
       bdia_sum_runs3_row2<T, kRow2Chunk>(vb, vlane, offs + s0, c, bw, x, row0, row, nrows, row < nrows,
                                       row + 1 < nrows, sa, sb);
+    // the fallback of both groups in the two-row layout when it fits one pass
+    // (or there is none): each lane stores its two adjacent rows as one access
+    {
+      int q0 = 0, q1 = 0, q2 = 0;
+      if constexpr (HAS_COO) {
+        q0 = gptr[2 * w2];
+        q1 = gptr[2 * w2 + 1];
+        q2 = hasB ? gptr[2 * w2 + 2] : q1;
+      }
+      if (!HAS_COO || q2 - q0 <= kWave) {
+        using V2 = typename Vec2<T>::type;
+        T ta = sa, tb = sb;
+        if constexpr (HAS_COO) {
+          T ca = T(0), cb = T(0);
+          if (q0 < q2)
+            coo_pair_sum<T, true>(cr, cc, cv, x, q0, q1, q2, row0, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], ca,
+                                  cb);
+          ta = sa + ca;  // y = bdia + coo for every row, as below
+          tb = sb + cb;
+        }
+        if (row + 1 < nrows) {
+          V2 t;
+          t.x = ta;
+          t.y = tb;
+          __builtin_nontemporal_store(t, reinterpret_cast<V2*>(y + row));
+        } else if (row < nrows) {
+          __builtin_nontemporal_store(ta, y + row);
+        }
+        return;
+      }
+    }
     T* sy = s_y[wv];
     sy[2 * lane] = sa;
     sy[2 * lane + 1] = sb;
@@ -934,7 +999,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
     }
   }
   // fallback sums of both 64-row groups: one pass when their entries fit one
-  T csA = 0.0, csB = T(0);
+  T csA = T(0), csB = T(0);
   if constexpr (HAS_COO) {
     const int p0 = gptr[2 * w2], p1 = gptr[2 * w2 + 1];
     const int p2 = hasB ? gptr[2 * w2 + 2] : p1;
