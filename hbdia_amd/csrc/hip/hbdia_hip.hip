@@ -146,11 +146,43 @@ __device__ __forceinline__ T coo_group_sum_direct(const int* __restrict__ cr, co
                                                   const T* __restrict__ cv, const T* __restrict__ x,
                                                   int p0, int p1, int64_t row0, int lane, bool* any) {
   const int target = static_cast<int>(row0) + lane;
-  // lower bound of target in cr[p0, p1): p0 + #entries with row < target
-  int lo = p0;
-  for (int step = 1 << (31 - __builtin_clz(p1 - p0)); step > 0; step >>= 1) {
-    const int probe = lo + step;
-    if (probe <= p1 && cr[probe - 1] < target) lo = probe;
+  // lower bound of target in cr[p0, p1) (the first entry whose row is not below
+  // target, or p1): from a guess that assumes equal entries per row, gallop to
+  // a bracket [lo, bh] that holds it, then bisect.  Rows of equal length (the
+  // usual dense group) find it in one or two probes.  Every probe lies in
+  // [p0, p1).
+  const int n = p1 - p0;
+  const int g = p0 + static_cast<int>((static_cast<int64_t>(lane) * n) >> 6);
+  int lo, bh;
+  if (cr[g] >= target) {
+    bh = g;  // cr[bh] >= target
+    lo = p0;
+    for (int step = 1;; step <<= 1) {
+      const int probe = bh - step;
+      if (probe < p0) break;
+      if (cr[probe] < target) {
+        lo = probe + 1;
+        break;
+      }
+      bh = probe;
+    }
+  } else {
+    lo = g + 1;  // cr[lo - 1] < target
+    bh = p1;
+    for (int step = 1;; step <<= 1) {
+      const int probe = lo + step - 1;
+      if (probe >= p1) break;
+      if (cr[probe] >= target) {
+        bh = probe;
+        break;
+      }
+      lo = probe + 1;
+    }
+  }
+  while (lo < bh) {
+    const int mid = (lo + bh) >> 1;
+    if (cr[mid] < target) lo = mid + 1;
+    else bh = mid;
   }
   // the next lane's lower bound ends this lane's rows; the last lane's is p1
   const int nxt = __shfl_down(lo, 1);
