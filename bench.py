@@ -54,7 +54,7 @@ def parse():
     p.add_argument("--grid", type=int, default=None, help="override the grid edge (testing only)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full-size parity check")
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64",
-                   help="value, x and y type of the c2 and ref128 workloads (HBDIA<double> or HBDIA<float>)")
+                   help="value, x and y type of the c2, ref128 and multirhs workloads (HBDIA<double> or HBDIA<float>)")
     p.add_argument("--cpu-reps", type=int, default=5)
     return p.parse_args()
 
@@ -323,36 +323,38 @@ def bench_multirhs(args) -> int:
     """k right-hand sides in one launch vs k single-vector launches, 128^3 27-point stencil."""
     from hbdia_amd import HBDIA, DeviceArray, HBDIAVector, gen_vector, hbdiaSpMV, hbdiaSpMVMulti
     n = args.grid or 128
-    A = HBDIA.stencil(n, 27, 0.0, seed=0xC4)
+    dt = np.float32 if args.dtype == "f32" else np.float64
+    s_v = np.dtype(dt).itemsize
+    A = HBDIA.stencil(n, 27, 0.0, seed=0xC4, dtype=dt)
     A.convertToHBDIAFormat(64, 2)
     info = A.info()
     N = n ** 3
     nnz = info["nnz"]
     stream = A.device_ptrs()["bdia_stream"]
-    X1, Y1 = HBDIAVector(gen_vector(N, 1)), HBDIAVector(np.zeros(N))
+    X1, Y1 = HBDIAVector(gen_vector(N, 1).astype(dt)), HBDIAVector(np.zeros(N, dt))
     _, t1 = _time_launches(lambda: hbdiaSpMV(A, X1, Y1, False, True), A.synchronize, stream, args.steps, args.warmup)
     rows = {}
     for k in (1, 4, 8, 16, 32):
-        Xh = np.concatenate([gen_vector(N, 10 + j) for j in range(k)])
-        X, Y = DeviceArray.from_host(Xh), DeviceArray(k * N, np.float64, zero=True)
+        Xh = np.concatenate([gen_vector(N, 10 + j) for j in range(k)]).astype(dt)
+        X, Y = DeviceArray.from_host(Xh), DeviceArray(k * N, dt, zero=True)
         hbdiaSpMVMulti(A, X, Y, k)
         A.synchronize()
         # column k-1 must equal the single-vector result
         Xs = HBDIAVector(Xh[(k - 1) * N:])
-        Ys = HBDIAVector(np.zeros(N))
+        Ys = HBDIAVector(np.zeros(N, dt))
         hbdiaSpMV(A, Xs, Ys, False, True)
         A.synchronize()
         if not np.array_equal(Y.download()[(k - 1) * N:], Ys.download()):
             print(json.dumps({"error": f"multi-RHS column mismatch at k={k}"}))
             return 4
         _, tk = _time_launches(lambda: hbdiaSpMVMulti(A, X, Y, k), A.synchronize, stream, args.steps, args.warmup)
-        b = nnz * 8 + k * N * 16
+        b = nnz * s_v + k * N * 2 * s_v
         rows[k] = {"ms": round(tk, 5), "gflops": round(2.0 * nnz * k / (tk * 1e-3) / 1e9, 1),
                    "gbs": round(b / (tk * 1e-3) / 1e9, 1), "speedup_vs_k_single": round(k * t1 / tk, 3)}
         log(f"k={k}: {rows[k]}")
-    line = {"metric": "multi-RHS SpMV GFLOP/s (FP64), k right-hand sides in one launch", "value": rows[32]["gflops"],
+    line = {"metric": f"multi-RHS SpMV GFLOP/s ({args.dtype.upper().replace('F', 'FP')}), k right-hand sides in one launch", "value": rows[32]["gflops"],
             "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": rows[32]["ms"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
             "data": "synthetic (seeded generator)",
             "config": {"workload": f"stencil27_{n}^3_multirhs", "grid": n, "rows": N, "block_width": 64,
                        "single_vector_ms": round(t1, 5)},

@@ -415,17 +415,21 @@ def test_dense_fallback_groups_are_bit_identical(gpu, N, bw, dtype):
     np.testing.assert_array_equal(outs[1], oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N))
 
 
-@pytest.mark.parametrize("N,bw,with_coo,k", [(1000, 64, True, 3), (1000, 64, False, 8), (4099, 128, True, 5),
-                                             (777, 256, True, 16)])
-def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k):
+@pytest.mark.parametrize("N,bw,with_coo,k,dtype", [(1000, 64, True, 3, np.float64), (1000, 64, False, 8, np.float64),
+                                                   (4099, 128, True, 5, np.float64), (777, 256, True, 16, np.float64),
+                                                   (1000, 64, True, 8, np.float32), (777, 256, False, 5, np.float32)])
+def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k, dtype):
     """The multi-RHS kernel's runs-of-three path: every column equals the hint-free multi-RHS result,
     the single-vector kernel and the oracle."""
-    print(f"case: runs3 multi-RHS N={N} bw={bw} coo={with_coo} k={k}", flush=True)
+    print(f"case: runs3 multi-RHS N={N} bw={bw} coo={with_coo} k={k} {np.dtype(dtype).name}", flush=True)
     values, offsets, block_start, x0, coo = _runs3_case(N, bw, 7 + N + bw + k, with_coo)
+    values = values.astype(dtype)
+    if coo is not None:
+        coo = (coo[0], coo[1], coo[2].astype(dtype))
     L = _layout(values, offsets, block_start, N, bw, coo)
     rng = np.random.default_rng(k)
-    Xh = np.vstack([x0] + [rng.standard_normal(N) for _ in range(k - 1)])
-    Y0 = np.vstack([np.linspace(-1.0, 1.0, N) + j for j in range(k)])
+    Xh = np.vstack([x0] + [rng.standard_normal(N) for _ in range(k - 1)]).astype(dtype)
+    Y0 = np.vstack([np.linspace(-1.0, 1.0, N) + j for j in range(k)]).astype(dtype)
     X = DeviceArray.from_host(Xh.ravel())
     outs = []
     for hints in (False, True):
