@@ -149,6 +149,8 @@ _CAPI_SIGS = {
     "hbdia_mixed_spmv": (_i, [_p, _p, _p]),
     "hbdia_mixed_spmv_raw": (_i, [_p, _p, _p]),
     "hbdia_mixed_fp64_rows": (_i, [_p, POINTER(c_int64)]),
+    "hbdia_mixed_error_bound": (_i, [_p, POINTER(c_double), POINTER(c_double)]),
+    "hbdia_mixed_fp64_masks": (_i, [_p, _p, c_int64]),
     "hbdia_gen_stencil": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_banded": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_double, c_uint64]),
     "hbdia_gen_vector": (_i, [c_int, c_int64, c_uint64, c_double, c_double, _p]),
@@ -169,6 +171,7 @@ _CAPI_SIGS = {
     "hbdia_arena_allocate": (_i, [_p, c_int64, POINTER(c_int64)]),
     "hbdia_arena_release": (_i, [_p, c_int64]),
     "hbdia_arena_info": (_i, [_p, POINTER(c_int64)]),
+    "hbdia_serve_arena_elems": (_i, [c_int64, c_int64, c_int64, POINTER(c_int64)]),
     "hbdia_gen_banded_batch": (_i, [POINTER(c_void_p), c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_double,
                                     c_uint64]),
     "hbdia_banded_batch_shape": (_i, [c_int, c_int, c_int, c_int, c_int, c_uint64, POINTER(c_int32),

@@ -440,9 +440,11 @@ def hbdiaSpMVMulti(matrix: HBDIA, X, Y, nrhs: int, ldx: int | None = None, ldy: 
 
     Asynchronous like hbdiaSpMV; every column equals hbdiaSpMV on that column bit for bit."""
     n = matrix.getNumRows()
-    ldx = n if ldx is None else int(ldx)
+    # fallback columns reach numCols - 1 (blocked columns stay below numRows)
+    nx = max(n, matrix.getNumCols()) if matrix.info()["num_coo"] else n
+    ldx = nx if ldx is None else int(ldx)
     ldy = n if ldy is None else int(ldy)
-    if X.size < ldx * (nrhs - 1) + n or Y.size < ldy * (nrhs - 1) + n:
+    if X.size < ldx * (nrhs - 1) + nx or Y.size < ldy * (nrhs - 1) + n:
         raise ValueError("X or Y too small for nrhs columns")
     try:
         check_capi(capi.hbdia_spmv_multi(matrix._h, X.ptr, ldx, Y.ptr, ldy, int(nrhs), int(bool(execCOOGPU))))
@@ -457,18 +459,28 @@ class HBDIAMixed:
     """FP32 / BF16 value storage for an FP64 matrix (include/Operations/HBDIAMixed.hpp).
 
     spmv(x, y) computes y = A x in FP64 vectors with FP32 accumulation; rows whose
-    cancellation ratio exceeds `cancellation_ratio` are recomputed in FP64."""
+    cancellation ratio exceeds `cancellation_ratio` are recomputed in FP64.  Without a
+    ratio, the one that guarantees DEFAULT_REL_TOLERANCE relative error per row is used;
+    rel_error_bound states the guarantee of the ratio in use."""
 
     PRECISIONS = {"fp32": 1, "bf16": 2}
+    DEFAULT_REL_TOLERANCE = 1e-2
 
-    def __init__(self, matrix: HBDIA, precision: str = "bf16", cancellation_ratio: float = 1e4):
+    def __init__(self, matrix: HBDIA, precision: str = "bf16", cancellation_ratio: float | None = None):
         if matrix.dtype != np.float64:
             raise TypeError("reduced-precision storage needs an FP64 matrix")
+        if cancellation_ratio is not None and not cancellation_ratio > 0:
+            raise ValueError("cancellation ratio must be positive")
         self.matrix = matrix  # keeps the FP64 matrix alive
         self.precision = precision
         h = c_void_p(None)
-        check_capi(capi.hbdia_mixed_create(byref(h), matrix._h, self.PRECISIONS[precision], float(cancellation_ratio)))
+        ratio = 0.0 if cancellation_ratio is None else float(cancellation_ratio)
+        check_capi(capi.hbdia_mixed_create(byref(h), matrix._h, self.PRECISIONS[precision], ratio))
         self._h = h.value
+        r, b = c_double(0.0), c_double(0.0)
+        check_capi(capi.hbdia_mixed_error_bound(self._h, byref(r), byref(b)))
+        self.cancellation_ratio = float(r.value)
+        self.rel_error_bound = float(b.value)
 
     def spmv(self, x: HBDIAVector, y: HBDIAVector) -> None:
         check_capi(capi.hbdia_mixed_spmv(self._h, x._h, y._h))
@@ -480,6 +492,13 @@ class HBDIAMixed:
         n = ctypes.c_int64(0)
         check_capi(capi.hbdia_mixed_fp64_rows(self._h, byref(n)))
         return int(n.value)
+
+    def fp64_mask(self) -> np.ndarray:
+        """Boolean per row: recomputed in FP64 by the last spmv."""
+        n = self.matrix.getNumRows()
+        words = np.zeros(-(-n // 64), np.uint64)
+        check_capi(capi.hbdia_mixed_fp64_masks(self._h, words.ctypes.data, words.size))
+        return np.unpackbits(words.view(np.uint8), bitorder="little")[:n].astype(bool)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -629,10 +648,17 @@ class HBDIAServer:
         return int(n.value)
 
     def submit(self, m: int, x, y, device: bool = False) -> int:
+        # The server copies num_rows(m) elements from x and into y asynchronously:
+        # shorter arrays would be read and written past their end.
+        n = self.num_rows(m)
+        if x.dtype != self.dtype or y.dtype != self.dtype:
+            raise TypeError(f"x and y must be {self.dtype} arrays")
+        if x.size < n or y.size < n:
+            raise ValueError(f"x and y need at least {n} elements for matrix {m} (got {x.size}, {y.size})")
         if device:
             xp, yp = x.ptr, y.ptr
         else:
-            if x.dtype != self.dtype or y.dtype != self.dtype or not x.flags.c_contiguous or not y.flags.c_contiguous:
+            if not x.flags.c_contiguous or not y.flags.c_contiguous:
                 raise TypeError(f"x and y must be contiguous {self.dtype} arrays")
             xp, yp = x.ctypes.data, y.ctypes.data
         rid = ctypes.c_int64(0)

@@ -1,6 +1,7 @@
 // HBDIAMixed.cpp -- reduced-precision storage (see Operations/HBDIAMixed.hpp).
 #include "Operations/HBDIAMixed.hpp"
 
+#include <algorithm>
 #include <stdexcept>
 #include <vector>
 
@@ -9,8 +10,36 @@ size_t elem_bytes(StoragePrecision p) { return p == StoragePrecision::FP32 ? 4 :
 int vdtype(StoragePrecision p) { return p == StoragePrecision::FP32 ? HBDIA_VALUES_F32 : HBDIA_VALUES_BF16; }
 }  // namespace
 
+double HBDIAMixed::effectiveUnitRoundoff(StoragePrecision precision, int64_t terms) {
+  const double u24 = 0x1p-24;
+  const double store = precision == StoragePrecision::FP32 ? u24 : 0x1p-8 + u24;
+  return store + static_cast<double>(terms + 2) * u24;
+}
+
+int64_t HBDIAMixed::maxRowTerms(const HBDIA<double>& matrix) {
+  int64_t blocked = 0;
+  for (const auto& o : matrix.getOffsetsPerBlock()) blocked = std::max<int64_t>(blocked, static_cast<int64_t>(o.size()));
+  // the fallback is row-sorted: its longest run of one row
+  const std::vector<int>& rows = matrix.getCpuRowIndices();
+  int64_t coo = 0;
+  for (size_t i = 0, j; i < rows.size(); i = j) {
+    for (j = i + 1; j < rows.size() && rows[j] == rows[i]; ++j) {
+    }
+    coo = std::max<int64_t>(coo, static_cast<int64_t>(j - i));
+  }
+  return blocked + coo;
+}
+
+double HBDIAMixed::ratioForTolerance(const HBDIA<double>& matrix, StoragePrecision precision, double relTol) {
+  if (!(relTol > 0.0)) throw std::invalid_argument("HBDIAMixed: the relative tolerance must be positive");
+  return relTol / effectiveUnitRoundoff(precision, maxRowTerms(matrix));
+}
+
+HBDIAMixed::HBDIAMixed(const HBDIA<double>& matrix, StoragePrecision precision)
+    : HBDIAMixed(matrix, precision, ratioForTolerance(matrix, precision, kDefaultRelTolerance)) {}
+
 HBDIAMixed::HBDIAMixed(const HBDIA<double>& matrix, StoragePrecision precision, double cancellationRatio)
-    : m_(matrix), precision_(precision), ratio_(cancellationRatio) {
+    : m_(matrix), precision_(precision), ratio_(cancellationRatio), maxTerms_(maxRowTerms(matrix)) {
   if (!matrix.isHBDIAFormat() || !matrix.areStreamsInitialized())
     throw std::runtime_error("HBDIAMixed: matrix is not converted and on the device");
   if (matrix.isPartialMatrix())
@@ -73,15 +102,21 @@ void HBDIAMixed::spmv(const double* x, double* y) {
 void HBDIAMixed::spmv(const HBDIAVector<double>& x, HBDIAVector<double>& y) {
   if (y.getLocalSize() < static_cast<size_t>(m_.getNumRows()) || x.getTotalSize() < static_cast<size_t>(m_.getNumRows()))
     throw std::invalid_argument("HBDIAMixed::spmv: vector shorter than the number of rows");
+  if (x.getTotalSize() < static_cast<size_t>(m_.getNumCols()))
+    throw std::invalid_argument("HBDIAMixed::spmv: input vector shorter than the number of columns");
   spmv(x.getDeviceDataPtr(), y.getDeviceLocalPtr());
 }
 
-int64_t HBDIAMixed::lastFp64Rows() const {
+std::vector<uint64_t> HBDIAMixed::lastFp64Masks() const {
   const int64_t groups = (static_cast<int64_t>(m_.getNumRows()) + HBDIA_ROW_GROUP - 1) / HBDIA_ROW_GROUP;
   std::vector<uint64_t> masks(static_cast<size_t>(groups));
   CHECK_HBDIA(hbdia_stream_synchronize(m_.getBDIAStream()));
   CHECK_HBDIA(hbdia_memcpy(masks.data(), workspace_, masks.size() * sizeof(uint64_t)));
+  return masks;
+}
+
+int64_t HBDIAMixed::lastFp64Rows() const {
   int64_t n = 0;
-  for (uint64_t w : masks) n += __builtin_popcountll(w);
+  for (uint64_t w : lastFp64Masks()) n += __builtin_popcountll(w);
   return n;
 }

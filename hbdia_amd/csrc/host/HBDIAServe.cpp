@@ -52,6 +52,15 @@ void VectorArena::release(int64_t offset) {
   free_[off] = size;
 }
 
+int64_t serveArenaElems(int64_t budgetBytes, int64_t residentBytes, int64_t elemBytes) {
+  if (elemBytes <= 0 || residentBytes < 0) throw std::invalid_argument("serveArenaElems: invalid sizes");
+  if (budgetBytes <= residentBytes)
+    throw std::invalid_argument("HBDIAServer: the HBM budget does not cover the resident matrices");
+  const int64_t elems = std::min((budgetBytes - residentBytes) / (2 * elemBytes), kMaxServeArenaElems) / 64 * 64;
+  if (elems < 64) throw std::invalid_argument("HBDIAServer: no room for request vectors in the budget");
+  return elems;
+}
+
 int64_t VectorArena::largestFree() const {
   int64_t best = 0;
   for (const auto& f : free_) best = std::max(best, f.second);
@@ -97,10 +106,7 @@ HBDIAServer<T>::HBDIAServer(const std::vector<const HBDIA<T>*>& matrices, const 
   pool_ = std::make_unique<HBDIABatch<T>>(matrices);
   const int64_t resident = pool_->storedBytes();
   const int64_t budget = cfg_.hbmBudgetBytes > 0 ? cfg_.hbmBudgetBytes : resident + (int64_t{1} << 30);
-  if (budget <= resident)
-    throw std::invalid_argument("HBDIAServer: the HBM budget does not cover the resident matrices");
-  arenaElems_ = (budget - resident) / (2 * static_cast<int64_t>(sizeof(T))) / 64 * 64;
-  if (arenaElems_ < 64) throw std::invalid_argument("HBDIAServer: no room for request vectors in the budget");
+  arenaElems_ = hbdia::serveArenaElems(budget, resident, static_cast<int64_t>(sizeof(T)));
   arena_ = hbdia::VectorArena(arenaElems_, 64);
   workgroups_ = cfg_.workgroups > 0 ? cfg_.workgroups : pool_->residentWorkgroups(HBDIA_BATCH_INTERLEAVED);
   try {

@@ -85,6 +85,10 @@ void check_ready(const HBDIA<T>& m, const HBDIAVector<T>& x, const HBDIAVector<T
     throw std::invalid_argument("hbdiaSpMV: output vector shorter than the number of rows");
   if (x.getLocalSize() < static_cast<size_t>(m.getNumRows()))
     throw std::invalid_argument("hbdiaSpMV: input vector shorter than the number of rows");
+  // The fallback gathers x[col] for every col < numCols from the data pointer
+  // (left halo included), so a rectangular matrix needs numCols entries there.
+  if (!m.isPartialMatrix() && x.getTotalSize() < static_cast<size_t>(m.getNumCols()))
+    throw std::invalid_argument("hbdiaSpMV: input vector shorter than the number of columns");
   if (m.isPartialMatrix()) {
     if (!m.hasLocalCOO()) throw std::runtime_error("hbdiaSpMV: partial matrix without its rank-local layout");
     if (x.getLeftSize() != static_cast<size_t>(m.getHaloLeftSize()) ||
@@ -218,6 +222,11 @@ bool hbdiaSpMVMulti(HBDIA<T>& matrix, const T* X, int64_t ldx, T* Y, int64_t ldy
     throw std::invalid_argument("hbdiaSpMVMulti: rank-local (partial) matrices take hbdiaSpMV with halo vectors");
   const bool hasCOOWork = !matrix.getCpuValues().empty();
   if (hasCOOWork && !execCOOGPU) throw std::runtime_error("Invalid COO execution flags");
+  if (nrhs < 1) throw std::invalid_argument("hbdiaSpMVMulti: nrhs must be at least 1");
+  // Blocked columns are clamped into [0, numRows); fallback columns reach numCols - 1.
+  const int64_t xNeed = hasCOOWork ? std::max(matrix.getNumRows(), matrix.getNumCols()) : matrix.getNumRows();
+  if (ldx < xNeed || ldy < matrix.getNumRows())
+    throw std::invalid_argument("hbdiaSpMVMulti: ldx must cover the columns and ldy the rows");
   if (hasCOOWork && !matrix.hasDeviceCOO())
     throw std::runtime_error("hbdiaSpMVMulti: the fallback is not on the device (convert with ExecutionMode::GPU_COO)");
   if (nrhs == 1) {

@@ -582,8 +582,24 @@ int hbdia_mixed_create(hbdia_mixed_t* h, hbdia_matrix_t m, int precision, double
     need(m->dtype == HBDIA_DTYPE_F64, "reduced-precision storage needs an FP64 matrix");
     need(precision == 1 || precision == 2, "precision must be 1 (FP32) or 2 (BF16)");
     auto o = std::make_unique<hbdia_mixed_opaque>();
-    o->impl = std::make_unique<HBDIAMixed>(*m->d, static_cast<StoragePrecision>(precision), ratio);
+    const auto p = static_cast<StoragePrecision>(precision);
+    o->impl = ratio > 0.0 ? std::make_unique<HBDIAMixed>(*m->d, p, ratio) : std::make_unique<HBDIAMixed>(*m->d, p);
     *h = o.release();
+  });
+}
+int hbdia_mixed_fp64_masks(hbdia_mixed_t h, uint64_t* masks, int64_t words) {
+  return guard("hbdia_mixed_fp64_masks", [&] {
+    need(h && masks, "null argument");
+    const std::vector<uint64_t> m = h->impl->lastFp64Masks();
+    need(words >= static_cast<int64_t>(m.size()), "mask buffer shorter than ceil(rows / 64) words");
+    std::copy(m.begin(), m.end(), masks);
+  });
+}
+int hbdia_mixed_error_bound(hbdia_mixed_t h, double* ratio, double* rel_error_bound) {
+  return guard("hbdia_mixed_error_bound", [&] {
+    need(h && ratio && rel_error_bound, "null argument");
+    *ratio = h->impl->cancellationRatio();
+    *rel_error_bound = h->impl->relErrorBound();
   });
 }
 int hbdia_mixed_destroy(hbdia_mixed_t h) {
@@ -1059,6 +1075,13 @@ int hbdia_arena_release(hbdia_arena_t a, int64_t offset) {
   return guard("hbdia_arena_release", [&] {
     need(a != nullptr, "null argument");
     a->arena.release(offset);
+  });
+}
+
+int hbdia_serve_arena_elems(int64_t budget_bytes, int64_t resident_bytes, int64_t elem_bytes, int64_t* elems) {
+  return guard("hbdia_serve_arena_elems", [&] {
+    need(elems != nullptr, "null argument");
+    *elems = hbdia::serveArenaElems(budget_bytes, resident_bytes, elem_bytes);
   });
 }
 

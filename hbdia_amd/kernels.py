@@ -183,7 +183,8 @@ class DeviceLayout:
     """
 
     def __init__(self, values: np.ndarray, offsets: np.ndarray, block_start: np.ndarray, num_rows: int,
-                 block_width: int, coo_rows=None, coo_cols=None, coo_values=None, stream: Stream | None = None):
+                 block_width: int, coo_rows=None, coo_cols=None, coo_values=None, stream: Stream | None = None,
+                 pad_offsets: bool = True):
         self.num_rows = int(num_rows)
         self.block_width = int(block_width)
         self.num_blocks = int(len(block_start) - 1)
@@ -191,9 +192,12 @@ class DeviceLayout:
         self.runs3 = DeviceArray.from_host(bits) if bits is not None else None
         self.dtype = np.dtype(values.dtype)
         self.values = DeviceArray.from_host(values if values.size else np.zeros(1, self.dtype))
-        # blocked kernels read HBDIA_OFFSETS_PAD entries past the last offset
-        self.offsets = DeviceArray.from_host(np.concatenate([np.asarray(offsets, np.int32),
-                                                             np.zeros(OFFSETS_PAD, np.int32)]))
+        # the multi-RHS and reduced-precision kernels read HBDIA_OFFSETS_PAD entries past the last
+        # offset; pad_offsets=False uploads the array at its exact size (single-vector kernels only)
+        offs = np.asarray(offsets, np.int32)
+        if pad_offsets:
+            offs = np.concatenate([offs, np.zeros(OFFSETS_PAD, np.int32)])
+        self.offsets = DeviceArray.from_host(offs if offs.size else np.zeros(1, np.int32))
         self.block_start = DeviceArray.from_host(np.asarray(block_start, dtype=np.int32))
         self.coo_nnz = 0 if coo_rows is None else int(len(coo_rows))
         if self.coo_nnz:

@@ -53,6 +53,14 @@ class VectorArena {
   std::unordered_map<int64_t, int64_t> taken_;  // offset -> size
 };
 
+// Elements of each of the server's x and y arenas for an HBM budget and the
+// resident matrices' bytes: half of what remains per vector, a multiple of 64,
+// and at most kMaxServeArenaElems, because the serve kernel addresses a
+// request's rows as int32 (pool rows + slot shift).  Throws when the budget
+// leaves no room.
+constexpr int64_t kMaxServeArenaElems = (int64_t{1} << 31) - 64;
+int64_t serveArenaElems(int64_t budgetBytes, int64_t residentBytes, int64_t elemBytes);
+
 }  // namespace hbdia
 
 struct ServeConfig {

@@ -205,13 +205,21 @@ int hbdia_vector_verify_halo(hbdia_vector_t v, hbdia_matrix_t m, int* ok);
 int hbdia_spmv_multi(hbdia_matrix_t m, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrhs, int exec_coo_gpu);
 
 /* Reduced-precision storage (HBDIAMixed, include/Operations/HBDIAMixed.hpp):
- * precision 1 = FP32, 2 = BF16; an FP64 matrix only.  x, y: FP64 vectors. */
+ * precision 1 = FP32, 2 = BF16; an FP64 matrix only.  x, y: FP64 vectors.
+ * cancellation_ratio <= 0 selects the threshold that guarantees
+ * HBDIAMixed::kDefaultRelTolerance (1e-2) relative error per row. */
 typedef struct hbdia_mixed_opaque* hbdia_mixed_t;
 int hbdia_mixed_create(hbdia_mixed_t* h, hbdia_matrix_t m, int precision, double cancellation_ratio);
 int hbdia_mixed_destroy(hbdia_mixed_t h);
 int hbdia_mixed_spmv(hbdia_mixed_t h, hbdia_vector_t x, hbdia_vector_t y);
 int hbdia_mixed_spmv_raw(hbdia_mixed_t h, const void* x, void* y);
 int hbdia_mixed_fp64_rows(hbdia_mixed_t h, int64_t* rows);
+/* The rows the last spmv recomputed in FP64: bit r of masks[g] is row 64 g + r;
+ * words >= ceil(rows / 64). */
+int hbdia_mixed_fp64_masks(hbdia_mixed_t h, uint64_t* masks, int64_t words);
+/* The cancellation ratio in use and the relative error guaranteed for every
+ * row kept in FP32 (HBDIAMixed::relErrorBound). */
+int hbdia_mixed_error_bound(hbdia_mixed_t h, double* ratio, double* rel_error_bound);
 
 /* Many matrices in one persistent launch (HBDIABatch and autotuneBatch,
  * include/Operations/HBDIABatch.hpp).  The matrices share one dtype, are
@@ -298,6 +306,11 @@ int hbdia_arena_destroy(hbdia_arena_t a);
 int hbdia_arena_allocate(hbdia_arena_t a, int64_t n, int64_t* offset); /* *offset = -1 when it does not fit */
 int hbdia_arena_release(hbdia_arena_t a, int64_t offset);
 int hbdia_arena_info(hbdia_arena_t a, int64_t* info); /* capacity, used, largest free block, free blocks */
+/* Elements of each request arena (x and y) that a server with this HBM budget,
+ * resident bytes and element size gets (host only): half the remainder per
+ * vector, a multiple of 64, capped at 2^31 - 64 because the serve kernel
+ * addresses rows as int32. */
+int hbdia_serve_arena_elems(int64_t budget_bytes, int64_t resident_bytes, int64_t elem_bytes, int64_t* elems);
 
 /* Seeded benchmark inputs (generators in Workloads.hpp). */
 int hbdia_gen_stencil(hbdia_matrix_t* m, int dtype, int n, int points, double scatter, uint64_t seed);

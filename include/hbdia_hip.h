@@ -30,10 +30,14 @@ extern "C" {
 
 /* Rows handled by one 64-lane wavefront; COO group pointers are per group. */
 #define HBDIA_ROW_GROUP 64
-/* Every `offsets` array passed to a blocked kernel must stay readable for
- * HBDIA_OFFSETS_PAD int32 past its last offset (block_start[num_blocks]); the
- * kernels read a chunk's offsets as one wide load and ignore what lies past a
- * block's count.  HBDIA<T>::prepareForGPU allocates this padding. */
+/* The multi-RHS (hbdia_hbdia_spmm_*) and reduced-precision
+ * (hbdia_hbdia_spmv_mixed*) entry points need their `offsets` array readable
+ * for HBDIA_OFFSETS_PAD int32 past its last offset (block_start[num_blocks]):
+ * they read a chunk's offsets as one wide load and ignore what lies past a
+ * block's count.  HBDIA<T>::prepareForGPU allocates this padding.  The
+ * single-vector entry points (hbdia_bdia_spmv_*, hbdia_hbdia_spmv_*, their _ex
+ * forms), which replace the reference's launch_bdia_spmv_kernel, read nothing
+ * past block_start[num_blocks]: an exactly-sized array is enough. */
 #define HBDIA_OFFSETS_PAD 32
 
 typedef struct hbdia_stream_opaque* hbdia_stream_t;

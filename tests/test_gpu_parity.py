@@ -643,3 +643,68 @@ def test_matrix_market_file_through_the_gpu_path(gpu, tmp_path, symmetric):
     assert np.max(np.abs(y - ref) / np.maximum(scale, 1e-300)) < 1e-12
     if symmetric:
         assert np.all(V == 1.0) and len(V) > len(lines)
+
+
+def test_rectangular_matrix_needs_x_over_every_column(gpu):
+    """A 500 x 900 matrix whose fallback reaches columns past the rows: an x of
+    500 entries is refused before any launch (the fallback gathers x[col] for
+    col < 900); with 900 entries the result is the oracle's."""
+    print("case: rectangular 500x900, x length check", flush=True)
+    nr, nc = 500, 900
+    rows, cols = [], []
+    for i in range(nr):
+        for c in sorted({max(i - 1, 0), i, min(i + 1, nr - 1), nr + (7 * i) % (nc - nr)}):
+            rows.append(i)
+            cols.append(c)
+    vals = np.random.default_rng(12).uniform(-1, 1, len(rows))
+    A = HBDIA(np.array(rows), np.array(cols), vals, nr, nc)
+    A.convertToHBDIAFormat(64, 2)
+    assert A.info()["num_coo"] >= nr
+    with pytest.raises(ValueError, match="columns"):
+        hbdiaSpMV(A, HBDIAVector(np.ones(nr)), HBDIAVector(np.zeros(nr)), False, True)
+    x = np.random.default_rng(13).uniform(-1, 1, nc)
+    y = run_spmv(A, x, np.zeros(nr), False, True)
+    assert y.tobytes() == oracle_y(A, x, np.zeros(nr)).tobytes()
+    X = DeviceArray.from_host(np.ones(nr))
+    Y = DeviceArray(nr, np.float64, zero=True)
+    with pytest.raises(ValueError):
+        from hbdia_amd import hbdiaSpMVMulti
+        hbdiaSpMVMulti(A, X, Y, 1)
+
+
+@pytest.mark.parametrize("N,bw,with_coo,dtype,hints", [(1000, 64, False, np.float64, True),
+                                                       (1000, 64, True, np.float64, True),
+                                                       (4099, 128, True, np.float64, True),
+                                                       (1000, 64, True, np.float32, True),
+                                                       (1000, 64, True, np.float64, False),
+                                                       (4096, 32, True, np.float64, True),
+                                                       (4096, 32, False, np.float32, True),
+                                                       (1003, 5, True, np.float64, False)])
+def test_exactly_sized_offsets_array(gpu, N, bw, with_coo, dtype, hints):
+    """The single-vector entry points (the reference's launch_bdia_spmv_kernel signature) read nothing
+    past block_start[num_blocks]: with the offsets uploaded at their exact size -- no padding -- every
+    path (one-row, two-row, width 32, per-lane) gives the oracle's y bit for bit."""
+    print(f"case: exact offsets N={N} bw={bw} coo={with_coo} {np.dtype(dtype).name} hints={hints}", flush=True)
+    if bw == 32:
+        values, offsets, block_start, x, coo = _width32_case(N, 77, with_coo, share=1.0, group=4)
+    else:
+        values, offsets, block_start, x, coo = _runs3_case(N, bw, 97 + N + bw, with_coo)
+    values, x = values.astype(dtype), x.astype(dtype)
+    if coo is not None:
+        coo = (coo[0], coo[1], coo[2].astype(dtype))
+    args = (np.asarray(values), np.asarray(offsets, np.int32), np.asarray(block_start, np.int32), N, bw)
+    L = kernels.DeviceLayout(*args, *(coo or ()), pad_offsets=False)
+    assert L.offsets.size == max(len(offsets), 1)
+    y0 = np.linspace(-1.0, 1.0, N).astype(dtype)
+    X = DeviceArray.from_host(x)
+    Y = DeviceArray.from_host(y0)
+    L.spmv(X, Y, hints=hints)
+    synchronize()
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": len(block_start) - 1,
+          "block_width": bw}
+    if coo is not None:
+        hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
+    else:
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0, values.dtype))
+    want = oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N)
+    assert Y.download().tobytes() == want.tobytes()

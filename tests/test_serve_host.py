@@ -73,3 +73,26 @@ def test_arena_rejects_bad_calls():
         a.allocate(0)
     with pytest.raises(HBDIAError):
         VectorArena(-1, 64)
+
+
+def _arena_elems(budget: int, resident: int, elem: int) -> int:
+    import ctypes
+
+    from hbdia_amd.api import capi, check_capi
+    out = ctypes.c_int64(0)
+    check_capi(capi.hbdia_serve_arena_elems(budget, resident, elem, ctypes.byref(out)))
+    return int(out.value)
+
+
+def test_serve_arena_is_capped_below_int32_rows():
+    # the serve kernel adds a request's slot shift to int32 pool rows: the
+    # arena must stay below 2^31 elements whatever the HBM budget
+    gib = 1 << 30
+    assert _arena_elems(resident=gib, budget=gib + (1 << 20), elem=8) == (1 << 20) // 16
+    assert _arena_elems(resident=0, budget=64 * gib, elem=8) == (1 << 31) - 64      # 64 GiB FP64 budget
+    assert _arena_elems(resident=5 * gib, budget=280 * gib, elem=4) == (1 << 31) - 64
+    assert _arena_elems(resident=0, budget=(1 << 31) * 16 - 1, elem=8) % 64 == 0
+    with pytest.raises(HBDIAError):
+        _arena_elems(resident=gib, budget=gib, elem=8)       # budget does not cover the matrices
+    with pytest.raises(HBDIAError):
+        _arena_elems(resident=gib, budget=gib + 1000, elem=8)  # room for fewer than 64 elements
