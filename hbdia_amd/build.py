@@ -1,6 +1,13 @@
 """Builds the native libraries of hbdia_amd in-tree.
 
-    python hbdia_amd/build.py [--oracle]   (a script: importing the package needs the built libraries)
+    python hbdia_amd/build.py [--oracle] [--sanitize]
+        (a script: importing the package needs the built libraries)
+
+--sanitize builds the host library (g++ only; no device code) with
+AddressSanitizer and UndefinedBehaviorSanitizer for a CPU test run:
+    LD_PRELOAD=$(g++ -print-file-name=libasan.so) ASAN_OPTIONS=detect_leaks=0 \
+        python -m pytest tests -m "not gpu"
+A plain build afterwards restores the optimised library.
 
 Outputs (git-ignored, shipped with the tree):
     hbdia_amd/lib/libhbdia_hip.so   hipcc, gfx950: kernels + the C ABI of include/hbdia_hip.h
@@ -56,9 +63,13 @@ def _compile_hip(src: Path) -> Path:
     return obj
 
 
-def _compile_host(src: Path) -> Path:
-    obj = OBJ / (src.stem + ".o")
-    _run([CXX, *CXX_FLAGS, f"-I{INCLUDE}", "-c", str(src), "-o", str(obj)])
+SANITIZE = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined"]
+
+
+def _compile_host(src: Path, sanitize: bool = False) -> Path:
+    obj = OBJ / (src.stem + (".asan.o" if sanitize else ".o"))
+    flags = [*CXX_FLAGS, "-O1", "-g", *SANITIZE] if sanitize else CXX_FLAGS
+    _run([CXX, *flags, f"-I{INCLUDE}", "-c", str(src), "-o", str(obj)])
     return obj
 
 
@@ -68,18 +79,18 @@ def _link_atomic(cmd_prefix: list[str], objs: list[Path], out: Path, extra: list
     os.replace(tmp, out)
 
 
-def build_all(jobs: int | None = None, verbose: bool = True) -> None:
+def build_all(jobs: int | None = None, verbose: bool = True, sanitize: bool = False) -> None:
     """Compile every HIP and host source and link the shared libraries."""
     LIB.mkdir(parents=True, exist_ok=True)
     OBJ.mkdir(parents=True, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         hip_objs = list(ex.map(_compile_hip, HIP_SOURCES))
-        host_objs = list(ex.map(_compile_host, HOST_SOURCES))
+        host_objs = list(ex.map(lambda src: _compile_host(src, sanitize), HOST_SOURCES))
     hip_lib = LIB / "libhbdia_hip.so"
     _link_atomic([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC"], hip_objs, hip_lib, [])
     host_lib = LIB / "libhbdia.so"
-    _link_atomic([CXX, "-shared", "-fopenmp"], host_objs, host_lib,
+    _link_atomic([CXX, "-shared", "-fopenmp", *(SANITIZE if sanitize else [])], host_objs, host_lib,
                  [f"-L{LIB}", "-lhbdia_hip", "-Wl,-rpath,$ORIGIN"])
     example = CSRC / "examples" / "simple_hbdia_example.cpp"
     if example.exists():
@@ -102,6 +113,6 @@ def build_oracle(verbose: bool = True) -> None:
 
 
 if __name__ == "__main__":
-    build_all()
+    build_all(sanitize="--sanitize" in sys.argv)
     if "--oracle" in sys.argv:
         build_oracle()

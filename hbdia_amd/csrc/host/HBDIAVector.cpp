@@ -10,35 +10,35 @@ template <typename T>
 HBDIAVector<T>::HBDIAVector() = default;
 
 template <typename T>
-HBDIAVector<T>::HBDIAVector(const std::vector<T>& localData) : localVector_(localData) {
-  size_local_ = localData.size();
+HBDIAVector<T>::HBDIAVector(const std::vector<T>& localData) : hostInit_(localData) {
+  localSize_ = localData.size();
   setupUnifiedMemory();
 }
 
 template <typename T>
 HBDIAVector<T>::HBDIAVector(const std::vector<T>& localData, bool unifiedMemory, bool unifiedMemoryMalloc,
                             bool unifiedMemoryManagedMalloc, bool unifiedNumaAllocOnNode)
-    : localVector_(localData) {
-  unifiedMemory_ = unifiedMemory;
-  unifiedMemoryMalloc_ = unifiedMemoryMalloc;
-  unifiedMemoryManagedMalloc_ = unifiedMemoryManagedMalloc;
-  unifiedNumaAllocOnNode_ = unifiedNumaAllocOnNode;
-  size_local_ = localData.size();
+    : hostInit_(localData) {
+  unified_ = unifiedMemory;
+  unifiedPinned_ = unifiedMemoryMalloc;
+  unifiedManaged_ = unifiedMemoryManagedMalloc;
+  unifiedNuma_ = unifiedNumaAllocOnNode;
+  localSize_ = localData.size();
   setupUnifiedMemory();
 }
 
 template <typename T>
 HBDIAVector<T>::HBDIAVector(const std::vector<T>& localData, const HBDIA<T>& matrix, int rank, int size)
-    : localVector_(localData) {
-  size_local_ = localData.size();
+    : hostInit_(localData) {
+  localSize_ = localData.size();
   if (matrix.isPartialMatrix() && matrix.hasPartialMatrixMetadata()) {
     const auto& md = matrix.getPartialMatrixMetadata();
     for (int p = 0; p < size && p < static_cast<int>(md.processDataRanges.size()); ++p) {
       if (p == rank) continue;
       for (const auto& range : md.processDataRanges[p]) {
         const size_t n = static_cast<size_t>(std::get<1>(range) - std::get<0>(range));
-        if (p < rank) size_recv_left_ += n;
-        else size_recv_right_ += n;
+        if (p < rank) haloLeft_ += n;
+        else haloRight_ += n;
       }
     }
   }
@@ -52,26 +52,26 @@ HBDIAVector<T>::~HBDIAVector() {
 
 template <typename T>
 void HBDIAVector<T>::moveFrom(HBDIAVector& o) noexcept {
-  localVector_ = std::move(o.localVector_);
-  size_recv_left_ = std::exchange(o.size_recv_left_, 0);
-  size_recv_right_ = std::exchange(o.size_recv_right_, 0);
-  size_local_ = std::exchange(o.size_local_, 0);
-  unifiedMemory_ = o.unifiedMemory_;
-  unifiedMemoryMalloc_ = o.unifiedMemoryMalloc_;
-  unifiedMemoryManagedMalloc_ = o.unifiedMemoryManagedMalloc_;
-  unifiedNumaAllocOnNode_ = o.unifiedNumaAllocOnNode_;
-  unified_data_ptr_ = std::exchange(o.unified_data_ptr_, nullptr);
-  unified_local_ptr_ = std::exchange(o.unified_local_ptr_, nullptr);
-  data_ptr_d_ = std::exchange(o.data_ptr_d_, nullptr);
-  left_ptr_d_ = std::exchange(o.left_ptr_d_, nullptr);
-  local_ptr_d_ = std::exchange(o.local_ptr_d_, nullptr);
-  right_ptr_d_ = std::exchange(o.right_ptr_d_, nullptr);
-  coo_result_ptr_d_ = std::exchange(o.coo_result_ptr_d_, nullptr);
-  data_ptr_h_ = std::exchange(o.data_ptr_h_, nullptr);
-  left_ptr_h_ = std::exchange(o.left_ptr_h_, nullptr);
-  local_ptr_h_ = std::exchange(o.local_ptr_h_, nullptr);
-  right_ptr_h_ = std::exchange(o.right_ptr_h_, nullptr);
-  coo_result_ptr_h_ = std::exchange(o.coo_result_ptr_h_, nullptr);
+  hostInit_ = std::move(o.hostInit_);
+  haloLeft_ = std::exchange(o.haloLeft_, 0);
+  haloRight_ = std::exchange(o.haloRight_, 0);
+  localSize_ = std::exchange(o.localSize_, 0);
+  unified_ = o.unified_;
+  unifiedPinned_ = o.unifiedPinned_;
+  unifiedManaged_ = o.unifiedManaged_;
+  unifiedNuma_ = o.unifiedNuma_;
+  unifiedBase_ = std::exchange(o.unifiedBase_, nullptr);
+  unifiedLocal_ = std::exchange(o.unifiedLocal_, nullptr);
+  devBase_ = std::exchange(o.devBase_, nullptr);
+  devLeft_ = std::exchange(o.devLeft_, nullptr);
+  devLocal_ = std::exchange(o.devLocal_, nullptr);
+  devRight_ = std::exchange(o.devRight_, nullptr);
+  devCooResult_ = std::exchange(o.devCooResult_, nullptr);
+  hostBase_ = std::exchange(o.hostBase_, nullptr);
+  hostLeft_ = std::exchange(o.hostLeft_, nullptr);
+  hostLocal_ = std::exchange(o.hostLocal_, nullptr);
+  hostRight_ = std::exchange(o.hostRight_, nullptr);
+  hostCooResult_ = std::exchange(o.hostCooResult_, nullptr);
 }
 
 template <typename T>
@@ -92,15 +92,15 @@ HBDIAVector<T>& HBDIAVector<T>::operator=(HBDIAVector&& o) noexcept {
 // local segment receives localData.
 template <typename T>
 void HBDIAVector<T>::setupUnifiedMemory() {
-  const size_t total = size_recv_left_ + size_local_ + size_recv_right_;
-  const size_t bytes = total * sizeof(T), cooBytes = size_local_ * sizeof(T);
+  const size_t total = haloLeft_ + localSize_ + haloRight_;
+  const size_t bytes = total * sizeof(T), cooBytes = localSize_ * sizeof(T);
   void* p = nullptr;
   void* q = nullptr;
-  if (unifiedMemory_) {
-    if (unifiedMemoryManagedMalloc_) {
+  if (unified_) {
+    if (unifiedManaged_) {
       CHECK_HBDIA(hbdia_malloc_managed(&p, bytes));
       CHECK_HBDIA(hbdia_malloc_managed(&q, cooBytes));
-    } else if (unifiedMemoryMalloc_ || unifiedNumaAllocOnNode_) {
+    } else if (unifiedPinned_ || unifiedNuma_) {
       CHECK_HBDIA(hbdia_malloc_host(&p, bytes));
       CHECK_HBDIA(hbdia_malloc_host(&q, cooBytes));
     } else {
@@ -110,84 +110,84 @@ void HBDIAVector<T>::setupUnifiedMemory() {
     CHECK_HBDIA(hbdia_malloc(&p, bytes));
     CHECK_HBDIA(hbdia_malloc(&q, cooBytes));
   }
-  data_ptr_d_ = static_cast<T*>(p);
-  coo_result_ptr_d_ = static_cast<T*>(q);
-  left_ptr_d_ = data_ptr_d_;
-  local_ptr_d_ = data_ptr_d_ ? data_ptr_d_ + size_recv_left_ : nullptr;
-  right_ptr_d_ = data_ptr_d_ ? data_ptr_d_ + size_recv_left_ + size_local_ : nullptr;
-  CHECK_HBDIA(hbdia_memset(data_ptr_d_, 0, bytes, nullptr));
-  CHECK_HBDIA(hbdia_memset(coo_result_ptr_d_, 0, cooBytes, nullptr));
+  devBase_ = static_cast<T*>(p);
+  devCooResult_ = static_cast<T*>(q);
+  devLeft_ = devBase_;
+  devLocal_ = devBase_ ? devBase_ + haloLeft_ : nullptr;
+  devRight_ = devBase_ ? devBase_ + haloLeft_ + localSize_ : nullptr;
+  CHECK_HBDIA(hbdia_memset(devBase_, 0, bytes, nullptr));
+  CHECK_HBDIA(hbdia_memset(devCooResult_, 0, cooBytes, nullptr));
   CHECK_HBDIA(hbdia_device_synchronize());
-  CHECK_HBDIA(hbdia_memcpy(local_ptr_d_, localVector_.data(), cooBytes));
-  if (unifiedMemory_) {
-    unified_data_ptr_ = data_ptr_d_;
-    unified_local_ptr_ = local_ptr_d_;
+  CHECK_HBDIA(hbdia_memcpy(devLocal_, hostInit_.data(), cooBytes));
+  if (unified_) {
+    unifiedBase_ = devBase_;
+    unifiedLocal_ = devLocal_;
   } else {
     void* h = nullptr;
     void* hc = nullptr;
     CHECK_HBDIA(hbdia_malloc_host(&h, bytes));
     CHECK_HBDIA(hbdia_malloc_host(&hc, cooBytes));
-    data_ptr_h_ = static_cast<T*>(h);
-    coo_result_ptr_h_ = static_cast<T*>(hc);
-    if (data_ptr_h_) std::fill(data_ptr_h_, data_ptr_h_ + total, T{});
-    if (coo_result_ptr_h_) std::fill(coo_result_ptr_h_, coo_result_ptr_h_ + size_local_, T{});
-    left_ptr_h_ = data_ptr_h_;
-    local_ptr_h_ = data_ptr_h_ ? data_ptr_h_ + size_recv_left_ : nullptr;
-    right_ptr_h_ = data_ptr_h_ ? data_ptr_h_ + size_recv_left_ + size_local_ : nullptr;
+    hostBase_ = static_cast<T*>(h);
+    hostCooResult_ = static_cast<T*>(hc);
+    if (hostBase_) std::fill(hostBase_, hostBase_ + total, T{});
+    if (hostCooResult_) std::fill(hostCooResult_, hostCooResult_ + localSize_, T{});
+    hostLeft_ = hostBase_;
+    hostLocal_ = hostBase_ ? hostBase_ + haloLeft_ : nullptr;
+    hostRight_ = hostBase_ ? hostBase_ + haloLeft_ + localSize_ : nullptr;
   }
 }
 
 template <typename T>
 void HBDIAVector<T>::cleanupUnifiedMemory() noexcept {
-  if (unifiedMemory_ && (unifiedMemoryMalloc_ || unifiedNumaAllocOnNode_) && !unifiedMemoryManagedMalloc_) {
-    if (data_ptr_d_) (void)hbdia_free_host(data_ptr_d_);
-    if (coo_result_ptr_d_) (void)hbdia_free_host(coo_result_ptr_d_);
+  if (unified_ && (unifiedPinned_ || unifiedNuma_) && !unifiedManaged_) {
+    if (devBase_) (void)hbdia_free_host(devBase_);
+    if (devCooResult_) (void)hbdia_free_host(devCooResult_);
   } else {
-    if (data_ptr_d_) (void)hbdia_free(data_ptr_d_);
-    if (coo_result_ptr_d_) (void)hbdia_free(coo_result_ptr_d_);
+    if (devBase_) (void)hbdia_free(devBase_);
+    if (devCooResult_) (void)hbdia_free(devCooResult_);
   }
-  if (data_ptr_h_) (void)hbdia_free_host(data_ptr_h_);
-  if (coo_result_ptr_h_) (void)hbdia_free_host(coo_result_ptr_h_);
-  data_ptr_d_ = left_ptr_d_ = local_ptr_d_ = right_ptr_d_ = coo_result_ptr_d_ = nullptr;
-  data_ptr_h_ = left_ptr_h_ = local_ptr_h_ = right_ptr_h_ = coo_result_ptr_h_ = nullptr;
-  unified_data_ptr_ = unified_local_ptr_ = nullptr;
+  if (hostBase_) (void)hbdia_free_host(hostBase_);
+  if (hostCooResult_) (void)hbdia_free_host(hostCooResult_);
+  devBase_ = devLeft_ = devLocal_ = devRight_ = devCooResult_ = nullptr;
+  hostBase_ = hostLeft_ = hostLocal_ = hostRight_ = hostCooResult_ = nullptr;
+  unifiedBase_ = unifiedLocal_ = nullptr;
 }
 
 template <typename T>
 void HBDIAVector<T>::upload(const std::vector<T>& data) {
-  if (data.size() != size_local_)
+  if (data.size() != localSize_)
     throw std::invalid_argument("HBDIAVector::upload: size " + std::to_string(data.size()) +
-                                " != local size " + std::to_string(size_local_));
-  localVector_ = data;
-  CHECK_HBDIA(hbdia_memcpy(local_ptr_d_, data.data(), size_local_ * sizeof(T)));
+                                " != local size " + std::to_string(localSize_));
+  hostInit_ = data;
+  CHECK_HBDIA(hbdia_memcpy(devLocal_, data.data(), localSize_ * sizeof(T)));
 }
 
 template <typename T>
 void HBDIAVector<T>::uploadHalo(const std::vector<T>& left, const std::vector<T>& right) {
-  if (left.size() != size_recv_left_ || right.size() != size_recv_right_)
+  if (left.size() != haloLeft_ || right.size() != haloRight_)
     throw std::invalid_argument("HBDIAVector::uploadHalo: halo sizes " + std::to_string(left.size()) + "+" +
-                                std::to_string(right.size()) + " != " + std::to_string(size_recv_left_) + "+" +
-                                std::to_string(size_recv_right_));
+                                std::to_string(right.size()) + " != " + std::to_string(haloLeft_) + "+" +
+                                std::to_string(haloRight_));
   if (!left.empty()) {
-    if (left_ptr_h_) std::copy(left.begin(), left.end(), left_ptr_h_);
-    CHECK_HBDIA(hbdia_memcpy(left_ptr_d_, left.data(), left.size() * sizeof(T)));
+    if (hostLeft_) std::copy(left.begin(), left.end(), hostLeft_);
+    CHECK_HBDIA(hbdia_memcpy(devLeft_, left.data(), left.size() * sizeof(T)));
   }
   if (!right.empty()) {
-    if (right_ptr_h_) std::copy(right.begin(), right.end(), right_ptr_h_);
-    CHECK_HBDIA(hbdia_memcpy(right_ptr_d_, right.data(), right.size() * sizeof(T)));
+    if (hostRight_) std::copy(right.begin(), right.end(), hostRight_);
+    CHECK_HBDIA(hbdia_memcpy(devRight_, right.data(), right.size() * sizeof(T)));
   }
 }
 
 template <typename T>
 void HBDIAVector<T>::download(std::vector<T>& out) const {
-  out.resize(size_local_);
-  CHECK_HBDIA(hbdia_memcpy(out.data(), local_ptr_d_, size_local_ * sizeof(T)));
+  out.resize(localSize_);
+  CHECK_HBDIA(hbdia_memcpy(out.data(), devLocal_, localSize_ * sizeof(T)));
 }
 
 template <typename T>
 std::vector<T> HBDIAVector<T>::downloadAll() const {
   std::vector<T> out(getTotalSize());
-  if (!out.empty()) CHECK_HBDIA(hbdia_memcpy(out.data(), data_ptr_d_, out.size() * sizeof(T)));
+  if (!out.empty()) CHECK_HBDIA(hbdia_memcpy(out.data(), devBase_, out.size() * sizeof(T)));
   return out;
 }
 
@@ -201,8 +201,8 @@ std::vector<T> HBDIAVector<T>::download() const {
 template <typename T>
 void HBDIAVector<T>::print(const std::string& vectorName) const {
   std::vector<T> v = download();
-  std::cout << vectorName << " (" << size_local_ << " local, halo " << size_recv_left_ << "+"
-            << size_recv_right_ << "):";
+  std::cout << vectorName << " (" << localSize_ << " local, halo " << haloLeft_ << "+"
+            << haloRight_ << "):";
   const size_t n = std::min<size_t>(v.size(), 32);
   for (size_t i = 0; i < n; ++i) std::cout << " " << v[i];
   if (n < v.size()) std::cout << " ...";

@@ -38,32 +38,32 @@ class HBDIAVector {
   HBDIAVector(HBDIAVector&& o) noexcept;
   HBDIAVector& operator=(HBDIAVector&& o) noexcept;
 
-  const std::vector<T>& getLocalVector() const { return localVector_; }
-  void setLocalVector(const std::vector<T>& data) { localVector_ = data; }
+  const std::vector<T>& getLocalVector() const { return hostInit_; }
+  void setLocalVector(const std::vector<T>& data) { hostInit_ = data; }
 
-  T* getLocalDataPtr() const { return unified_local_ptr_; }
-  size_t getLocalSize() const { return size_local_; }
-  size_t getTotalSize() const { return size_recv_left_ + size_local_ + size_recv_right_; }
+  T* getLocalDataPtr() const { return unifiedLocal_; }
+  size_t getLocalSize() const { return localSize_; }
+  size_t getTotalSize() const { return haloLeft_ + localSize_ + haloRight_; }
 
-  T* getUnifiedDataPtr() const { return unified_data_ptr_; }
-  T* getDeviceDataPtr() const { return data_ptr_d_; }
-  T* getDeviceLocalPtr() const { return local_ptr_d_; }
-  T* getDeviceLeftPtr() const { return left_ptr_d_; }
-  T* getDeviceRightPtr() const { return right_ptr_d_; }
-  T* getDeviceCOOResultsPtr() const { return coo_result_ptr_d_; }
-  T* getHostDataPtr() const { return data_ptr_h_; }
-  T* getHostLocalPtr() const { return local_ptr_h_; }
-  T* getHostLeftPtr() const { return left_ptr_h_; }
-  T* getHostRightPtr() const { return right_ptr_h_; }
-  T* getHostCOOResultsPtr() const { return coo_result_ptr_h_; }
+  T* getUnifiedDataPtr() const { return unifiedBase_; }
+  T* getDeviceDataPtr() const { return devBase_; }
+  T* getDeviceLocalPtr() const { return devLocal_; }
+  T* getDeviceLeftPtr() const { return devLeft_; }
+  T* getDeviceRightPtr() const { return devRight_; }
+  T* getDeviceCOOResultsPtr() const { return devCooResult_; }
+  T* getHostDataPtr() const { return hostBase_; }
+  T* getHostLocalPtr() const { return hostLocal_; }
+  T* getHostLeftPtr() const { return hostLeft_; }
+  T* getHostRightPtr() const { return hostRight_; }
+  T* getHostCOOResultsPtr() const { return hostCooResult_; }
 
-  bool isUnifiedMemory() const { return unifiedMemory_; }
-  bool isUnifiedMemoryMalloc() const { return unifiedMemoryMalloc_; }
-  bool isUnifiedMemoryManagedMalloc() const { return unifiedMemoryManagedMalloc_; }
-  bool isUnifiedNumaAllocOnNode() const { return unifiedNumaAllocOnNode_; }
+  bool isUnifiedMemory() const { return unified_; }
+  bool isUnifiedMemoryMalloc() const { return unifiedPinned_; }
+  bool isUnifiedMemoryManagedMalloc() const { return unifiedManaged_; }
+  bool isUnifiedNumaAllocOnNode() const { return unifiedNuma_; }
 
-  size_t getLeftSize() const { return size_recv_left_; }
-  size_t getRightSize() const { return size_recv_right_; }
+  size_t getLeftSize() const { return haloLeft_; }
+  size_t getRightSize() const { return haloRight_; }
 
   // Synchronous copies of the local segment.
   void upload(const std::vector<T>& data);
@@ -82,30 +82,30 @@ class HBDIAVector {
   void cleanupUnifiedMemory() noexcept;
   void moveFrom(HBDIAVector& o) noexcept;
 
-  std::vector<T> localVector_;
-  size_t size_recv_left_ = 0;
-  size_t size_recv_right_ = 0;
-  size_t size_local_ = 0;
+  std::vector<T> hostInit_;
+  size_t haloLeft_ = 0;
+  size_t haloRight_ = 0;
+  size_t localSize_ = 0;
 
-  bool unifiedMemory_ = false;
-  bool unifiedMemoryMalloc_ = false;
-  bool unifiedMemoryManagedMalloc_ = false;
-  bool unifiedNumaAllocOnNode_ = false;
+  bool unified_ = false;
+  bool unifiedPinned_ = false;
+  bool unifiedManaged_ = false;
+  bool unifiedNuma_ = false;
 
-  T* unified_data_ptr_ = nullptr;
-  T* unified_local_ptr_ = nullptr;
+  T* unifiedBase_ = nullptr;
+  T* unifiedLocal_ = nullptr;
 
-  T* data_ptr_d_ = nullptr;
-  T* left_ptr_d_ = nullptr;
-  T* local_ptr_d_ = nullptr;
-  T* right_ptr_d_ = nullptr;
-  T* coo_result_ptr_d_ = nullptr;
+  T* devBase_ = nullptr;
+  T* devLeft_ = nullptr;
+  T* devLocal_ = nullptr;
+  T* devRight_ = nullptr;
+  T* devCooResult_ = nullptr;
 
-  T* data_ptr_h_ = nullptr;
-  T* left_ptr_h_ = nullptr;
-  T* local_ptr_h_ = nullptr;
-  T* right_ptr_h_ = nullptr;
-  T* coo_result_ptr_h_ = nullptr;
+  T* hostBase_ = nullptr;
+  T* hostLeft_ = nullptr;
+  T* hostLocal_ = nullptr;
+  T* hostRight_ = nullptr;
+  T* hostCooResult_ = nullptr;
 };
 
 #endif  // HBDIAVECTOR_HPP

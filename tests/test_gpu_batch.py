@@ -144,18 +144,33 @@ def test_autotune_picks_a_candidate_and_stays_exact(gpu):
         np.testing.assert_array_equal(y[B.row_offsets[i]:B.row_offsets[i + 1]], oracle_y(m, xs[i], y0s[i]))
 
 
-def test_full_size_set_against_per_matrix_launches(gpu):
-    # 2048 matrices (rows log-uniform in [1e3, 1e4]): one launch equals 2048 single launches
-    print("case: batch 2048 matrices", flush=True)
-    ms = HBDIA.banded_batch(count=2048, max_rows=10000, seed=0xC4B)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_configured_2048_set_against_the_oracle(gpu, dtype):
+    """The batching workload at its configured size: 2048 banded matrices with rows log-uniform in
+    [1e3, 1e5] (44.6M rows), 3-9 offsets in [-64, 64] and 1 % off-band entries, block width 64,
+    threshold 2, in one launch.  Every matrix equals its own per-matrix launch; a seeded sample of 40
+    matrices -- the largest and the smallest among them -- equals the oracle bit for bit."""
+    print(f"case: batch 2048 configured set {np.dtype(dtype).name}", flush=True)
+    ms = HBDIA.banded_batch(count=2048, seed=0xC4B, dtype=dtype)
     for m in ms:
         m.convertToHBDIAFormatOnHost(64, 2)
+    rows = np.array([m.getNumRows() for m in ms])
+    assert rows.min() >= 1000 and rows.max() > 90000 and rows.sum() > 40_000_000
     B = HBDIABatch(ms)
     n = B.rows
-    X = DeviceArray.from_host(gen_vector(n, 1))
-    Y1 = DeviceArray(n, np.float64, zero=True)
-    Y2 = DeviceArray(n, np.float64, zero=True)
+    x = gen_vector(n, 1, dtype=dtype)
+    X = DeviceArray.from_host(x)
+    Y1 = DeviceArray(n, dtype, zero=True)
+    Y2 = DeviceArray(n, dtype, zero=True)
     B.spmv(X, Y1)
     B.spmv_per_matrix(X, Y2)
     B.synchronize()
-    np.testing.assert_array_equal(Y1.download(), Y2.download())
+    y = Y1.download()
+    assert y.tobytes() == Y2.download().tobytes()
+    rng = np.random.default_rng(0xC4B)
+    sample = set(int(i) for i in rng.choice(len(ms), 38, replace=False))
+    sample |= {int(rows.argmax()), int(rows.argmin())}
+    for i in sorted(sample):
+        lo, hi = B.row_offsets[i], B.row_offsets[i + 1]
+        want = oracle_y(ms[i], x[lo:hi], np.zeros(hi - lo, dtype))
+        assert y[lo:hi].tobytes() == want.tobytes(), f"matrix {i} ({rows[i]} rows) differs from the oracle"

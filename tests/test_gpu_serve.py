@@ -158,3 +158,55 @@ def test_open_loop_driver_reports_latencies(gpu):
     lat0, _ = S.open_loop(0.0, 100, 4, flat, 2, ys)  # burst
     assert np.all(lat0 > 0)
     S.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_serving_pool_of_configured_shapes(gpu, dtype):
+    """A pool shaped like the serving workload's: 27-point stencils at 32^3, 48^3 and 64^3, a 7-point
+    stencil at 40^3 and banded matrices of the batching set (up to 1e5 rows), widths 64 / 128.
+    Every request's y equals the oracle's alone and in a 4-thread burst of 120 requests."""
+    print(f"case: serve configured shapes {np.dtype(dtype).name}", flush=True)
+    ms = [HBDIA.stencil(32, 27, 0.0, seed=11, dtype=dtype), HBDIA.stencil(48, 27, 0.005, seed=12, dtype=dtype),
+          HBDIA.stencil(64, 27, 0.0, seed=13, dtype=dtype), HBDIA.stencil(40, 7, 0.0, seed=14, dtype=dtype)]
+    ms += HBDIA.banded_batch(count=4, min_rows=20000, max_rows=100000, seed=15, dtype=dtype)
+    for i, m in enumerate(ms):
+        m.convertToHBDIAFormatOnHost([64, 128][i % 2], 2)
+    xs = xs_for(ms, dtype, variants=2)
+    want = {(m, v): expected(ms[m], xs[m][v]) for m in range(len(ms)) for v in range(2)}
+    S = HBDIAServer(ms, max_jobs=32, max_in_flight=2)
+    for (m, v), w in want.items():
+        y = np.full(ms[m].getNumRows(), np.nan, dtype)
+        S.wait(S.submit(m, xs[m][v], y))
+        assert y.tobytes() == w.tobytes(), f"matrix {m} alone differs from the oracle"
+    rng = np.random.default_rng(21)
+    picks = [(int(rng.integers(len(ms))), int(rng.integers(2))) for _ in range(120)]
+    outs = [np.full(ms[m].getNumRows(), np.nan, dtype) for m, _ in picks]
+
+    def submitter(k):
+        for i in range(k, len(picks), 4):
+            m, v = picks[i]
+            S.submit(m, xs[m][v], outs[i])
+
+    th = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    S.drain()
+    for i, (m, v) in enumerate(picks):
+        assert outs[i].tobytes() == want[m, v].tobytes(), f"request {i} (matrix {m}) differs in the mix"
+    assert S.info()["largest_launch"] > 1
+    S.close()
+
+
+def test_submit_rejects_short_vectors(gpu):
+    ms = pool()
+    S = HBDIAServer(ms)
+    n = S.num_rows(2)
+    with pytest.raises(ValueError):
+        S.submit(2, np.zeros(n - 1), np.zeros(n))
+    with pytest.raises(ValueError):
+        S.submit(2, np.zeros(n), np.zeros(n - 1))
+    with pytest.raises(TypeError):
+        S.submit(2, np.zeros(n, np.float32), np.zeros(n))
+    S.close()
