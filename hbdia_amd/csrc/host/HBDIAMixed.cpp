@@ -13,7 +13,8 @@ int vdtype(StoragePrecision p) { return p == StoragePrecision::FP32 ? HBDIA_VALU
 double HBDIAMixed::effectiveUnitRoundoff(StoragePrecision precision, int64_t terms) {
   const double u24 = 0x1p-24;
   const double store = precision == StoragePrecision::FP32 ? u24 : 0x1p-8 + u24;
-  return store + static_cast<double>(terms + 2) * u24;
+  // the kernels evaluate the bound in FP32 and widen it by 1.0001 for that rounding
+  return (store + static_cast<double>(terms + 2) * u24) * 1.0001;
 }
 
 int64_t HBDIAMixed::maxRowTerms(const HBDIA<double>& matrix) {

@@ -10,8 +10,8 @@
 // hbdiaSpMV's GPU branch.  The ratio is taken against the smallest |y|
 // consistent with the FP32 sum and its error bound, so a kept row of n terms
 // has relative error at most u_eff(n) * threshold, with
-// u_eff(n) = u_store + (n + 2) 2^-24 (u_store = 2^-24 for FP32, 2^-8 + 2^-24
-// for BF16).  relErrorBound() states that figure for the matrix's longest row.
+// u_eff(n) = 1.0001 (u_store + (n + 2) 2^-24) (u_store = 2^-24 for FP32,
+// 2^-8 + 2^-24 for BF16; the factor covers the FP32 evaluation of the bound).  relErrorBound() states that figure for the matrix's longest row.
 // Without an explicit threshold the constructor derives it from a target
 // relative error (kDefaultRelTolerance): FP32 storage then keeps nearly every
 // row, BF16 storage recomputes most of them, since BF16 values alone carry a
