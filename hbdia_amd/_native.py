@@ -94,6 +94,8 @@ _HIP_SIGS = {
                                      c_int64, _p, _p]),
     "hbdia_hbdia_spmv_mixed_ex": (_i, [c_int, _p, _p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, _p, c_int64,
                                        c_double, _p, _p, _p]),
+    "hbdia_hbdia_spmv_mixed_noalloc": (_i, [c_int, _p, _p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, _p,
+                                            c_int64, c_double, _p, _p]),
     "hbdia_hbdia_spmv_mixed": (_i, [c_int, _p, _p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, _p, c_int64,
                                     c_double, _p, _p]),
     "hbdia_coo_spmv_f64": (_i, [_p, _p, _p, _p, _p, _p, c_int64, _p]),

@@ -21,10 +21,12 @@
 //     writes y exactly once with no separate vector-add pass.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -3051,6 +3053,67 @@ int hbdia_hbdia_spmv_mixed_ex(int vdtype, const void* values_lp, const double* v
                       coo_values_lp, coo_values_f64, coo_group_ptr, x, y, num_rows, ratio, workspace, stream,
                       runs3_blocks);
 }
+}  // extern "C"
+
+// Scratch of the allocation-free reduced-precision product: one 64-bit FP64
+// mask per 64-row group (the workspace of hbdia_hbdia_spmv_mixed_ex), 2 MiB
+// for up to HBDIA_NOALLOC_MAX_ROWS rows.  Each device holds its own copy for
+// the life of the process.  In host code `scratch` names a host-side stand-in:
+// its device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) uint64_t scratch[HBDIA_NOALLOC_MAX_ROWS / HBDIA_ROW_GROUP + 1];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 256;
+std::atomic<uint64_t*> g_scratch_dev[kMaxScratchDevices];
+std::mutex g_scratch_mu;
+
+// The current device's copy of `scratch`, resolved on the first call on that device.
+int scratch_ptr(uint64_t** out, const char* name) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices)
+    return fail(HBDIA_ERROR_DEVICE, std::string(name) + ": no current device");
+  uint64_t* p = g_scratch_dev[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    p = g_scratch_dev[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      void* addr = nullptr;
+      const int rc = check_hip(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)), name);
+      if (rc != HBDIA_SUCCESS) return rc;
+      if (!addr) return fail(HBDIA_ERROR_DEVICE, std::string(name) + ": no device address for the scratch array");
+      p = static_cast<uint64_t*>(addr);
+      g_scratch_dev[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return HBDIA_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hbdia_hbdia_spmv_mixed_noalloc(int vdtype, const void* values_lp, const double* values_f64,
+                                   const int32_t* offsets, const int32_t* block_start, int32_t num_blocks,
+                                   int32_t block_width, const int32_t* coo_rows, const int32_t* coo_cols,
+                                   const void* coo_values_lp, const double* coo_values_f64,
+                                   const int32_t* coo_group_ptr, const double* x, double* y, int64_t num_rows,
+                                   double ratio, const uint32_t* runs3_blocks, hbdia_stream_t stream) {
+  const char* name = "hbdia_hbdia_spmv_mixed_noalloc";
+  if (num_rows > HBDIA_NOALLOC_MAX_ROWS)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT,
+                std::string(name) + ": num_rows exceeds HBDIA_NOALLOC_MAX_ROWS (" +
+                    std::to_string(HBDIA_NOALLOC_MAX_ROWS) + "); use hbdia_hbdia_spmv_mixed_ex with a workspace");
+  uint64_t* masks = nullptr;
+  const int rc = scratch_ptr(&masks, name);
+  if (rc != HBDIA_SUCCESS) return rc;
+  // every launch rewrites the mask of every group it covers: nothing to clear
+  return launch_mixed(vdtype, values_lp, values_f64, offsets, block_start, num_blocks, block_width, coo_rows, coo_cols,
+                      coo_values_lp, coo_values_f64, coo_group_ptr, x, y, num_rows, ratio, masks, stream,
+                      runs3_blocks);
+}
+
 int hbdia_coo_spmv_f64(const int32_t* coo_rows, const int32_t* coo_cols, const double* coo_values,
                        const int32_t* coo_group_ptr, const double* x, double* res,
                        int64_t num_rows, hbdia_stream_t stream) {

@@ -215,6 +215,24 @@ int hbdia_hbdia_spmv_mixed_ex(int vdtype, const void* values_lp, const double* v
                               double* y, int64_t num_rows, double ratio, void* workspace,
                               const uint32_t* runs3_blocks, hbdia_stream_t stream);
 
+/* Allocation-free form of hbdia_hbdia_spmv_mixed_ex (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same product and results, with the per-group FP64 masks kept in a
+ * library-owned device array (2 MiB per device, held until the process ends)
+ * instead of a workspace argument; the masks are not returned.  It calls no
+ * allocating routine; the first call on a device resolves that device's copy
+ * of the array.  num_rows <= HBDIA_NOALLOC_MAX_ROWS, else
+ * HBDIA_ERROR_INVALID_ARGUMENT.  All calls on one device share the array:
+ * calls that overlap on one device (other streams, other threads) race. */
+#define HBDIA_NOALLOC_MAX_ROWS (1LL << 24)
+int hbdia_hbdia_spmv_mixed_noalloc(int vdtype, const void* values_lp, const double* values_f64,
+                                   const int32_t* offsets, const int32_t* block_start, int32_t num_blocks,
+                                   int32_t block_width, const int32_t* coo_rows, const int32_t* coo_cols,
+                                   const void* coo_values_lp, const double* coo_values_f64,
+                                   const int32_t* coo_group_ptr, const double* x, double* y, int64_t num_rows,
+                                   double ratio, const uint32_t* runs3_blocks, hbdia_stream_t stream);
+
 /* Many independent matrices in one persistent launch (no reference
  * counterpart: the reference multiplies one matrix per call).  The matrices'
  * HBDIA arrays are concatenated into pools (values, offsets, fallback rows /
