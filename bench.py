@@ -35,6 +35,10 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# The timed CPU baseline runs with the reference's thread placement; libgomp reads this when the
+# native libraries load, which happens after this line.
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+
 HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = ROOT / "profiles" / "traffic_hbdia_spmv_stencil256.json"
 
@@ -55,8 +59,20 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full-size parity check")
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64",
                    help="value, x and y type of the c2, ref128 and multirhs workloads (HBDIA<double> or HBDIA<float>)")
-    p.add_argument("--cpu-reps", type=int, default=5)
+    p.add_argument("--no-forms", action="store_true",
+                   help="default workload only: skip the forms (ref128, precision, multirhs, batch, serve)")
     return p.parse_args()
+
+
+class BenchError(Exception):
+    """A parity or consistency failure inside a workload; its argument is the JSON-able record."""
+
+
+def _emit(res) -> int:
+    if isinstance(res, int):
+        return res
+    print(json.dumps(res), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -64,21 +80,86 @@ def main() -> int:
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
-    if args.workload == "precision":
-        return bench_precision(args)
-    if args.workload == "multirhs":
-        return bench_multirhs(args)
-    if args.workload == "batch":
-        return bench_batch(args)
-    if args.workload == "serve":
-        return bench_serve(args)
-    if args.workload == "copy":
-        return bench_copy(args)
-    if args.workload == "convert":
-        return bench_convert(args)
-    if args.workload == "ranklocal":
-        return bench_ranklocal(args)
+    legs = {"precision": bench_precision, "multirhs": bench_multirhs, "batch": bench_batch, "serve": bench_serve,
+            "copy": bench_copy, "convert": bench_convert, "ranklocal": bench_ranklocal}
+    try:
+        if args.workload in legs:
+            return _emit(legs[args.workload](args))
+        line = bench_spmv(args)
+    except BenchError as e:
+        print(json.dumps(e.args[0]), flush=True)
+        return 4
+    if isinstance(line, int):
+        return line
+    if args.workload == "c2" and not args.no_forms and args.dtype == "f64" and args.grid is None:
+        line["forms"] = bench_forms(args)
+    print(json.dumps(line), flush=True)
+    return 0
 
+
+def _reference_method(A, X, Y, gpu_coo: bool, iterations: int = 100) -> dict:
+    """The reference's own timing (benchmarking/benchHBDIA.cpp:30-50): per round y cleared, device
+    synchronized, hbdiaSpMV + device synchronize timed on the host clock; round 0 dropped and the mean
+    of the rest taken, as its plots do (benchmarking/Plotting/plot_hbdia_vs_cusparse_runtime.py:30-32,
+    :59-61)."""
+    from hbdia_amd.api import spmv_time_synced
+    t = spmv_time_synced(A, X, Y, False, gpu_coo, iterations)[1:]
+    return {"iterations": iterations, "dropped": 1, "mean_ms": round(float(t.mean()), 5),
+            "median_ms": round(float(np.median(t)), 5), "min_ms": round(float(t.min()), 5),
+            "max_ms": round(float(t.max()), 5)}
+
+
+def _cpu_baseline(hb: dict, x: np.ndarray, N: int, flop: float, dt, what: str) -> dict:
+    """The reference path on the host cores: the oracle's restatement (bdia_spmv_kernel per row with
+    OpenMP, cpu_coo_spmv_kernel, y += coo) compiled with the reference's host flags on this machine,
+    OMP_PROC_BIND=spread, 2 warm-ups then the median of 10, on the same matrix and x."""
+    import oracle
+    try:
+        base = oracle.CpuBaseline()
+        build = f"{oracle.CpuBaseline.FLAGS} (reference CMakeLists.txt:39), built on this host"
+        run = base.hbdia_spmv
+        cores = base.max_threads()
+    except RuntimeError as e:  # no compiler on this host: the checker's build, said so
+        log(f"cpu baseline build failed ({e}); timing the oracle's own build")
+        build = "-O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (oracle build; the reference flags failed to build)"
+        run = lambda hb_, x_, y_, n_, c_: oracle.hbdia_spmv(hb_, x_, y_, n_, c_)  # noqa: E731
+        cores = oracle.max_threads()
+    y = np.zeros(N, dt)
+    coo_buf = np.zeros(N, dt)
+    times = []
+    for rep in range(12):
+        y[:] = 0.0
+        ts = time.perf_counter()
+        run(hb, x, y, N, coo_buf)
+        te = time.perf_counter()
+        if rep >= 2:
+            times.append(te - ts)
+    t_cpu = float(np.median(times))
+    return {"value": round(flop / t_cpu / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+            "ms_per_spmv": round(t_cpu * 1e3, 3), "build": build,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+            "sample": f"{what}: median of 10 SpMVs after 2 warm-ups (oracle restatement of bdia_spmv_kernel + "
+                      f"cpu_coo_spmv_kernel + add, OpenMP)"}
+
+
+def _traffic(kernel_ms: float) -> int | None:
+    """HBM bytes per launch of the c2 kernel from the committed rocprofv3 counter runs, used only while
+    the kernel source is the one those runs measured (its hash is recorded beside the figure)."""
+    import hashlib
+    try:
+        rec = json.loads(PROFILE_TRAFFIC.read_text())
+    except Exception:
+        return None
+    src = ROOT / "hbdia_amd" / "csrc" / "hip" / "hbdia_hip.hip"
+    digest = hashlib.sha256(src.read_bytes()).hexdigest()[:16]
+    if rec.get("source_sha256_16") != digest:
+        log(f"traffic figure not used: profiled source {rec.get('source_sha256_16')} != current {digest}")
+        return None
+    return rec.get("hbm_bytes_per_launch")
+
+
+def bench_spmv(args) -> dict | int:
+    """The drop-in product: c2 (the headline) or ref128 (the reference's published configuration)."""
     import hbdia_amd
     from hbdia_amd import HBDIA, Event, HBDIAVector, gen_vector, hbdiaSpMV
     from hbdia_amd._native import check_hip, hip
@@ -125,11 +206,10 @@ def main() -> int:
 
     X = HBDIAVector(x)
     Y = HBDIAVector(np.zeros(N, dt))
-    use_cpu = False
     gpu_coo = nnz_coo > 0
 
     def step():
-        hbdiaSpMV(A, X, Y, use_cpu, gpu_coo)
+        hbdiaSpMV(A, X, Y, False, gpu_coo)
 
     step()
     A.synchronize()
@@ -141,31 +221,15 @@ def main() -> int:
         import oracle  # the checker and the CPU baseline only
         hb = A.getHBDIA()
         hb["num_blocks"], hb["block_width"] = A.getNumBlocks(), bw
-        cores = oracle.max_threads()
-        y_cpu = np.zeros(N, dt)
-        coo_buf = np.zeros(N, dt)
-        oracle.hbdia_spmv(hb, x, y_cpu, N, coo_buf)
+        y_cpu = oracle.hbdia_spmv(hb, x, np.zeros(N, dt), N, np.zeros(N, dt))
         diff = y_gpu != y_cpu
         rel = np.abs(y_gpu - y_cpu) / np.maximum(np.abs(y_cpu), 1e-300)
         parity = {"rows": N, "rows_differing": int(diff.sum()), "max_rel_err": float(rel.max()),
                   "bit_exact": bool(not diff.any())}
         log(f"parity vs oracle at full size: {parity}")
         if rel.max() > 1e-12:
-            print(json.dumps({"error": "parity failure", "parity": parity}))
-            return 4
-        times = []
-        for rep in range(args.cpu_reps + 1):
-            y_cpu[:] = 0.0
-            ts = time.perf_counter()
-            oracle.hbdia_spmv(hb, x, y_cpu, N, coo_buf)
-            te = time.perf_counter()
-            if rep > 0:
-                times.append(te - ts)
-        t_cpu = float(np.median(times))
-        cpu_baseline = {"value": round(flop / t_cpu / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-                        "ms_per_spmv": round(t_cpu * 1e3, 3),
-                        "sample": f"full matrix, median of {args.cpu_reps} SpMVs after 1 warm-up "
-                                  f"(oracle restatement of bdia_spmv_kernel + cpu_coo_spmv_kernel + add, OpenMP)"}
+            raise BenchError({"error": "parity failure", "parity": parity})
+        cpu_baseline = _cpu_baseline(hb, x, N, flop, dt, "full matrix")
         log(f"cpu baseline: {cpu_baseline}")
         del hb
 
@@ -188,19 +252,19 @@ def main() -> int:
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     log(f"{args.steps} steps: {wall_ms:.4f} ms/step wall, {kern_ms:.4f} ms/launch on the stream, "
         f"{achieved:.1f} GB/s algorithmic")
+    refm = _reference_method(A, X, Y, gpu_coo)
+    log(f"reference timing method (synchronized per call, 100 rounds, first dropped): {refm}")
 
     traffic = None
-    if PROFILE_TRAFFIC.exists() and args.workload == "c2" and (args.grid or 256) == 256 and args.dtype == "f64":
-        try:
-            traffic = json.loads(PROFILE_TRAFFIC.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    if args.workload == "c2" and (args.grid or 256) == 256 and args.dtype == "f64":
+        traffic = _traffic(kern_ms)
 
     value = flop / (wall_ms * 1e-3) / 1e9
     vs = None
     if args.workload == "ref128" and (args.grid or 128) == 128 and args.dtype == "f64":
-        vs = round(value / (flop / 0.40e-3 / 1e9), 3)  # reference figure: 0.40 ms on a GH200-class GPU
-    line = {
+        # the reference's figure, 0.40 ms on a GH200-class GPU, is its own method's mean
+        vs = round(0.40 / refm["mean_ms"], 3)
+    return {
         "metric": f"SpMV GFLOP/s ({args.dtype.upper().replace('F', 'FP')} hbdiaSpMV) + % HBM roofline",
         "value": round(value, 2),
         "unit": "GFLOP/s",
@@ -218,6 +282,7 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu_baseline,
         "kernel_ms": round(kern_ms, 5),
+        "reference_method": refm,
         "algorithmic_bytes": int(alg_bytes),
         "stored_bytes": int(stored_bytes),
         "flop": int(flop),
@@ -226,8 +291,51 @@ def main() -> int:
         "parity": parity,
         "setup_s": round(time.time() - t0, 1),
     }
-    print(json.dumps(line), flush=True)
-    return 0
+
+
+def bench_forms(args) -> dict:
+    """The other halves of the metric, each at its configured workload, run after the headline in
+    the default invocation: the reference's configuration with its own timing method, the
+    reduced-precision, multi-RHS and batched forms (each with its algorithmic bytes and roofline
+    fraction) and the served SpMVs/s with p50/p99 latency.  A leg that fails records its error."""
+    import copy
+    out = {}
+
+    def leg(name, fn, steps, warmup=3, **over):
+        a = copy.copy(args)
+        a.steps, a.warmup, a.grid, a.count = steps, warmup, None, None
+        for k, v in over.items():
+            setattr(a, k, v)
+        t = time.time()
+        try:
+            r = fn(a)
+        except BenchError as e:
+            r = {"error": e.args[0]}
+        except Exception as e:  # keep the headline line whatever a form does
+            r = {"error": f"{type(e).__name__}: {e}"}
+        log(f"form {name} done in {time.time() - t:.1f}s")
+        return r
+
+    r = leg("ref128", bench_spmv, 50, 5, workload="ref128")
+    if isinstance(r, dict) and "error" not in r:
+        out["ref128"] = {"workload": r["config"]["workload"], "kernel_ms": r["kernel_ms"],
+                         "ms_per_step_pipelined": r["ms_per_step"], "reference_method": r["reference_method"],
+                         "vs_reference_0.40ms": r["vs_baseline"], "gflops": r["value"],
+                         "algorithmic_bytes": r["algorithmic_bytes"], "roofline_frac": r["roofline"]["frac"],
+                         "parity": r["parity"], "cpu_baseline": r["cpu_baseline"]}
+    else:
+        out["ref128"] = r
+    r = leg("precision", bench_precision, 20)
+    out["precision"] = r.get("forms", r) if isinstance(r, dict) else r
+    if isinstance(r, dict) and "config" in r:
+        out["precision"] = {"config": r["config"], **r["forms"]}
+    r = leg("multirhs", bench_multirhs, 20)
+    out["multirhs"] = {"config": r["config"], "by_k": r["by_k"]} if isinstance(r, dict) and "by_k" in r else r
+    r = leg("batch", bench_batch, 20)
+    out["batch"] = {"config": r["config"], **r["forms"]} if isinstance(r, dict) and "forms" in r else r
+    r = leg("serve", bench_serve, 0)
+    out["serve"] = {"config": r["config"], "results": r["results"]} if isinstance(r, dict) and "results" in r else r
+    return out
 
 
 def _time_launches(fn, sync, stream, steps: int, warmup: int):
@@ -250,7 +358,7 @@ def _time_launches(fn, sync, stream, steps: int, warmup: int):
     return (te - ts) * 1e3 / steps, ev0.elapsed_ms(ev1) / steps
 
 
-def bench_precision(args) -> int:
+def bench_precision(args) -> dict:
     """FP64 / FP32 / BF16 value storage on the 192^3 stencil + scatter matrix (x, y FP64).
 
     Reports, for each storage form, the time per SpMV, algorithmic GB/s (s_v = 8, 4, 2),
@@ -282,14 +390,14 @@ def bench_precision(args) -> int:
     hb["num_blocks"], hb["block_width"] = A.getNumBlocks(), 64
     ycpu = oracle.hbdia_spmv(hb, x, np.zeros(N), N)
     if not np.array_equal(ycpu, y64):
-        print(json.dumps({"error": "FP64 parity failure"}))
-        return 4
+        raise BenchError({"error": "FP64 parity failure"})
     del hb
     forms = {}
     wall, kern = _time_launches(lambda: hbdiaSpMV(A, X, Y, False, True), A.synchronize, stream, args.steps, args.warmup)
     forms["fp64"] = {"ms": round(kern, 5), "gbs": round(bytes_for(8) / (kern * 1e-3) / 1e9, 1),
-                     "gflops": round(flop / (kern * 1e-3) / 1e9, 1), "max_rel_err": 0.0, "rms_rel_err": 0.0,
-                     "fp64_rows": N}
+                     "gflops": round(flop / (kern * 1e-3) / 1e9, 1), "algorithmic_bytes": int(bytes_for(8)),
+                     "roofline_frac": round(bytes_for(8) / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "max_rel_err": 0.0, "rms_rel_err": 0.0, "fp64_rows": N}
     denom = np.maximum(np.abs(y64), 1e-300)
     for prec, sv, ratio in (("fp32", 4, 1e4), ("bf16", 2, 1e4), ("bf16", 2, 64.0), ("bf16", 2, 8.0)):
         M = HBDIAMixed(A, prec, ratio)
@@ -301,7 +409,9 @@ def bench_precision(args) -> int:
         wall, kern = _time_launches(lambda: M.spmv(X, Y), A.synchronize, stream, args.steps, args.warmup)
         key = f"{prec}_ratio{ratio:g}"
         forms[key] = {"ms": round(kern, 5), "gbs": round(bytes_for(sv) / (kern * 1e-3) / 1e9, 1),
-                      "gflops": round(flop / (kern * 1e-3) / 1e9, 1), "cancellation_ratio": ratio,
+                      "gflops": round(flop / (kern * 1e-3) / 1e9, 1), "algorithmic_bytes": int(bytes_for(sv)),
+                      "roofline_frac": round(bytes_for(sv) / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "cancellation_ratio": ratio, "guaranteed_max_rel_err": M.rel_error_bound,
                       "max_rel_err": float(rel.max()), "rms_rel_err": float(np.sqrt(np.mean(rel ** 2))),
                       "p99_rel_err": float(np.quantile(rel, 0.99)), "median_rel_err": float(np.median(rel)),
                       "fp64_rows": nfix, "speedup_vs_fp64": round(forms["fp64"]["ms"] / kern, 3)}
@@ -315,11 +425,10 @@ def bench_precision(args) -> int:
             "config": {"workload": f"stencil27_scatter_{n}^3_precision", "grid": n, "rows": N, "scatter": 0.005,
                        "block_width": 64, "threshold": 2, "cancellation_ratio": 1e4, "accumulate": "fp32"},
             "forms": forms}
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def bench_multirhs(args) -> int:
+def bench_multirhs(args) -> dict:
     """k right-hand sides in one launch vs k single-vector launches, 128^3 27-point stencil."""
     from hbdia_amd import HBDIA, DeviceArray, HBDIAVector, gen_vector, hbdiaSpMV, hbdiaSpMVMulti
     n = args.grid or 128
@@ -345,12 +454,13 @@ def bench_multirhs(args) -> int:
         hbdiaSpMV(A, Xs, Ys, False, True)
         A.synchronize()
         if not np.array_equal(Y.download()[(k - 1) * N:], Ys.download()):
-            print(json.dumps({"error": f"multi-RHS column mismatch at k={k}"}))
-            return 4
+            raise BenchError({"error": f"multi-RHS column mismatch at k={k}"})
         _, tk = _time_launches(lambda: hbdiaSpMVMulti(A, X, Y, k), A.synchronize, stream, args.steps, args.warmup)
         b = nnz * s_v + k * N * 2 * s_v
         rows[k] = {"ms": round(tk, 5), "gflops": round(2.0 * nnz * k / (tk * 1e-3) / 1e9, 1),
-                   "gbs": round(b / (tk * 1e-3) / 1e9, 1), "speedup_vs_k_single": round(k * t1 / tk, 3)}
+                   "gbs": round(b / (tk * 1e-3) / 1e9, 1), "algorithmic_bytes": int(b),
+                   "roofline_frac": round(b / (tk * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "speedup_vs_k_single": round(k * t1 / tk, 3)}
         log(f"k={k}: {rows[k]}")
     line = {"metric": f"multi-RHS SpMV GFLOP/s ({args.dtype.upper().replace('F', 'FP')}), k right-hand sides in one launch", "value": rows[32]["gflops"],
             "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": rows[32]["ms"],
@@ -359,11 +469,10 @@ def bench_multirhs(args) -> int:
             "config": {"workload": f"stencil27_{n}^3_multirhs", "grid": n, "rows": N, "block_width": 64,
                        "single_vector_ms": round(t1, 5)},
             "by_k": rows}
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def bench_batch(args) -> int:
+def bench_batch(args) -> dict:
     """2048 independent banded matrices (rows log-uniform in [1e3, 1e5], 3-9 diagonal offsets in
     [-64, 64], 1% off-band entries; diagonally dominant) in one persistent launch replayed from a
     HIP graph, with block width and tile schedule chosen by the autotuner, against the plain form:
@@ -400,8 +509,7 @@ def bench_batch(args) -> int:
         B.synchronize()
         y = Y.download()
         if not np.array_equal(y, Y2.download()):
-            print(json.dumps({"error": f"{name}: batch differs from per-matrix launches"}))
-            return 4
+            raise BenchError({"error": f"{name}: batch differs from per-matrix launches"})
         off = B.row_offsets
         for i in np.linspace(0, count - 1, 8).astype(int):
             A = ms[i]
@@ -410,8 +518,7 @@ def bench_batch(args) -> int:
             n_i = A.getNumRows()
             want = oracle.hbdia_spmv(hb, x[off[i]:off[i + 1]].copy(), np.zeros(n_i, dt), n_i)
             if not np.array_equal(y[off[i]:off[i + 1]], want):
-                print(json.dumps({"error": f"{name}: matrix {i} differs from the oracle"}))
-                return 4
+                raise BenchError({"error": f"{name}: matrix {i} differs from the oracle"})
         B.capture_graph(X, Y)
         stream = B.stream_handle
         _, kern = _time_launches(B.launch_graph, B.synchronize, stream, args.steps, args.warmup)
@@ -449,8 +556,7 @@ def bench_batch(args) -> int:
             "roofline": {"bound": "hbm", "achieved": forms["fp64"]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": forms["fp64"]["roofline_frac"], "traffic": None},
             "forms": forms}
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 def _serve_pool(count: int, seed: int):
@@ -476,7 +582,7 @@ def _serve_pool(count: int, seed: int):
     return ms, desc
 
 
-def bench_serve(args) -> int:
+def bench_serve(args) -> dict:
     """Serving: a continuous-batching request loop over 64 matrices resident in HBM (see _serve_pool).
     Each request brings a fresh x (two seeded U[-1,1] vectors per matrix, pinned host memory or HBM) and
     gets y = A x.  Poisson arrivals at 30%, 60% and 90% of the server's measured capacity (a burst of
@@ -491,6 +597,13 @@ def bench_serve(args) -> int:
     t0 = time.time()
     ms, desc = _serve_pool(count, seed)
     rows = [m.getNumRows() for m in ms]
+    # algorithmic bytes of one request (uniformly random matrix): values, fallback entries with their
+    # indices, x read and y written, FP64
+    req_bytes = []
+    for m in ms:
+        mi = m.info()
+        req_bytes.append((mi["nnz"] - mi["num_coo"]) * 8 + mi["num_coo"] * 16 + 2 * mi["rows"] * 8)
+    mean_bytes = float(np.mean(req_bytes))
     log(f"{count} matrices ({sum(rows)} rows) generated and converted in {time.time() - t0:.1f}s")
     variants = 2
     xh = [PinnedArray.from_host(gen_vector(n, seed * 1000 + 2 * i + v)) for i, n in enumerate(rows)
@@ -511,8 +624,7 @@ def bench_serve(args) -> int:
             hb["num_blocks"], hb["block_width"] = m.getNumBlocks(), m.getBlockWidth()
             want = oracle.hbdia_spmv(hb, xh[i * variants].array.copy(), np.zeros(rows[i]), rows[i])
             if not np.array_equal(y, want):
-                print(json.dumps({"error": f"{name} server: matrix {i} differs from the oracle"}))
-                return 4
+                raise BenchError({"error": f"{name} server: matrix {i} differs from the oracle"})
     log("parity: every matrix bit-exact against the oracle through both servers")
 
     def stats(lat, wall, n):
@@ -532,6 +644,7 @@ def bench_serve(args) -> int:
             lat, wall = S.open_loop(0.0, nb, seed + 1, xs, variants, ys, device=dev)
             caps[name] = nb / wall
             res[f"{name}_capacity_per_s"] = round(caps[name], 1)
+            res[f"{name}_capacity_roofline_frac"] = round(caps[name] * mean_bytes / 1e9 / HBM_PEAK_GBS, 4)
             log(f"{vec} x: {name} capacity {caps[name]:.0f} SpMVs/s (burst of {nb})")
         for frac in (0.3, 0.6, 0.9):
             rate = frac * caps["continuous"]
@@ -557,13 +670,13 @@ def bench_serve(args) -> int:
             "config": {"workload": "serve_pool64", "matrices": count, "rows": int(sum(rows)),
                        "resident_bytes": info["resident_bytes"], "arena_bytes": info["arena_bytes"],
                        "pool": desc, "x_variants_per_matrix": variants, "loads": "30/60/90% of measured capacity",
+                       "request_algorithmic_bytes_mean": int(mean_bytes),
                        "error_vs_fp64_oracle": "0 (bit-exact)"},
             "results": results}
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def bench_copy(args) -> int:
+def bench_copy(args) -> dict:
     """Counter calibration: a 1 GiB device-to-device copy (hipMemcpyAsync on the null stream), the
     plain copy whose FETCH_SIZE / WRITE_SIZE readings scale the SpMV kernels' counters to bytes."""
     from hbdia_amd import DeviceArray, Event
@@ -584,11 +697,10 @@ def bench_copy(args) -> int:
             "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "zeros",
             "config": {"workload": "copy_1GiB", "bytes_read": n * 8, "bytes_written": n * 8}}
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def bench_convert(args) -> int:
+def bench_convert(args) -> dict | int:
     """COO -> HBDIA conversion of the 256^3 stencil + scatter matrix (453M entries, block width 64,
     threshold 2): the device conversion with the COO already in HBM (hbdia_convert_coo, every result
     array left in HBM), the whole convertToHBDIAFormatOnDevice (COO upload, device conversion, host
@@ -684,13 +796,12 @@ def bench_convert(args) -> int:
             "host_convert_upload_s": round(min(host_s), 3),
             "host_threads": hbdia_amd.api.capi.hbdia_get_num_threads(),
             "host_over_method": round(min(host_s) / min(method_s), 2), "identical": same, "cpu_baseline": cpu}
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 
 
-def bench_ranklocal(args) -> int:
+def bench_ranklocal(args) -> dict | int:
     """One rank of eight of the c2 matrix on one GPU: rows [3q, 4q) (q = N / 8) with global-row
     offsets, the columns owned by other ranks in the COO part, x = [left halo | local | right halo]
     filled from the global x.  Parity: bit-exact against the oracle's rank-local SpMV on the same
@@ -753,19 +864,18 @@ def bench_ranklocal(args) -> int:
                   "bit_exact": bool(not diff.any())}
         log(f"parity vs oracle (rank-local layout): {parity}")
         if rel.max() > 1e-12:
-            print(json.dumps({"error": "parity failure", "parity": parity}))
-            return 4
+            raise BenchError({"error": "parity failure", "parity": parity})
         times = []
-        for rep in range(args.cpu_reps + 1):
+        for rep in range(12):
             ts = time.perf_counter()
             oracle.rank_local_spmv(loc, x_ext, np.zeros(nl))
             te = time.perf_counter()
-            if rep > 0:
+            if rep >= 2:
                 times.append(te - ts)
         t_cpu = float(np.median(times))
         cpu_baseline = {"value": round(flop / t_cpu / 1e9, 3), "unit": "GFLOP/s", "cores": oracle.max_threads(),
                         "kind": "port", "ms_per_spmv": round(t_cpu * 1e3, 3),
-                        "sample": f"the rank's full matrix, median of {args.cpu_reps} SpMVs after 1 warm-up "
+                        "sample": f"the rank's full matrix, median of 10 SpMVs after 2 warm-ups "
                                   f"(oracle: bdia_spmv_kernel over the local segment + cpu_coo_spmv_kernel over "
                                   f"the rank-local COO part + add, OpenMP)"}
         log(f"cpu baseline: {cpu_baseline}")
@@ -803,8 +913,7 @@ def bench_ranklocal(args) -> int:
         "parity": parity,
         "setup_s": round(time.time() - t0, 1),
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 if __name__ == "__main__":

@@ -137,6 +137,7 @@ _CAPI_SIGS = {
     "hbdia_vector_download_coo": (_i, [_p, _p, c_int64]),
     "hbdia_vector_device_ptrs": (_i, [_p, POINTER(c_void_p), POINTER(c_void_p)]),
     "hbdia_spmv": (_i, [_p, _p, _p, c_int, c_int]),
+    "hbdia_spmv_time_synced": (_i, [_p, _p, _p, c_int, c_int, c_int, POINTER(c_double)]),
     "hbdia_matrix_rank_local": (_i, [POINTER(c_void_p), _p, c_int, c_int, c_int, c_int, c_int, c_int]),
     "hbdia_matrix_build_local_coo": (_i, [_p]),
     "hbdia_matrix_local_info": (_i, [_p, _p]),

@@ -434,6 +434,17 @@ def hbdiaSpMV(matrix: HBDIA, inputVector: HBDIAVector, outputVector: HBDIAVector
     return True
 
 
+def spmv_time_synced(matrix: HBDIA, inputVector: HBDIAVector, outputVector: HBDIAVector, execCOOCPU: bool,
+                     execCOOGPU: bool, iterations: int = 100) -> np.ndarray:
+    """The reference's timing loop (benchmarking/benchHBDIA.cpp:30-50), run natively: per round, y
+    cleared, device synchronized, then hbdiaSpMV + device synchronize timed on the host clock (ms)."""
+    out = np.zeros(int(iterations), np.float64)
+    check_capi(capi.hbdia_spmv_time_synced(matrix._h, inputVector._h, outputVector._h, int(bool(execCOOCPU)),
+                                           int(bool(execCOOGPU)), int(iterations),
+                                           out.ctypes.data_as(ctypes.POINTER(c_double))))
+    return out
+
+
 def hbdiaSpMVMulti(matrix: HBDIA, X, Y, nrhs: int, ldx: int | None = None, ldy: int | None = None,
                    execCOOGPU: bool = True) -> bool:
     """Y[:, j] = A X[:, j] for j < nrhs in one launch (X, Y: DeviceArray, column-major).

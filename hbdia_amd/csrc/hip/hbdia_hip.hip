@@ -471,19 +471,23 @@ int xcd_chunk_shift(int64_t nwg) {
 
 // The wave-uniform sums read a block's offsets in whole chunks (one wide
 // scalar load each; the two-row sums read the first 3 kChunk entries at once),
-// so they read up to 3 kChunk - 2 entries past the block's last offset.  For a
-// block that ends that close to the end of the offsets array (total entries)
-// the callers take the per-lane sums instead, which clamp every offset read
-// and have the same arithmetic: the entry points then need no padding behind
-// the array.
+// so they read up to 3 kChunk - 2 entries past the block's last offset.  In
+// k_hbdia (every launch without the runs-of-three hints) a block that ends
+// that close to the end of the offsets array (total entries) takes the
+// per-lane sums instead, which clamp every offset read and have the same
+// arithmetic, so those launches need no padding behind the array.  The
+// two-row kernels, launched only with the hints, keep the wide reads with no
+// test (a test there costs registers the kernels do not have): callers that
+// pass hints also pass HBDIA_OFFSETS_PAD entries of padding.
 __device__ __forceinline__ bool reads_past_end(int start, int cnt, int total) {
-  return static_cast<int64_t>(start) + max(cnt + 3 * kChunk - 2, 3 * kChunk) > total;
+  // offsets counts are below 2^31 - 3 kChunk, so the sum does not overflow
+  return start + max(cnt + 3 * kChunk - 2, 3 * kChunk) > total;
 }
 
 // The blocked part of one 64-row group (rows row0 + lane) for a block width
 // that is not a multiple of 64; start and cnt are the calling lane's block's
 // (zero past the last row).
-template <typename T, bool X32, bool RUNS3>
+template <typename T, bool X32, bool RUNS3, bool CHECK_END>
 __device__ __forceinline__ T group_bdia_nonuniform(const T* __restrict__ vals, const int* __restrict__ offs,
                                                    const int* __restrict__ blk, const T* __restrict__ x,
                                                    int64_t row0, int lane, int64_t nrows, int nb, int bw, int64_t b,
@@ -504,7 +508,7 @@ __device__ __forceinline__ T group_bdia_nonuniform(const T* __restrict__ vals, c
     b0 = row0 / 32;
     if (b0 + 1 < nb) {
       const int s0 = blk[b0], s1 = blk[b0 + 1], e1 = blk[b0 + 2];
-      if (s1 - s0 == e1 - s1 && s1 > s0 && s1 - s0 <= kWave && !reads_past_end(s0, s1 - s0, total)) {
+      if (s1 - s0 == e1 - s1 && s1 > s0 && s1 - s0 <= kWave && !(CHECK_END && reads_past_end(s0, s1 - s0, total))) {
         const int c0 = s1 - s0;
         const int oa = lane < c0 ? offs[s0 + lane] : 0;
         const int ob = lane < c0 ? offs[s1 + lane] : 0;
@@ -577,8 +581,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia(
       sum = bdia_sum_lane<T, X32>(vp, offs + start, cnt, bw, x, row, nrows);
     }
   } else {
-    sum = group_bdia_nonuniform<T, X32, RUNS3>(vals, offs, blk, x, row0, lane, nrows, nb, bw, b, start, cnt,
-                                               runs3_blocks, total);
+    sum = group_bdia_nonuniform<T, X32, RUNS3, true>(vals, offs, blk, x, row0, lane, nrows, nb, bw, b, start, cnt,
+                                                     runs3_blocks, total);
   }
   if constexpr (HAS_COO) {
     const int p0 = gptr[grp], p1 = gptr[grp + 1];
@@ -778,11 +782,11 @@ __device__ __forceinline__ void bdia_sum_runs3_row2_27(const T* __restrict__ vb,
 template <typename T, bool X32>
 __device__ __forceinline__ T half_bdia(const T* __restrict__ vals, const int* __restrict__ offs,
                                             const T* __restrict__ x, int64_t r0, int64_t b, int start,
-                                            int cnt, int bw, int lane, int64_t nrows, bool runs, int total) {
+                                            int cnt, int bw, int lane, int64_t nrows, bool runs) {
   const int64_t row = r0 + lane;
   const bool active = row < nrows;
   const bool small = static_cast<int64_t>(cnt) * bw * static_cast<int64_t>(sizeof(T)) < (1ll << 32);
-  if (cnt > 0 && small && !reads_past_end(start, cnt, total)) {
+  if (cnt > 0 && small) {
     const T* vb = vals + static_cast<int64_t>(start) * bw + (r0 - b * bw);
     if (runs) return bdia_sum_runs3<T, X32>(vb, offs + start, cnt, bw, lane, x, r0, row, nrows, active);
     return bdia_sum_uniform<T, X32>(vb, offs + start, cnt, bw, lane, x, row, nrows, active);
@@ -825,11 +829,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
   const bool runsA = bA < nb && ((runs3_blocks[bA >> 5] >> (bA & 31)) & 1u);
   const bool runsB = hasB && bB < nb && ((runs3_blocks[bB >> 5] >> (bB & 31)) & 1u);
-  const int total = blk[nb];
   // both halves in one offset list of runs: one block, or two with equal offsets
-  // (the chunked reads from sA cover block bB's list too, which follows)
   bool same = false;
-  if (runsA && cA > 0 && cA <= kWave && !reads_past_end(sA, cA, total)) {
+  if (runsA && cA > 0 && cA <= kWave) {
     if (bA == bB) {
       same = true;
     } else if (hasB && cB == cA) {
@@ -890,8 +892,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k
     yB = sy[kWave + lane];
     wave_sync();
   } else {
-    yA = half_bdia<T, X32>(vals, offs, x, row0, bA, sA, cA, bw, lane, nrows, runsA, total);
-    if (hasB) yB = half_bdia<T, X32>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB, total);
+    yA = half_bdia<T, X32>(vals, offs, x, row0, bA, sA, cA, bw, lane, nrows, runsA);
+    if (hasB) yB = half_bdia<T, X32>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB);
   }
   // fallback sums of both 64-row groups: one pass when their entries fit one
   T csA = T(0), csB = T(0);
@@ -958,13 +960,12 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
   if (row0 >= nrows) return;
   const bool hasB = row0 + kWave < nrows;
   const int64_t b0 = row0 / bw;
-  const int total = blk[nb];
   bool quad = false;
   int s0 = 0, c = 0;
   if (b0 + 3 < nb) {
     s0 = blk[b0];
     c = blk[b0 + 1] - s0;
-    if (c > 0 && c <= kWave && !reads_past_end(s0, c, total) && blk[b0 + 2] == s0 + 2 * c && blk[b0 + 3] == s0 + 3 * c &&
+    if (c > 0 && c <= kWave && blk[b0 + 2] == s0 + 2 * c && blk[b0 + 3] == s0 + 3 * c &&
         blk[b0 + 4] == s0 + 4 * c && ((runs3_blocks[b0 >> 5] >> (b0 & 31)) & 1u)) {
       const int oa = lane < c ? offs[s0 + lane] : 0;
       bool diff = false;
@@ -1038,8 +1039,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_row2_w32(
         cnt = blk[b + 1] - start;
       }
       const T sum =
-          group_bdia_nonuniform<T, X32, true>(vals, offs, blk, x, r0, lane, nrows, nb, bw, b, start, cnt,
-                                                   runs3_blocks, total);
+          group_bdia_nonuniform<T, X32, true, false>(vals, offs, blk, x, r0, lane, nrows, nb, bw, b, start, cnt,
+                                                          runs3_blocks, 0);
       if (h == 0) {
         yA = sum;
         cntA = cnt;

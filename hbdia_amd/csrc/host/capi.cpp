@@ -4,6 +4,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -473,6 +474,42 @@ int hbdia_spmv(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_co
     need(m->dtype == x->dtype && m->dtype == y->dtype, "dtype mismatch between matrix and vectors");
     if (m->dtype == HBDIA_DTYPE_F64) hbdiaSpMV<double>(*m->d, *x->d, *y->d, exec_coo_cpu != 0, exec_coo_gpu != 0);
     else hbdiaSpMV<float>(*m->f, *x->f, *y->f, exec_coo_cpu != 0, exec_coo_gpu != 0);
+  });
+}
+
+}  // extern "C"
+
+namespace {
+// benchHBDIA's loop (reference benchmarking/benchHBDIA.cpp:30-50) for one value type.
+template <typename T>
+void time_synced(HBDIA<T>& A, const HBDIAVector<T>& x, HBDIAVector<T>& y, bool cpu, bool gpu, int iterations,
+                 double* times_ms) {
+  const size_t bytes = static_cast<size_t>(A.getNumRows()) * sizeof(T);
+  for (int i = 0; i < iterations; ++i) {
+    CHECK_HBDIA(hbdia_memset(y.getDeviceLocalPtr(), 0, bytes, nullptr));
+    CHECK_HBDIA(hbdia_memset(y.getDeviceCOOResultsPtr(), 0, bytes, nullptr));
+    CHECK_HBDIA(hbdia_device_synchronize());
+    const auto t0 = std::chrono::steady_clock::now();
+    hbdiaSpMV<T>(A, x, y, cpu, gpu);
+    CHECK_HBDIA(hbdia_device_synchronize());
+    const auto t1 = std::chrono::steady_clock::now();
+    times_ms[i] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int hbdia_spmv_time_synced(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_coo_cpu, int exec_coo_gpu,
+                           int iterations, double* times_ms) {
+  return guard("hbdia_spmv_time_synced", [&] {
+    need(m && x && y && times_ms, "null argument");
+    need(iterations >= 0, "negative iteration count");
+    need(m->dtype == x->dtype && m->dtype == y->dtype, "dtype mismatch between matrix and vectors");
+    if (m->dtype == HBDIA_DTYPE_F64)
+      time_synced<double>(*m->d, *x->d, *y->d, exec_coo_cpu != 0, exec_coo_gpu != 0, iterations, times_ms);
+    else
+      time_synced<float>(*m->f, *x->f, *y->f, exec_coo_cpu != 0, exec_coo_gpu != 0, iterations, times_ms);
   });
 }
 

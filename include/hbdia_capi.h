@@ -164,6 +164,12 @@ int hbdia_vector_download_coo(hbdia_vector_t v, void* out, int64_t n);
 int hbdia_vector_device_ptrs(hbdia_vector_t v, void** local, void** coo);
 
 int hbdia_spmv(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_coo_cpu, int exec_coo_gpu);
+/* The reference's timing loop (benchmarking/benchHBDIA.cpp:30-50): each of
+ * `iterations` rounds clears y and its COO-result buffer, synchronizes the
+ * device, then times hbdiaSpMV plus a device synchronize on the host clock
+ * into times_ms[i].  The reference's plots drop round 0 as warm-up. */
+int hbdia_spmv_time_synced(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_coo_cpu, int exec_coo_gpu,
+                           int iterations, double* times_ms);
 
 /* Rank-local form on one GPU (createRankLocalMatrix, include/DataExchange/BasicExtractor.hpp):
  * rows [rank*q, rank*q + q) of the global matrix (q = rows / size, the last rank also the

@@ -30,14 +30,15 @@ extern "C" {
 
 /* Rows handled by one 64-lane wavefront; COO group pointers are per group. */
 #define HBDIA_ROW_GROUP 64
-/* The multi-RHS (hbdia_hbdia_spmm_*) and reduced-precision
- * (hbdia_hbdia_spmv_mixed*) entry points need their `offsets` array readable
- * for HBDIA_OFFSETS_PAD int32 past its last offset (block_start[num_blocks]):
- * they read a chunk's offsets as one wide load and ignore what lies past a
- * block's count.  HBDIA<T>::prepareForGPU allocates this padding.  The
- * single-vector entry points (hbdia_bdia_spmv_*, hbdia_hbdia_spmv_*, their _ex
- * forms), which replace the reference's launch_bdia_spmv_kernel, read nothing
- * past block_start[num_blocks]: an exactly-sized array is enough. */
+/* The single-vector entry points hbdia_bdia_spmv_* and hbdia_hbdia_spmv_*,
+ * which replace the reference's launch_bdia_spmv_kernel, read nothing past
+ * block_start[num_blocks] in `offsets`: an exactly-sized array is enough (so
+ * is the _ex form with runs3_blocks == NULL).  Every other blocked entry point
+ * -- the _ex forms given a runs-of-three bitset, multi-RHS (hbdia_hbdia_spmm_*)
+ * and reduced precision (hbdia_hbdia_spmv_mixed*) -- reads a chunk's offsets
+ * as one wide load and needs the array readable for HBDIA_OFFSETS_PAD int32
+ * past its last offset; what lies there is ignored.  HBDIA<T>::prepareForGPU
+ * and hbdia_convert_coo allocate this padding. */
 #define HBDIA_OFFSETS_PAD 32
 
 typedef struct hbdia_stream_opaque* hbdia_stream_t;

@@ -21,10 +21,13 @@ _DIR = Path(__file__).resolve().parent
 _LIB_PATH = _DIR / "build" / "libhbdia_oracle.so"
 
 
-def _load():
-    if not _LIB_PATH.exists():
-        raise RuntimeError(f"{_LIB_PATH} missing; run `make -C oracle`")
-    lib = ctypes.CDLL(str(_LIB_PATH), mode=os.RTLD_NOW)
+_BASELINE_PATH = _DIR / "build" / "libhbdia_cpu_baseline.so"
+
+
+def _load(path: Path = _LIB_PATH):
+    if not path.exists():
+        raise RuntimeError(f"{path} missing; run `make -C oracle`")
+    lib = ctypes.CDLL(str(path), mode=os.RTLD_NOW)
     P = c_void_p
     sig = {
         "orc_stencil27_f64": (P, [c_int, c_int, c_int, c_double, c_int]),
@@ -66,6 +69,35 @@ def _sfx(dtype) -> str:
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data if a.size else None
+
+
+class CpuBaseline:
+    """The restatement built with the reference's host flags (oracle/Makefile `baseline`: -O3
+    -march=native -ffp-contract=fast -fopenmp), compiled on the machine that times it.  Only
+    bench.py's cpu_baseline leg uses it, as the timed CPU path; the checker stays the oracle above."""
+
+    FLAGS = "-O3 -march=native -ffp-contract=fast -fopenmp"
+
+    def __init__(self):
+        import shutil
+        import subprocess
+        make = shutil.which("make")
+        proc = subprocess.run([make or "make", "-s", "-C", str(_DIR), "baseline"], stdout=subprocess.PIPE,
+                              stderr=subprocess.STDOUT, text=True) if make else None
+        if proc is None or proc.returncode != 0:
+            raise RuntimeError("building the CPU baseline failed: " + (proc.stdout if proc else "make not found"))
+        self._lib = _load(_BASELINE_PATH)
+
+    def max_threads(self) -> int:
+        return int(self._lib.orc_max_threads())
+
+    def hbdia_spmv(self, hb: dict, x: np.ndarray, y: np.ndarray, num_rows: int, coo_buf: np.ndarray) -> np.ndarray:
+        """hbdia_spmv below, through this build."""
+        getattr(self._lib, "orc_hbdia_spmv_" + _sfx(hb["values"].dtype))(
+            _ptr(hb["values"]), _ptr(hb["offsets"]), _ptr(hb["block_start"]), int(hb["num_blocks"]),
+            int(hb["block_width"]), _ptr(hb["coo_rows"]), _ptr(hb["coo_cols"]), _ptr(hb["coo_values"]),
+            len(hb["coo_values"]), x.ctypes.data, y.ctypes.data, coo_buf.ctypes.data, int(num_rows))
+        return y
 
 
 def set_threads(n: int) -> None:

@@ -672,18 +672,15 @@ def test_rectangular_matrix_needs_x_over_every_column(gpu):
         hbdiaSpMVMulti(A, X, Y, 1)
 
 
-@pytest.mark.parametrize("N,bw,with_coo,dtype,hints", [(1000, 64, False, np.float64, True),
-                                                       (1000, 64, True, np.float64, True),
-                                                       (4099, 128, True, np.float64, True),
-                                                       (1000, 64, True, np.float32, True),
-                                                       (1000, 64, True, np.float64, False),
-                                                       (4096, 32, True, np.float64, True),
-                                                       (4096, 32, False, np.float32, True),
-                                                       (1003, 5, True, np.float64, False)])
-def test_exactly_sized_offsets_array(gpu, N, bw, with_coo, dtype, hints):
-    """The single-vector entry points (the reference's launch_bdia_spmv_kernel signature) read nothing
-    past block_start[num_blocks]: with the offsets uploaded at their exact size -- no padding -- every
-    path (one-row, two-row, width 32, per-lane) gives the oracle's y bit for bit."""
+@pytest.mark.parametrize("N,bw,with_coo,dtype", [(1000, 64, False, np.float64), (1000, 64, True, np.float64),
+                                                 (4099, 128, True, np.float64), (1000, 64, True, np.float32),
+                                                 (4096, 32, True, np.float64), (4096, 32, False, np.float32),
+                                                 (1003, 5, True, np.float64), (64, 64, True, np.float64)])
+def test_exactly_sized_offsets_array(gpu, N, bw, with_coo, dtype):
+    """The single-vector entry points of the reference's launch_bdia_spmv_kernel signature (no
+    layout hints) read nothing past block_start[num_blocks]: with the offsets uploaded at their exact
+    size -- no padding -- the uniform, width-32 and per-lane paths give the oracle's y bit for bit."""
+    hints = False
     print(f"case: exact offsets N={N} bw={bw} coo={with_coo} {np.dtype(dtype).name} hints={hints}", flush=True)
     if bw == 32:
         values, offsets, block_start, x, coo = _width32_case(N, 77, with_coo, share=1.0, group=4)
