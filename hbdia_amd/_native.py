@@ -88,6 +88,8 @@ _HIP_SIGS = {
     "hbdia_serve_spmv_f32": (_i, [_p, _p, c_int32, c_int64, c_int32, c_int32, _p, _p, _p]),
     "hbdia_batch_spmv_f32": (_i, [_p, _p, _p, _p, _p]),
     "hbdia_batch_occupancy": (_i, [_p, c_int32, c_int32, POINTER(c_int)]),
+    "hbdia_hbdia_spmm_rm_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int32, c_int64, _p]),
+    "hbdia_hbdia_spmm_rm_f32": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int32, c_int64, _p]),
     "hbdia_hbdia_spmm_ex_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, c_int64, _p, c_int64, c_int32,
                                      c_int64, _p, _p]),
     "hbdia_hbdia_spmm_ex_f32": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, c_int64, _p, c_int64, c_int32,
@@ -147,6 +149,7 @@ _CAPI_SIGS = {
     "hbdia_vector_fill_halo": (_i, [_p, _p, _p, c_int64]),
     "hbdia_vector_verify_halo": (_i, [_p, _p, POINTER(c_int)]),
     "hbdia_spmv_multi": (_i, [_p, _p, c_int64, _p, c_int64, c_int, c_int]),
+    "hbdia_spmv_multi_rm": (_i, [_p, _p, _p, c_int, c_int]),
     "hbdia_mixed_create": (_i, [POINTER(c_void_p), _p, c_int, c_double]),
     "hbdia_mixed_destroy": (_i, [_p]),
     "hbdia_mixed_spmv": (_i, [_p, _p, _p]),

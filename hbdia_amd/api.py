@@ -446,11 +446,26 @@ def spmv_time_synced(matrix: HBDIA, inputVector: HBDIAVector, outputVector: HBDI
 
 
 def hbdiaSpMVMulti(matrix: HBDIA, X, Y, nrhs: int, ldx: int | None = None, ldy: int | None = None,
-                   execCOOGPU: bool = True) -> bool:
-    """Y[:, j] = A X[:, j] for j < nrhs in one launch (X, Y: DeviceArray, column-major).
+                   execCOOGPU: bool = True, layout: str = "col") -> bool:
+    """Y[:, j] = A X[:, j] for j < nrhs in one launch (X, Y: DeviceArray).
 
-    Asynchronous like hbdiaSpMV; every column equals hbdiaSpMV on that column bit for bit."""
+    layout "col": column-major, column j at X + j * ldx; "row": row-major (interleaved), row r's
+    nrhs values adjacent (hbdiaSpMVMultiRowMajor; ldx / ldy unused).  Asynchronous like
+    hbdiaSpMV; every column equals hbdiaSpMV on that column bit for bit."""
     n = matrix.getNumRows()
+    if layout == "row":
+        nx = max(n, matrix.getNumCols()) if matrix.info()["num_coo"] else n
+        if X.size < nx * nrhs or Y.size < n * nrhs:
+            raise ValueError("X or Y too small for nrhs row-major columns")
+        try:
+            check_capi(capi.hbdia_spmv_multi_rm(matrix._h, X.ptr, Y.ptr, int(nrhs), int(bool(execCOOGPU))))
+        except HBDIAError as e:
+            if "Invalid COO execution flags" in str(e):
+                raise RuntimeError("Invalid COO execution flags") from None
+            raise _map_error(e, "hbdiaSpMVMulti") from None
+        return True
+    if layout != "col":
+        raise ValueError("layout must be 'col' or 'row'")
     # fallback columns reach numCols - 1 (blocked columns stay below numRows)
     nx = max(n, matrix.getNumCols()) if matrix.info()["num_coo"] else n
     ldx = nx if ldx is None else int(ldx)

@@ -34,6 +34,13 @@ struct Dev<double> {
     return hbdia_vector_add_f64(d, s_, n, s);
   }
   static double fma(double a, double b, double c) { return std::fma(a, b, c); }
+  static int multi_rm(const HBDIA<double>& m, bool coo, const double* X, double* Y, int k, hbdia_stream_t s) {
+    return hbdia_hbdia_spmm_rm_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
+                                   m.getBlockStartIndicesDevice(), m.getNumBlocks(), m.getBlockWidth(),
+                                   coo ? m.getCpuRowIndicesDevice() : nullptr, coo ? m.getCpuColIndicesDevice() : nullptr,
+                                   coo ? m.getCpuValuesDevice() : nullptr, coo ? m.getCooGroupPtrDevice() : nullptr, X,
+                                   Y, k, m.getNumRows(), s);
+  }
   static int multi(const HBDIA<double>& m, bool coo, const double* X, int64_t ldx, double* Y, int64_t ldy, int k,
                    hbdia_stream_t s) {
     return hbdia_hbdia_spmm_ex_f64(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
@@ -65,6 +72,13 @@ struct Dev<float> {
     return hbdia_vector_add_f32(d, s_, n, s);
   }
   static float fma(float a, float b, float c) { return std::fmaf(a, b, c); }
+  static int multi_rm(const HBDIA<float>& m, bool coo, const float* X, float* Y, int k, hbdia_stream_t s) {
+    return hbdia_hbdia_spmm_rm_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
+                                   m.getBlockStartIndicesDevice(), m.getNumBlocks(), m.getBlockWidth(),
+                                   coo ? m.getCpuRowIndicesDevice() : nullptr, coo ? m.getCpuColIndicesDevice() : nullptr,
+                                   coo ? m.getCpuValuesDevice() : nullptr, coo ? m.getCooGroupPtrDevice() : nullptr, X,
+                                   Y, k, m.getNumRows(), s);
+  }
   static int multi(const HBDIA<float>& m, bool coo, const float* X, int64_t ldx, float* Y, int64_t ldy, int k,
                    hbdia_stream_t s) {
     return hbdia_hbdia_spmm_ex_f32(m.getHBDIADataDevice(), m.getFlattenedOffsetsDevice(),
@@ -240,6 +254,30 @@ bool hbdiaSpMVMulti(HBDIA<T>& matrix, const T* X, int64_t ldx, T* Y, int64_t ldy
   return true;
 }
 
+template <typename T>
+bool hbdiaSpMVMultiRowMajor(HBDIA<T>& matrix, const T* X, T* Y, int nrhs, bool execCOOGPU) {
+  if (!matrix.isHBDIAFormat() || !matrix.areStreamsInitialized())
+    throw std::runtime_error("hbdiaSpMVMultiRowMajor: matrix is not converted and on the device");
+  if (matrix.isPartialMatrix())
+    throw std::invalid_argument("hbdiaSpMVMultiRowMajor: rank-local (partial) matrices take hbdiaSpMV with halo vectors");
+  if (nrhs < 1) throw std::invalid_argument("hbdiaSpMVMultiRowMajor: nrhs must be at least 1");
+  const bool hasCOOWork = !matrix.getCpuValues().empty();
+  if (hasCOOWork && !execCOOGPU) throw std::runtime_error("Invalid COO execution flags");
+  if (hasCOOWork && !matrix.hasDeviceCOO())
+    throw std::runtime_error("hbdiaSpMVMultiRowMajor: the fallback is not on the device (convert with ExecutionMode::GPU_COO)");
+  hbdia_stream_t s = matrix.getBDIAStream();
+  if (nrhs == 1) {
+    if (hasCOOWork) CHECK_HBDIA(Dev<T>::fused(matrix, X, Y, s));
+    else CHECK_HBDIA(Dev<T>::bdia(matrix, X, Y, s));
+  } else {
+    CHECK_HBDIA(Dev<T>::multi_rm(matrix, hasCOOWork, X, Y, nrhs, s));
+  }
+  CHECK_HBDIA(hbdia_event_record(matrix.getBDIAEvent(), s));
+  return true;
+}
+
+template bool hbdiaSpMVMultiRowMajor<float>(HBDIA<float>&, const float*, float*, int, bool);
+template bool hbdiaSpMVMultiRowMajor<double>(HBDIA<double>&, const double*, double*, int, bool);
 template bool hbdiaSpMVMulti<float>(HBDIA<float>&, const float*, int64_t, float*, int64_t, int, bool);
 template bool hbdiaSpMVMulti<double>(HBDIA<double>&, const double*, int64_t, double*, int64_t, int, bool);
 template bool hbdiaSpMV<float>(HBDIA<float>&, const HBDIAVector<float>&, HBDIAVector<float>&, bool, bool);

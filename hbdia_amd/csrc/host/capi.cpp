@@ -613,6 +613,18 @@ int hbdia_spmv_multi(hbdia_matrix_t m, const void* X, int64_t ldx, void* Y, int6
   });
 }
 
+int hbdia_spmv_multi_rm(hbdia_matrix_t m, const void* X, void* Y, int nrhs, int exec_coo_gpu) {
+  return guard("hbdia_spmv_multi_rm", [&] {
+    need(m != nullptr && X != nullptr && Y != nullptr, "null argument");
+    if (m->dtype == HBDIA_DTYPE_F64)
+      hbdiaSpMVMultiRowMajor<double>(*m->d, static_cast<const double*>(X), static_cast<double*>(Y), nrhs,
+                                     exec_coo_gpu != 0);
+    else
+      hbdiaSpMVMultiRowMajor<float>(*m->f, static_cast<const float*>(X), static_cast<float*>(Y), nrhs,
+                                    exec_coo_gpu != 0);
+  });
+}
+
 int hbdia_mixed_create(hbdia_mixed_t* h, hbdia_matrix_t m, int precision, double ratio) {
   return guard("hbdia_mixed_create", [&] {
     need(h != nullptr && m != nullptr, "null argument");

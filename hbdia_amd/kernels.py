@@ -96,6 +96,16 @@ def hbdia_spmm(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArr
                  _h(stream)))
 
 
+def hbdia_spmm_rm(values: DeviceArray, offsets: DeviceArray, block_start: DeviceArray, num_blocks: int,
+                  block_width: int, coo, X: DeviceArray, Y: DeviceArray, nrhs: int, num_rows: int,
+                  stream: Stream | None = None) -> None:
+    """Many right-hand sides, X and Y row-major (row r's nrhs values adjacent); coo as for hbdia_spmm."""
+    fn = getattr(hip, "hbdia_hbdia_spmm_rm_" + _sfx(values.dtype))
+    cr, cc, cv, gp = (a.ptr for a in coo) if coo is not None else (None, None, None, None)
+    check_hip(fn(values.ptr, offsets.ptr, block_start.ptr, int(num_blocks), int(block_width), cr, cc, cv, gp, X.ptr,
+                 Y.ptr, int(nrhs), int(num_rows), _h(stream)))
+
+
 def coo_spmv(coo_rows: DeviceArray, coo_cols: DeviceArray, coo_values: DeviceArray, coo_group_ptr: DeviceArray,
              x: DeviceArray, res: DeviceArray, num_rows: int, stream: Stream | None = None) -> None:
     fn = getattr(hip, "hbdia_coo_spmv_" + _sfx(coo_values.dtype))
@@ -220,6 +230,11 @@ class DeviceLayout:
         coo = (self.coo_rows, self.coo_cols, self.coo_values, self.group_ptr) if self.coo_nnz else None
         hbdia_spmm(self.values, self.offsets, self.block_start, self.num_blocks, self.block_width, coo, X,
                    self.num_rows, Y, self.num_rows, nrhs, self.num_rows, stream, self.runs3 if hints else None)
+
+    def spmm_rm(self, X: DeviceArray, Y: DeviceArray, nrhs: int, stream: Stream | None = None) -> None:
+        coo = (self.coo_rows, self.coo_cols, self.coo_values, self.group_ptr) if self.coo_nnz else None
+        hbdia_spmm_rm(self.values, self.offsets, self.block_start, self.num_blocks, self.block_width, coo, X, Y, nrhs,
+                      self.num_rows, stream)
 
     def spmv(self, x: DeviceArray, y: DeviceArray, stream: Stream | None = None, hints: bool = False) -> None:
         """y = A x with the fused kernel (blocked-DIA only when there is no fallback); `hints` passes

@@ -31,6 +31,15 @@ bool hbdiaSpMV(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector, HBDIAVector<
 template <typename T>
 bool hbdiaSpMVMulti(HBDIA<T>& matrix, const T* X, int64_t ldx, T* Y, int64_t ldy, int nrhs, bool execCOOGPU);
 
+// The same with X and Y row-major (interleaved): row r's nrhs values are
+// adjacent, X[r * nrhs + j].  A row's values must be a multiple of 16 bytes
+// and at most 512 (FP64: nrhs even and <= 64; FP32: nrhs a multiple of 4 and
+// <= 128); nrhs == 1 takes hbdiaSpMV's kernel.  X holds max(numRows, numCols)
+// rows when the matrix has a fallback, Y numRows rows.  Each column equals
+// hbdiaSpMV on that column bit for bit.  Asynchronous.
+template <typename T>
+bool hbdiaSpMVMultiRowMajor(HBDIA<T>& matrix, const T* X, T* Y, int nrhs, bool execCOOGPU);
+
 // COO fallback on the host (reference HBDIASpMV.cpp:101-131).
 template <typename T>
 void executeCOOOnCPU(HBDIA<T>& matrix, const HBDIAVector<T>& inputVector,

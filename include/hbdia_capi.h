@@ -209,6 +209,8 @@ int hbdia_vector_fill_halo(hbdia_vector_t v, hbdia_matrix_t m, const void* globa
 int hbdia_vector_verify_halo(hbdia_vector_t v, hbdia_matrix_t m, int* ok);
 /* hbdiaSpMVMulti: X, Y device arrays (column-major, leading dimensions ldx, ldy). */
 int hbdia_spmv_multi(hbdia_matrix_t m, const void* X, int64_t ldx, void* Y, int64_t ldy, int nrhs, int exec_coo_gpu);
+/* hbdiaSpMVMultiRowMajor: X, Y device arrays, row-major (row r's nrhs values adjacent). */
+int hbdia_spmv_multi_rm(hbdia_matrix_t m, const void* X, void* Y, int nrhs, int exec_coo_gpu);
 
 /* Reduced-precision storage (HBDIAMixed, include/Operations/HBDIAMixed.hpp):
  * precision 1 = FP32, 2 = BF16; an FP64 matrix only.  x, y: FP64 vectors.

@@ -181,6 +181,25 @@ int hbdia_hbdia_spmm_ex_f32(const float* values, const int32_t* offsets, const i
                             const float* X, int64_t ldx, float* Y, int64_t ldy, int32_t nrhs, int64_t num_rows,
                             const uint32_t* runs3_blocks, hbdia_stream_t stream);
 
+/* The same product with X and Y row-major (interleaved): row r's values of
+ * the nrhs columns are adjacent, X[r * nrhs + j], Y[r * nrhs + j].  Every
+ * column equals the single-vector result bit for bit, as above.  A row's nrhs
+ * values must be a multiple of 16 bytes and at most
+ * HBDIA_SPMM_RM_MAX_CHUNKS x 16 bytes (FP64: nrhs even, <= 64; FP32: nrhs a
+ * multiple of 4, <= 128); X and Y 16-byte aligned; block_width % 64 == 0;
+ * offsets padded as for the other multi-RHS entry points.  The x windows of
+ * each run of consecutive offsets are staged into LDS with direct
+ * global-to-LDS loads. */
+#define HBDIA_SPMM_RM_MAX_CHUNKS 32
+int hbdia_hbdia_spmm_rm_f64(const double* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const double* coo_values, const int32_t* coo_group_ptr,
+                            const double* X, double* Y, int32_t nrhs, int64_t num_rows, hbdia_stream_t stream);
+int hbdia_hbdia_spmm_rm_f32(const float* values, const int32_t* offsets, const int32_t* block_start,
+                            int32_t num_blocks, int32_t block_width, const int32_t* coo_rows,
+                            const int32_t* coo_cols, const float* coo_values, const int32_t* coo_group_ptr,
+                            const float* X, float* Y, int32_t nrhs, int64_t num_rows, hbdia_stream_t stream);
+
 /* Reduced-precision storage (no reference counterpart: the reference stores
  * and computes in one type).  values_lp / coo_values_lp are FP32 or BF16 copies
  * (vdtype) of the FP64 arrays, made with hbdia_convert_values (FP64 -> FP32

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle
-from hbdia_amd import HBDIA, DeviceArray, ExecutionMode, Graph, HBDIAVector, Stream, gen_vector, hbdiaSpMV
+from hbdia_amd import HBDIA, DeviceArray, ExecutionMode, Graph, HBDIAError, HBDIAVector, Stream, gen_vector, hbdiaSpMV
 from hbdia_amd import kernels
 from hbdia_amd.device import synchronize
 
@@ -705,3 +705,103 @@ def test_exactly_sized_offsets_array(gpu, N, bw, with_coo, dtype):
         hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0, values.dtype))
     want = oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N)
     assert Y.download().tobytes() == want.tobytes()
+
+
+def _mixed_runs_case(N, bw, seed, with_coo):
+    """Like _runs3_case, but each block's sorted offsets are groups of one, two or three consecutive
+    values (groups may touch, making longer runs), some outside [-N, N)."""
+    rng = np.random.default_rng(seed)
+    nb = -(-N // bw)
+    offs_per = []
+    for _ in range(nb):
+        k = int(rng.integers(0, 8))
+        starts = np.sort(rng.choice(np.arange(-N - 5, N + 5, 4), size=k, replace=False))
+        offs = sorted({int(s0 + d) for s0 in starts for d in range(int(rng.integers(1, 4)))})
+        offs_per.append(offs)
+    block_start = np.zeros(nb + 1, np.int32)
+    block_start[1:] = np.cumsum([len(o) for o in offs_per])
+    offsets = np.array(sum(offs_per, []), np.int32)
+    values = rng.standard_normal(len(offsets) * bw)
+    values[rng.random(values.size) < 0.2] = 0.0
+    x = rng.standard_normal(N)
+    x[rng.random(N) < 0.02] = np.nan
+    coo = None
+    if with_coo:
+        m = 3 * N // 2
+        r = np.sort(rng.integers(0, N, m)).astype(np.int32)
+        coo = (r, rng.integers(0, N, m).astype(np.int32), rng.standard_normal(m))
+    return values, offsets, block_start, x, coo
+
+
+@pytest.mark.parametrize("N,bw,with_coo,k,dtype", [(1000, 64, False, 2, np.float64), (1000, 64, True, 4, np.float64),
+                                                   (4099, 128, True, 8, np.float64), (777, 256, True, 32, np.float64),
+                                                   (1000, 64, True, 64, np.float64), (1000, 64, True, 6, np.float64),
+                                                   (1000, 64, True, 4, np.float32), (4099, 128, False, 32, np.float32),
+                                                   (777, 64, True, 128, np.float32), (64, 64, True, 12, np.float32)])
+def test_row_major_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k, dtype):
+    """The row-major (interleaved) multi-RHS kernel on random layouts -- runs of three and lone
+    offsets, columns outside [0, N), zero values over NaN x entries, a fallback in some cases,
+    FP64 and FP32, k up to 512 bytes per row: every column equals the oracle's single-vector result."""
+    print(f"case: row-major multi-RHS N={N} bw={bw} coo={with_coo} k={k} {np.dtype(dtype).name}", flush=True)
+    values, offsets, block_start, x0, coo = _mixed_runs_case(N, bw, 11 + N + bw + k, with_coo)
+    rng = np.random.default_rng(N + k)
+    values = values.astype(dtype)
+    if coo is not None:
+        coo = (coo[0], coo[1], coo[2].astype(dtype))
+    L = _layout(values, offsets, block_start, N, bw, coo)
+    Xh = np.vstack([x0] + [rng.standard_normal(N) for _ in range(k - 1)]).astype(dtype)  # (k, N)
+    Y0 = np.vstack([np.linspace(-1.0, 1.0, N) + j for j in range(k)]).astype(dtype)
+    X = DeviceArray.from_host(np.ascontiguousarray(Xh.T).ravel())
+    Y = DeviceArray.from_host(np.ascontiguousarray(Y0.T).ravel())
+    L.spmm_rm(X, Y, k)
+    synchronize()
+    out = Y.download().reshape(N, k).T
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": len(block_start) - 1,
+          "block_width": bw}
+    if coo is not None:
+        hb.update(coo_rows=coo[0], coo_cols=coo[1], coo_values=coo[2])
+    else:
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0, values.dtype))
+    for j in range(k):
+        want = oracle.hbdia_spmv(hb, Xh[j].copy(), Y0[j].copy(), N)
+        assert out[j].tobytes() == want.tobytes(), f"column {j}"
+
+
+@pytest.mark.parametrize("k,dtype", [(2, np.float64), (8, np.float64), (32, np.float64), (4, np.float32),
+                                     (32, np.float32)])
+def test_row_major_multi_rhs_stencil_and_api(gpu, k, dtype):
+    """hbdiaSpMVMulti(layout="row") on a 20^3 stencil with 1 % scatter (block width 64) and on a
+    matrix whose blocks cover only part of the rows: every column equals the oracle; rows without a
+    block keep Y; unsupported widths are refused."""
+    print(f"case: row-major multi-RHS stencil k={k} {np.dtype(dtype).name}", flush=True)
+    from hbdia_amd import hbdiaSpMVMulti
+    n = 20
+    A = HBDIA.stencil(n, 27, 0.01, seed=21, dtype=dtype)
+    A.convertToHBDIAFormat(64, 2)
+    N = n ** 3
+    rng = np.random.default_rng(k)
+    Xh = rng.uniform(-1, 1, (N, k)).astype(dtype)
+    Y0 = np.full((N, k), 0.25, dtype)
+    X, Y = DeviceArray.from_host(Xh.ravel()), DeviceArray.from_host(Y0.ravel())
+    hbdiaSpMVMulti(A, X, Y, k, layout="row")
+    A.synchronize()
+    out = Y.download().reshape(N, k)
+    for j in range(k):
+        assert out[:, j].tobytes() == oracle_y(A, np.ascontiguousarray(Xh[:, j]), Y0[:, j].copy()).tobytes()
+    # blocks over the first 100 of 200 rows only, no fallback: rows past them keep Y
+    rows = np.array([0, 0, 1, 63, 64, 64], np.int32)
+    cols = np.array([0, 1, 1, 63, 64, 63], np.int32)
+    B = HBDIA(rows, cols, np.array([2.0, 1.0, 3.0, 4.0, 5.0, 6.0], dtype), 200, 100)
+    B.convertToHBDIAFormat(64, 1)
+    Xb = rng.uniform(-1, 1, (200, k)).astype(dtype)
+    Yb0 = np.full((200, k), 7.0, dtype)
+    XB, YB = DeviceArray.from_host(Xb.ravel()), DeviceArray.from_host(Yb0.ravel())
+    hbdiaSpMVMulti(B, XB, YB, k, layout="row")
+    B.synchronize()
+    ob = YB.download().reshape(200, k)
+    for j in range(k):
+        assert ob[:, j].tobytes() == oracle_y(B, np.ascontiguousarray(Xb[:, j]), Yb0[:, j].copy()).tobytes()
+    assert np.all(ob[128:] == 7.0)
+    bad = 3 if dtype == np.float64 else 6  # a row of 24 bytes: not a multiple of 16
+    with pytest.raises((HBDIAError, ValueError)):
+        hbdiaSpMVMulti(A, DeviceArray(N * bad, dtype), DeviceArray(N * bad, dtype), bad, layout="row")
