@@ -1483,6 +1483,77 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi_row2(
 // flight, and the current tile's fallback entries are loaded before its
 // blocked part is summed.
 
+// Two tiles of one matrix, rows row0 .. row0 + 127, whose blocks share the
+// offset list off[0 .. cnt) (cnt <= kChunk): lane l holds rows row0 + 2 l and
+// row0 + 2 l + 1, its two values of an offset one 16-byte load from tile A's
+// lines (l < 32) or tile B's.  With a fallback (fused) its entries [p0, p2)
+// (tile A's [p0, p1)) number at most 64 and are summed in the two-row layout.
+// Per row the arithmetic is the single tile's: ascending-offset FMA chain with
+// the zero and column-range skips, then y = (blocked sum) + fallback.
+template <typename T, bool HAS_COO, bool X32>
+__device__ __forceinline__ void tile_pair(int value_line_a, int value_line_b, int rows, int row0, int num_rows, int bw,
+                                          int cnt, const int* off, int p0, int p1, int p2, int fused,
+                                          const T* __restrict__ vals, const int* __restrict__ cr,
+                                          const int* __restrict__ cc, const T* __restrict__ cv,
+                                          const T* __restrict__ xp, T* __restrict__ yp, int lane, T* sv, T* sx,
+                                          int* slo, int* shi) {
+  using V2 = typename Vec2<T>::type;
+  const T* x = xp + rows;
+  T* y = yp + rows;
+  const int64_t nrows = num_rows;
+  const int64_t row = static_cast<int64_t>(row0) + 2 * lane;
+  const bool act_a = row < nrows, act_b = row + 1 < nrows;
+  const int64_t vline = lane < kWave / 2 ? static_cast<int64_t>(value_line_a) * kWave + 2 * lane
+                                         : static_cast<int64_t>(value_line_b) * kWave + 2 * (lane - kWave / 2);
+  const T* vb = vals + vline;
+  const int lastk = cnt - 1;
+  V2 v[kChunk], xv[kChunk];
+#pragma unroll
+  for (int k = 0; k < kChunk; ++k)
+    v[k] = __builtin_nontemporal_load(reinterpret_cast<const V2*>(vb + static_cast<int64_t>(min(k, lastk)) * bw));
+  const int64_t last = nrows - 1;
+#pragma unroll
+  for (int k = 0; k < kChunk; ++k) {
+    const int64_t o = off[k];
+    if (row0 + o >= 0 && row0 + o + 2 * kWave <= last + 1) {  // wave-uniform: every column inside x
+      xv[k] = *reinterpret_cast<const V2*>(x + (row + o));
+    } else {
+      int64_t c0 = row + o, c1 = row + 1 + o;
+      c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+      c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+      xv[k].x = x[c0];
+      xv[k].y = x[c1];
+    }
+  }
+  T ca = T(0), cb = T(0);
+  if constexpr (HAS_COO) {
+    if (fused && p0 < p2)
+      coo_pair_sum<T, true>(cr, cc, cv, x, p0, p1, p2, row0, lane, sv, sx, slo, shi, ca, cb);
+  }
+  const uint32_t n32 = static_cast<uint32_t>(num_rows);
+  T sa = T(0), sb = T(0);
+#pragma unroll
+  for (int k = 0; k < kChunk; ++k) {
+    const uint32_t c = static_cast<uint32_t>(row) + static_cast<uint32_t>(off[k]);
+    const bool oka = (k < cnt) & act_a & (v[k].x != T(0)) & (c < n32);
+    const bool okb = (k < cnt) & act_b & (v[k].y != T(0)) & (c + 1u < n32);
+    sa = masked_fma(oka, v[k].x, xv[k].x, sa);
+    sb = masked_fma(okb, v[k].y, xv[k].y, sb);
+  }
+  if (HAS_COO && fused) {
+    sa = sa + ca;
+    sb = sb + cb;
+  }
+  if (act_b) {
+    V2 t;
+    t.x = sa;
+    t.y = sb;
+    __builtin_nontemporal_store(t, reinterpret_cast<V2*>(y + row));
+  } else if (act_a) {
+    __builtin_nontemporal_store(sa, y + row);
+  }
+}
+
 constexpr int kTileGroup = 16;  // tile records fetched together by one wave
 constexpr int kTileDwords = 20;  // 80-byte tile record
 static_assert(sizeof(hbdia_batch_tile_t) == kTileDwords * 4, "tile record layout");
@@ -1502,8 +1573,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
   static_assert(!(JOBS && CONTIG), "jobs run on the interleaved schedule");
   __shared__ T s_v[kWavesPerWG][kWave];
   __shared__ T s_x[kWavesPerWG][kWave];
-  __shared__ int s_lo[kWavesPerWG][kWave];
-  __shared__ int s_hi[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][2 * kWave];  // 128 rows for a pair of tiles
+  __shared__ int s_hi[kWavesPerWG][2 * kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
@@ -1583,10 +1654,28 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
     Fields nextf = read_fields(0);
     for (int jj = 0; jj < ng; ++jj) {
       const Fields cf = nextf;
+      if (jj + 1 < ng) nextf = read_fields(jj + 1);
+      // ---- two consecutive tiles of one matrix whose blocks hold the same
+      // offsets (one chunk at most) and whose fallback fits one pass: 128 rows,
+      // two adjacent rows per lane, 16-byte value loads; the arithmetic per row
+      // is the single tile's below
+      bool pair = jj + 1 < ng && nextf.rows == cf.rows && nextf.row0 == cf.row0 + kWave && nextf.bw == cf.bw &&
+                  nextf.cnt == cf.cnt && cf.cnt > 0 && cf.cnt <= kChunk && nextf.fused == cf.fused &&
+                  (!HAS_COO || !cf.fused || (nextf.p0 == cf.p1 && nextf.p1 - cf.p0 <= kWave));
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) pair = pair && (k >= cf.cnt || nextf.off[k] == cf.off[k]);
+      if (pair) {
+        const Fields cg = nextf;
+        ++jj;
+        if (jj + 1 < ng) nextf = read_fields(jj + 1);
+        tile_pair<T, HAS_COO, X32>(cf.value_line, cg.value_line, cf.rows, cf.row0, cf.num_rows, cf.bw, cf.cnt,
+                                   cf.off, cf.p0, cf.p1, cg.p1, cf.fused, vals, cr, cc, cv, xp, yp, lane, sv, sx,
+                                   slo, shi);
+        continue;
+      }
       const int value_line = cf.value_line, rows = cf.rows, row0 = cf.row0, num_rows = cf.num_rows, bw = cf.bw,
                 cnt = cf.cnt, offs_at = cf.offs_at, p0 = cf.p0, p1 = cf.p1, fused = cf.fused;
       const int* off = cf.off;
-      if (jj + 1 < ng) nextf = read_fields(jj + 1);
       const int64_t nrows = num_rows;
       const int64_t row = static_cast<int64_t>(row0) + lane;  // row within the matrix
       const bool active = row < nrows;
