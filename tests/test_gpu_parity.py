@@ -327,6 +327,90 @@ def test_runs3_hint_is_bit_identical(gpu, N, bw, with_coo, dtype):
     np.testing.assert_array_equal(outs[1], want)
 
 
+def _shared_runs3_case(N, bw, seed, coo):
+    """Runs-of-three layouts whose consecutive blocks mostly share one offset list (as a stencil's
+    do), so that the multi-row waves take their shared-list path, with every way out of it mixed in:
+    a block whose list differs in one offset, one with an extra run, an empty block, a list longer
+    than 27 offsets, and offsets whose x windows cross either end of x.  coo: None, "light" (about
+    an eighth of an entry per row) or "heavy" (one and a half per row)."""
+    rng = np.random.default_rng(seed)
+    nb = -(-N // bw)
+    group = max(1, 256 // bw)
+
+    def runs(k):
+        centres = 3 * np.sort(rng.choice(np.arange(-N // 6, N // 6), size=k, replace=False)) + 1
+        if rng.random() < 0.3:  # windows across the ends of x
+            centres[0], centres[-1] = -N + 2, N - 2
+            centres = np.unique(centres)
+        return [int(c + d) for c in centres for d in (-1, 0, 1)]
+
+    offs_per = []
+    for g0 in range(0, nb, group):
+        base = runs(int(rng.integers(1, 11)))  # 3 .. 30 offsets
+        for j in range(min(group, nb - g0)):
+            o = list(base)
+            u = rng.random()
+            if u < 0.06:
+                o = o + [o[-1] + 2, o[-1] + 3, o[-1] + 4]  # an extra run
+            elif u < 0.12:
+                o = o[:-3] + [v + 3 for v in o[-3:]]  # the last run moved
+            elif u < 0.16:
+                o = []
+            elif u < 0.2:
+                o = runs(max(1, len(base) // 3))
+            offs_per.append(o)
+    block_start = np.zeros(nb + 1, np.int32)
+    block_start[1:] = np.cumsum([len(o) for o in offs_per])
+    offsets = np.array(sum(offs_per, []), np.int32)
+    values = rng.standard_normal(len(offsets) * bw)
+    values[rng.random(values.size) < 0.2] = 0.0
+    x = rng.standard_normal(N)
+    x[rng.random(N) < 0.02] = np.nan
+    x[np.isnan(x) & (rng.random(N) < 0.5)] = 0.0
+    cooc = None
+    if coo is not None:
+        m = N // 8 if coo == "light" else 3 * N // 2
+        r = np.sort(rng.integers(0, N, m)).astype(np.int32)
+        c = rng.integers(0, N, m).astype(np.int32)
+        cooc = (r, c, rng.standard_normal(m))
+    return values, offsets, block_start, x, cooc
+
+
+@pytest.mark.parametrize("N,bw,coo,dtype", [(5000, 64, "light", np.float32), (5000, 64, "heavy", np.float32),
+                                            (5000, 64, None, np.float32), (4099, 128, "light", np.float32),
+                                            (3000, 256, None, np.float32), (1000, 512, "heavy", np.float32),
+                                            (300, 64, "light", np.float32), (65, 64, None, np.float32),
+                                            (5000, 64, "light", np.float64), (4099, 128, "heavy", np.float64)])
+def test_shared_offset_lists_multi_row_waves(gpu, N, bw, coo, dtype):
+    """Consecutive blocks with one offset list: the FP32 four-row and FP64 two-row waves (shared-list
+    path, per-block path, one-pass and per-group fallback sums) equal the hint-free launch and the
+    oracle bit for bit."""
+    print(f"case: shared runs3 N={N} bw={bw} coo={coo} {np.dtype(dtype).name}", flush=True)
+    values, offsets, block_start, x, cooc = _shared_runs3_case(N, bw, 7 * N + bw, coo)
+    values, x = values.astype(dtype), x.astype(dtype)
+    if cooc is not None:
+        cooc = (cooc[0], cooc[1], cooc[2].astype(dtype))
+    L = _layout(values, offsets, block_start, N, bw, cooc)
+    assert L.runs3 is not None
+    y0 = np.linspace(-1.0, 1.0, N).astype(dtype)
+    X = DeviceArray.from_host(x)
+    outs = []
+    for hints in (False, True):
+        Y = DeviceArray.from_host(y0)
+        L.spmv(X, Y, hints=hints)
+        synchronize()
+        outs.append(Y.download())
+    hb = {"values": values, "offsets": offsets, "block_start": block_start, "num_blocks": len(block_start) - 1,
+          "block_width": bw}
+    if cooc is not None:
+        hb.update(coo_rows=cooc[0], coo_cols=cooc[1], coo_values=cooc[2])
+    else:
+        hb.update(coo_rows=np.zeros(0, np.int32), coo_cols=np.zeros(0, np.int32), coo_values=np.zeros(0, values.dtype))
+    want = oracle.hbdia_spmv(hb, x.copy(), y0.copy(), N)
+    assert outs[0].tobytes() == want.tobytes()
+    assert outs[1].tobytes() == want.tobytes()
+
+
 @pytest.mark.parametrize("N,cover,bw,with_coo,dtype", [(1000, 700, 64, True, np.float64), (1000, 700, 64, False, np.float64),
                                                        (4096, 3000, 32, True, np.float64), (2000, 1100, 128, True, np.float64),
                                                        (1000, 700, 64, True, np.float32), (4096, 3000, 32, False, np.float32)])
