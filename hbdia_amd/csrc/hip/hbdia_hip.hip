@@ -1280,16 +1280,262 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi(
                                           runs3_blocks, grp, j0, kb, lane, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv]);
 }
 
+// __builtin_amdgcn_class mask of the finite non-zero doubles (normal and
+// subnormal, either sign).
+constexpr int kFiniteNonzero = 0x198;
+
+// Element i of one column of X: a 32-bit byte offset from the column's
+// (uniform) base when the column fits 4 GiB.
+template <bool X32>
+__device__ __forceinline__ const double* x_at(const double* xj, int64_t i) {
+  if constexpr (X32)
+    return reinterpret_cast<const double*>(reinterpret_cast<const char*>(xj) +
+                                           static_cast<uint64_t>(static_cast<uint32_t>(i) * 8u));
+  return xj + i;
+}
+
+// Whole-wave shifts with zero fill: lane i gets lane i - 1's value (lane 0
+// gets 0), or lane i + 1's (lane 63 gets 0).
+__device__ __forceinline__ double wave_shr1_zero(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(b), 0x138, 0xF, 0xF, true));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), 0x138, 0xF, 0xF, true));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double wave_shl1_zero(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(b), 0x130, 0xF, 0xF, true));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), 0x130, 0xF, 0xF, true));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// The blocked sums of a 128-row wave's two adjacent rows per lane (row, row +
+// 1) for kb <= KB columns of X (column j at Xj + j ldx; columns past kb repeat
+// column kb - 1 and are not stored), over an offset list of runs of three
+// whose count is a multiple of 3.  The rows' values of an offset are one
+// 16-byte load, shared by the columns.  Per column and run: the centre pair
+// is one 16-byte load, the outer neighbours come by zero-fill DPP shifts from
+// the adjacent lanes, and the two values beyond the wave's ends (left of lane
+// 0's rows, right of lane 63's) are scalar loads, each added by one more FMA
+// whose value is zero outside its lane.
+//
+// Every term that the exact sum skips (a zero value, a column outside the
+// matrix) or that is added in place of another (a shifted-in zero, an edge
+// FMA in another lane) is a zero value times an x, or a value times zero.
+// With that x finite it adds a signed zero, which leaves a non-zero sum
+// unchanged and a zero sum zero; an infinite or NaN x makes the sum NaN.  The
+// remaining terms are the exact sum's FMAs in its order.  So a sum that ends
+// finite and non-zero equals the exact sum bit for bit; the caller recomputes
+// a wave in which any stored sum does not.  A run whose columns all lie inside
+// the matrix for every lane of the wave needs no per-term test at all.
+// One run's loads for multi_row2_fast.
+template <int KB>
+struct MultiRun {
+  dbl2a8 v[3];   // the rows' values of the run's offsets
+  dbl2a8 m[KB];  // per column: x at the run's centre for both rows
+  double el[KB], er[KB];  // per column: x left of lane 0's rows, right of lane 63's
+  int off[3];
+  bool inside;   // every column of the run inside the matrix for every lane
+};
+
+template <int KB, bool X32, bool NT>
+__device__ __forceinline__ void multi_run_load(MultiRun<KB>& q, const double* __restrict__ vb, int64_t vlane,
+                                               const int* __restrict__ op, int o0, int bw,
+                                               const double* const* xc, int64_t row0, int64_t row,
+                                               int64_t last) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) q.off[k] = op[o0 + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const dbl2a8* p = reinterpret_cast<const dbl2a8*>(vb + static_cast<int64_t>(o0 + k) * bw + vlane);
+    q.v[k] = NT ? __builtin_nontemporal_load(p) : *p;
+  }
+  const int64_t o = q.off[1];
+  const int64_t lo = row0 + o - 1, hi = row0 + o + 2 * kWave;  // wave-uniform
+  q.inside = lo >= 0 && hi <= last;
+  const int64_t cl = lo < 0 ? 0 : (lo > last ? last : lo), ch = hi < 0 ? 0 : (hi > last ? last : hi);
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    q.el[j] = *x_at<X32>(xc[j], cl);  // scalar loads
+    q.er[j] = *x_at<X32>(xc[j], ch);
+  }
+  if (q.inside) {
+#pragma unroll
+    for (int j = 0; j < KB; ++j) q.m[j] = *reinterpret_cast<const dbl2a8*>(x_at<X32>(xc[j], row + o));
+  } else {
+    int64_t c0 = row + o, c1 = row + 1 + o;
+    c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+    c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      q.m[j].x = *x_at<X32>(xc[j], c0);
+      q.m[j].y = *x_at<X32>(xc[j], c1);
+    }
+  }
+}
+
+// The FMAs of one run: va / vbb are the rows' values, vl / vr those of the
+// edge terms (x[lo] for lane 0's first row, x[hi] for lane 63's second).
+template <int KB>
+__device__ __forceinline__ void multi_run_fma(const MultiRun<KB>& q, const double* va, const double* vbb, int lane,
+                                              double* sa, double* sb) {
+  const double vl = lane == 0 ? va[0] : 0.0;
+  const double vr = lane == kWave - 1 ? vbb[2] : 0.0;
+#pragma unroll
+  for (int j = 0; j < KB; ++j) {
+    const double left = wave_shr1_zero(q.m[j].y), right = wave_shl1_zero(q.m[j].x);
+    sa[j] = __builtin_fma(va[0], left, sa[j]);
+    sa[j] = __builtin_fma(vl, q.el[j], sa[j]);
+    sa[j] = __builtin_fma(va[1], q.m[j].x, sa[j]);
+    sa[j] = __builtin_fma(va[2], q.m[j].y, sa[j]);
+    sb[j] = __builtin_fma(vbb[0], q.m[j].x, sb[j]);
+    sb[j] = __builtin_fma(vbb[1], q.m[j].y, sb[j]);
+    sb[j] = __builtin_fma(vbb[2], right, sb[j]);
+    sb[j] = __builtin_fma(vr, q.er[j], sb[j]);
+  }
+}
+
+template <int KB>
+__device__ __forceinline__ void multi_run_sum(const MultiRun<KB>& q, uint32_t row32, uint32_t n32, int lane,
+                                              double* sa, double* sb) {
+  if (q.inside) {
+    const double va[3] = {q.v[0].x, q.v[1].x, q.v[2].x}, vbb[3] = {q.v[0].y, q.v[1].y, q.v[2].y};
+    multi_run_fma<KB>(q, va, vbb, lane, sa, sb);
+  } else {
+    // columns outside the matrix: their values as zeros
+    double va[3], vbb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t ca = row32 + static_cast<uint32_t>(q.off[k]);
+      va[k] = ca < n32 ? q.v[k].x : 0.0;
+      vbb[k] = ca + 1u < n32 ? q.v[k].y : 0.0;
+    }
+    multi_run_fma<KB>(q, va, vbb, lane, sa, sb);
+  }
+}
+
+// The blocked sums of a 128-row wave's two adjacent rows per lane (row, row +
+// 1) for kb <= KB columns of X (column j at Xj + j ldx; columns past kb repeat
+// column kb - 1 and are not stored), over an offset list of runs of three
+// whose count is a multiple of 3.  The rows' values of an offset are one
+// 16-byte load, shared by the columns (non-temporal when NT: no later column
+// pass reads them).  Per column and run: the centre pair is one 16-byte load,
+// the outer neighbours come by zero-fill DPP shifts from the adjacent lanes,
+// and the two values beyond the wave's ends (left of lane 0's rows, right of
+// lane 63's) are scalar loads, each added by one more FMA whose value is zero
+// outside its lane.  (Issuing each run's loads before the previous run's FMAs
+// measured no faster: the launch is bound by L2 requests, not latency.)
+//
+// Every term that the exact sum skips (a zero value, a column outside the
+// matrix) or that is added in place of another (a shifted-in zero, an edge
+// FMA in another lane) is a zero value times an x, or a value times zero.
+// With that x finite it adds a signed zero, which leaves a non-zero sum
+// unchanged and a zero sum zero; an infinite or NaN x makes the sum NaN.  The
+// remaining terms are the exact sum's FMAs in its order.  So a sum that ends
+// finite and non-zero equals the exact sum bit for bit; the caller recomputes
+// a wave in which any stored sum does not.  A run whose columns all lie inside
+// the matrix for every lane of the wave needs no per-term test at all.
+template <int KB, bool X32, bool NT>
+__device__ __forceinline__ void multi_row2_fast(const double* __restrict__ vb, int64_t vlane,
+                                                const int* __restrict__ op, int cnt, int bw,
+                                                const double* __restrict__ Xj, int64_t ldx, int kb, int64_t row0,
+                                                int64_t row, int64_t nrows, int lane, double* sa, double* sb) {
+  const int64_t last = nrows - 1;
+  const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
+  const double* xc[KB];
+#pragma unroll
+  for (int j = 0; j < KB; ++j) xc[j] = Xj + static_cast<int64_t>(min(j, kb - 1)) * ldx;
+#pragma unroll
+  for (int j = 0; j < KB; ++j) sa[j] = sb[j] = 0.0;
+  for (int o0 = 0; o0 < cnt; o0 += 3) {  // one run of three per step
+    MultiRun<KB> q;
+    multi_run_load<KB, X32, NT>(q, vb, vlane, op, o0, bw, xc, row0, row, last);
+    multi_run_sum<KB>(q, row32, n32, lane, sa, sb);
+  }
+}
+
+// multi_row2_fast's sums with every skip of the exact order applied per term.
+template <int KB>
+__device__ __forceinline__ void multi_row2_exact(const double* __restrict__ vb, int64_t vlane,
+                                                 const int* __restrict__ op, int cnt, int bw,
+                                                 const double* __restrict__ Xj, int64_t ldx, int kb, int64_t row0,
+                                                 int64_t row, int64_t nrows, double* sa, double* sb) {
+  const int64_t last = nrows - 1;
+  const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
+#pragma unroll
+  for (int j = 0; j < KB; ++j) sa[j] = sb[j] = 0.0;
+  for (int o0 = 0; o0 < cnt; o0 += 3) {  // one run of three per step
+    int off[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) off[k] = op[o0 + k];  // padded array
+    dbl2a8 v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      v[k] = __builtin_nontemporal_load(
+          reinterpret_cast<const dbl2a8*>(vb + static_cast<int64_t>(min(o0 + k, cnt - 1)) * bw + vlane));
+    // every column's centre pair and edge values before any FMA
+    const int64_t o = off[1];
+    const bool inside = row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last;  // wave-uniform
+    int64_t cf = row0 + o - 1, ce = row0 + o + 2 * kWave;
+    cf = cf < 0 ? 0 : (cf > last ? last : cf);
+    ce = ce < 0 ? 0 : (ce > last ? last : ce);
+    int64_t c0 = row + o, c1 = row + 1 + o;
+    c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+    c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+    dbl2a8 m[KB];
+    double first[KB], after[KB];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j < kb) {
+        const double* xj = Xj + static_cast<int64_t>(j) * ldx;
+        if (inside) {
+          m[j] = *reinterpret_cast<const dbl2a8*>(xj + (row + o));
+        } else {
+          m[j].x = xj[c0];
+          m[j].y = xj[c1];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j < kb) {
+        const double* xj = Xj + static_cast<int64_t>(j) * ldx;
+        first[j] = xj[cf];  // wave-uniform: scalar loads
+        after[j] = xj[ce];
+      }
+    }
+    // rows past the end compute sums that are never stored
+    bool oka[3], okb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t ca = row32 + static_cast<uint32_t>(off[k]), cb = ca + 1u;
+      oka[k] = (o0 + k < cnt) & (v[k].x != 0.0) & (ca < n32);
+      okb[k] = (o0 + k < cnt) & (v[k].y != 0.0) & (cb < n32);
+    }
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      if (j >= kb) break;
+      const double xa[3] = {wave_shr1(m[j].y, first[j]), m[j].x, m[j].y};
+      const double xb[3] = {m[j].x, m[j].y, wave_shl1(m[j].x, after[j])};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        sa[j] = masked_fma(oka[k], v[k].x, xa[k], sa[j]);
+        sb[j] = masked_fma(okb[k], v[k].y, xb[k], sb[j]);
+      }
+    }
+  }
+}
+
 // FP64 multi-RHS, block width a multiple of 64, runs-of-three hints: 128-row
 // waves with two adjacent rows per lane, as k_hbdia_row2.  A wave whose two
-// halves share one offset list loads both rows' values of an offset as one
-// 16-byte access (shared by the columns) and, per column, both rows' x of a
-// run's centre as one 16-byte access, with the neighbours shifted in by DPP
-// and the edge values as scalar loads.  Other waves run the 64-row code per
-// half.  Each column's values, skips and FMA order per row are the
-// single-vector kernel's, so every column equals a single-vector SpMV bit for
-// bit.
-template <int KB, bool HAS_COO, bool X32>
+// halves share one offset list sums with multi_row2_fast (the skips folded
+// into the shared values, no per-term test per column) and recomputes with
+// every skip applied only when a stored sum is zero or not finite (NT: one
+// column pass, the values read non-temporally).  Other
+// waves run the 64-row code per half.  Each column's values, skips and FMA
+// order per row are the single-vector kernel's, so every column equals a
+// single-vector SpMV bit for bit.
+template <int KB, bool HAS_COO, bool X32, bool NT>
 __global__ __launch_bounds__(kWG) void k_hbdia_multi_row2(
     const double* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const double* __restrict__ X, int64_t ldx, double* __restrict__ Y, int64_t ldy, int nrhs, int64_t nrows,
@@ -1346,75 +1592,23 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi_row2(
   double* Yj = Y + static_cast<int64_t>(j0) * ldy;
   const int64_t row = row0 + 2 * lane;
   const bool act_a = row < nrows, act_b = row + 1 < nrows;
-  const int64_t last = nrows - 1;
-  const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
   const double* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
   // lanes of half B read block bB's lines, which follow block bA's
   const int64_t vlane = 2 * lane + ((lane >= kWave / 2 && bB != bA) ? static_cast<int64_t>(cA - 1) * bw : 0);
   const int* op = offs + sA;
   double sa[KB], sb[KB];
+  multi_row2_fast<KB, X32, NT>(vb, vlane, op, cA, bw, Xj, ldx, kb, row0, row, nrows, lane, sa, sb);
+  // a finite non-zero sum is the exact sum bit for bit; a wave with another
+  // (a zero, an infinity or NaN) in a row it stores recomputes every skip
+  bool redo = false;
 #pragma unroll
-  for (int j = 0; j < KB; ++j) sa[j] = sb[j] = 0.0;
-  for (int o0 = 0; o0 < cA; o0 += 3) {  // one run of three per step
-    int off[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) off[k] = op[o0 + k];  // padded array
-    dbl2a8 v[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      v[k] = __builtin_nontemporal_load(
-          reinterpret_cast<const dbl2a8*>(vb + static_cast<int64_t>(min(o0 + k, cA - 1)) * bw + vlane));
-    // every column's centre pair and edge values before any FMA
-    const int64_t o = off[1];
-    const bool inside = row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last;  // wave-uniform
-    int64_t cf = row0 + o - 1, ce = row0 + o + 2 * kWave;
-    cf = cf < 0 ? 0 : (cf > last ? last : cf);
-    ce = ce < 0 ? 0 : (ce > last ? last : ce);
-    int64_t c0 = row + o, c1 = row + 1 + o;
-    c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
-    c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
-    dbl2a8 m[KB];
-    double first[KB], after[KB];
-#pragma unroll
-    for (int j = 0; j < KB; ++j) {
-      if (j < kb) {
-        const double* xj = Xj + static_cast<int64_t>(j) * ldx;
-        if (inside) {
-          m[j] = *reinterpret_cast<const dbl2a8*>(xj + (row + o));
-        } else {
-          m[j].x = xj[c0];
-          m[j].y = xj[c1];
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < KB; ++j) {
-      if (j < kb) {
-        const double* xj = Xj + static_cast<int64_t>(j) * ldx;
-        first[j] = xj[cf];  // wave-uniform: scalar loads
-        after[j] = xj[ce];
-      }
-    }
-    // rows past the end compute sums that are never stored
-    bool oka[3], okb[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const uint32_t ca = row32 + static_cast<uint32_t>(off[k]), cb = ca + 1u;
-      oka[k] = (o0 + k < cA) & (v[k].x != 0.0) & (ca < n32);
-      okb[k] = (o0 + k < cA) & (v[k].y != 0.0) & (cb < n32);
-    }
-#pragma unroll
-    for (int j = 0; j < KB; ++j) {
-      if (j >= kb) break;
-      const double xa[3] = {wave_shr1(m[j].y, first[j]), m[j].x, m[j].y};
-      const double xb[3] = {m[j].x, m[j].y, wave_shl1(m[j].x, after[j])};
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        sa[j] = masked_fma(oka[k], v[k].x, xa[k], sa[j]);
-        sb[j] = masked_fma(okb[k], v[k].y, xb[k], sb[j]);
-      }
+  for (int j = 0; j < KB; ++j) {
+    if (j < kb) {
+      redo = redo | (act_a & !__builtin_amdgcn_class(sa[j], kFiniteNonzero));
+      redo = redo | (act_b & !__builtin_amdgcn_class(sb[j], kFiniteNonzero));
     }
   }
+  if (__ballot(redo) != 0) multi_row2_exact<KB>(vb, vlane, op, cA, bw, Xj, ldx, kb, row0, row, nrows, sa, sb);
   if constexpr (!HAS_COO) {
     // both rows of a lane are adjacent in every column: one 16-byte store
 #pragma unroll
@@ -2479,11 +2673,17 @@ void launch_multi_row2_kb(const double* vals, const int32_t* offs, const int32_t
                           const int32_t* cr, const int32_t* cc, const double* cv, const int32_t* gptr,
                           const double* X, int64_t ldx, double* Y, int64_t ldy, int nrhs, int64_t nrows,
                           bool with_coo, bool x32, const uint32_t* runs3, int kshift, dim3 grid, hipStream_t s) {
-#define HBDIA_LAUNCH_M2(C, XX)                                                                                \
-  hipLaunchKernelGGL((k_hbdia_multi_row2<KB, C, XX>), grid, dim3(kWG), 0, s, vals, offs, blk, X, ldx, Y, ldy, \
+#define HBDIA_LAUNCH_M2(C, XX, N)                                                                                \
+  hipLaunchKernelGGL((k_hbdia_multi_row2<KB, C, XX, N>), grid, dim3(kWG), 0, s, vals, offs, blk, X, ldx, Y, ldy, \
                      nrhs, nrows, nb, bw, cr, cc, cv, gptr, runs3, kshift)
-  if (with_coo) { if (x32) HBDIA_LAUNCH_M2(true, true); else HBDIA_LAUNCH_M2(true, false); }
-  else { if (x32) HBDIA_LAUNCH_M2(false, true); else HBDIA_LAUNCH_M2(false, false); }
+  // one column pass streams the values; with more, the later passes find them in L2
+  if (nrhs <= KB) {
+    if (with_coo) { if (x32) HBDIA_LAUNCH_M2(true, true, true); else HBDIA_LAUNCH_M2(true, false, true); }
+    else { if (x32) HBDIA_LAUNCH_M2(false, true, true); else HBDIA_LAUNCH_M2(false, false, true); }
+  } else {
+    if (with_coo) { if (x32) HBDIA_LAUNCH_M2(true, true, false); else HBDIA_LAUNCH_M2(true, false, false); }
+    else { if (x32) HBDIA_LAUNCH_M2(false, true, false); else HBDIA_LAUNCH_M2(false, false, false); }
+  }
 #undef HBDIA_LAUNCH_M2
 }
 
@@ -2517,14 +2717,17 @@ int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
   hipStream_t s = hs(stream);
   if constexpr (std::is_same<T, double>::value) {
     if (runs3) {
-      // 128-row waves, two rows per lane, at most 4 columns per pass (8 need
-      // about 128 VGPRs and spill scalars: slower); a row block's column passes
-      // are adjacent work items, so the later ones read its values from L2
-      const int kb2 = kb > 4 ? 4 : kb;
+      // 128-row waves, two rows per lane, up to 8 columns per pass; a row
+      // block's column passes are adjacent work items, so the later ones read
+      // its values from L2.  The launch is bound by L2 requests (the values
+      // once per pass, x nine times per column): 8 columns per pass measured
+      // k = 32 at 0.617 ms against 0.668 ms with 4.
+      const int kb2 = kb;
       const int64_t items2 = (nrows + 2 * kWG - 1) / (2 * kWG) * ((nrhs + kb2 - 1) / kb2);
       const dim3 grid2(static_cast<unsigned>(items2));
       const int ks2 = xcd_chunk_shift(items2);
       switch (kb2) {
+        case 8: launch_multi_row2_kb<8>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks2, grid2, s); break;
         case 4: launch_multi_row2_kb<4>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks2, grid2, s); break;
         case 2: launch_multi_row2_kb<2>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks2, grid2, s); break;
         default: launch_multi_row2_kb<1>(vals, offs, blk, nb, bw, cr, cc, cv, gptr, X, ldx, Y, ldy, nrhs, nrows, with_coo, x32, runs3, ks2, grid2, s); break;

@@ -533,6 +533,45 @@ def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k, dtype):
                                       err_msg=f"column {j}")
 
 
+@pytest.mark.parametrize("k", [5, 8])
+def test_multi_rhs_zero_and_nonfinite_sums_bit_exact(gpu, k):
+    """The FP64 multi-RHS kernel sums without per-term skips and redoes a wave with them when a stored
+    sum is zero, infinite or NaN.  Columns whose sums underflow to -0, are +0, pass through an
+    infinity or NaN, or stay finite and non-zero all equal the oracle and the hint-free launch bit for
+    bit, signs of zero included."""
+    print(f"case: multi-RHS zero and non-finite sums k={k}", flush=True)
+    n = 16
+    A = HBDIA.stencil(n, 27, 0.0, seed=3)
+    A.convertToHBDIAFormat(64, 2)
+    hb = A.getHBDIA()
+    N = n ** 3
+    values = np.abs(hb["values"]) * 1e-200  # products with x of -1e-200 underflow to -0
+    L = kernels.DeviceLayout(values, hb["offsets"], hb["block_start"], N, 64)
+    assert L.runs3 is not None
+    cols = [gen_vector(N, 5), np.full(N, -1e-200), np.full(N, -0.0), np.zeros(N), gen_vector(N, 6),
+            gen_vector(N, 7), np.full(N, 1e-200), gen_vector(N, 8)]
+    cols[4][N // 3] = np.nan
+    cols[5][[7, N // 2]] = [np.inf, -np.inf]
+    cols[7][: N // 2] = -1e-200
+    Xh = np.vstack(cols[:k])
+    Y0 = np.full((k, N), 0.5)
+    X = DeviceArray.from_host(Xh.ravel())
+    outs = []
+    for hints in (True, False):
+        Y = DeviceArray.from_host(Y0.ravel())
+        L.spmm(X, Y, k, hints=hints)
+        synchronize()
+        outs.append(Y.download().reshape(k, N))
+    assert outs[0].tobytes() == outs[1].tobytes()
+    hbo = {"values": values, "offsets": hb["offsets"], "block_start": hb["block_start"],
+           "num_blocks": A.getNumBlocks(), "block_width": 64, "coo_rows": np.zeros(0, np.int32),
+           "coo_cols": np.zeros(0, np.int32), "coo_values": np.zeros(0)}
+    for j in range(k):
+        want = oracle.hbdia_spmv(hbo, Xh[j].copy(), Y0[j].copy(), N)
+        assert outs[0][j].tobytes() == want.tobytes(), f"column {j}"
+    assert np.signbit(outs[0][1]).all() and (outs[0][1] == 0).all()  # the -0 sums survived
+
+
 def _width32_case(N, seed, with_coo, share, group=2, kmax=5):
     """Block width 32: blocks after the first of each `group` (2: pairs 2k, 2k+1; 4: quads 4k..4k+3)
     that with probability `share` repeat the previous block's offset list (then one wave runs them
