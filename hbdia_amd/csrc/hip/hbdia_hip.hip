@@ -2134,24 +2134,6 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
   }
 }
 
-// Two adjacent rows per lane for the reduced-precision kernel (block width a
-// multiple of 64, runs-of-three hints): a 128-row wave loads both rows' stored
-// values of an offset in one access and both rows' x of a run's centre in one
-// 16-byte load; the per-row arithmetic, flags and stores are k_hbdia_mixed's.
-
-// both rows' stored values of one offset: p at an even element
-__device__ __forceinline__ void lp_pair(const float* p, float& a, float& b) {
-  typedef float f2a8 __attribute__((ext_vector_type(2), aligned(8)));
-  const f2a8 t = __builtin_nontemporal_load(reinterpret_cast<const f2a8*>(p));
-  a = t.x;
-  b = t.y;
-}
-__device__ __forceinline__ void lp_pair(const uint16_t* p, float& a, float& b) {
-  const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
-  a = bf16_to_f32(static_cast<uint16_t>(w & 0xffffu));
-  b = bf16_to_f32(static_cast<uint16_t>(w >> 16));
-}
-
 // FP32 sum and sum of |a x| of one row, the blocked part (rows r0 + lane of a
 // 64-row half), exactly k_hbdia_mixed's arithmetic
 template <typename TV>
@@ -2235,54 +2217,6 @@ __device__ __forceinline__ void mixed_coo_group(int p0, int p1, int64_t r0, int 
   }
 }
 
-// mixed_coo_group for both 64-row groups of a 128-row wave whose entries
-// [p0, p2) fit one 64-entry pass (group A [p0, p1), group B [p1, p2)): one
-// load for both, then the same per-row sums in entry order.  s_lo and s_hi
-// hold 128 entries.
-template <typename TV>
-__device__ __forceinline__ void mixed_coo_pair(int p0, int p1, int p2, int64_t row0, int lane,
-                                               const double* __restrict__ x, const int* __restrict__ cr,
-                                               const int* __restrict__ cc, const TV* __restrict__ cv, float* s_v,
-                                               float* s_x, int* s_lo, int* s_hi, float* cs, float* cabs,
-                                               int* nterms) {
-  const int i = p0 + lane;
-  const bool ok = i < p2;
-  int r = -1;
-  float vv = 0.0f, xx = 0.0f;
-  if (ok) {
-    r = cr[i] - static_cast<int>(row0);
-    vv = lp_to_f32<TV>(cv[i]);
-    xx = static_cast<float>(x[cc[i]]);
-    // an entry outside its own group's rows (unsorted input) is dropped
-    if (r < 0 || r >= 2 * kWave || ((i < p1) != (r < kWave))) r = -1;
-  }
-  const int rp = __shfl_up(r, 1);
-  const int rn = __shfl_down(r, 1);
-  s_lo[lane] = 0;
-  s_lo[kWave + lane] = 0;
-  s_hi[lane] = 0;
-  s_hi[kWave + lane] = 0;
-  s_v[lane] = vv;
-  s_x[lane] = xx;
-  wave_sync();
-  if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
-  if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p2 || rn != r)) s_hi[r] = lane + 1;
-  wave_sync();
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int a = s_lo[h * kWave + lane], e = s_hi[h * kWave + lane];
-    nterms[h] += e - a;
-    float c = 0.0f, ca = 0.0f;
-    for (int k = a; k < e; ++k) {
-      c = __builtin_fmaf(s_v[k], s_x[k], c);
-      ca = __builtin_fmaf(__builtin_fabsf(s_v[k]), __builtin_fabsf(s_x[k]), ca);
-    }
-    cs[h] = c;
-    cabs[h] = ca;
-  }
-  wave_sync();
-}
-
 // The cancellation flag and the store of one 64-row group (rows r0 + lane)
 // from its blocked and fallback sums, exactly k_hbdia_mixed's
 template <bool HAS_COO>
@@ -2308,8 +2242,89 @@ __device__ __forceinline__ void mixed_store(int64_t grp, int64_t r0, int lane, f
   }
 }
 
+// Four adjacent rows per lane for the reduced-precision kernel (block width a
+// multiple of 64, runs-of-three hints): a 256-row wave whose four 64-row
+// quarters share one offset list loads the four rows' stored values of an
+// offset as one access (16 bytes FP32, 8 bytes BF16) and the four rows' x of a
+// run's centre as two 16-byte loads; the run's outer neighbours come by DPP
+// (converted to FP32 first) and the two values beyond the wave's ends are
+// scalar loads.  Per run the four rows' FP32 sums take every offset of the run
+// as an FMA: a zero value adds a signed zero, which leaves the sums and the
+// bound unchanged, and an infinite or NaN x makes the row's sum NaN, which
+// flags it for FP64.  So a run whose columns lie inside the matrix for every
+// lane needs no per-term test; other runs zero the values of columns outside
+// the matrix.  The flags, fallback sums and stores are k_hbdia_mixed's per
+// 64-row group, after the sums are moved back to row-per-lane order in LDS.
+
+typedef float flt4a4m __attribute__((ext_vector_type(4), aligned(4)));
+typedef unsigned u32x2a4 __attribute__((ext_vector_type(2), aligned(4)));
+
+// the four rows' stored values of one offset (p at a multiple of 4 elements)
+__device__ __forceinline__ void lp_quad(const float* p, float* v) {
+  const flt4a4m t = __builtin_nontemporal_load(reinterpret_cast<const flt4a4m*>(p));
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+  v[3] = t.w;
+}
+__device__ __forceinline__ void lp_quad(const uint16_t* p, float* v) {
+  const u32x2a4 t = __builtin_nontemporal_load(reinterpret_cast<const u32x2a4*>(p));
+  v[0] = __uint_as_float(t.x << 16);
+  v[1] = __uint_as_float(t.x & 0xffff0000u);
+  v[2] = __uint_as_float(t.y << 16);
+  v[3] = __uint_as_float(t.y & 0xffff0000u);
+}
+
+// mixed_coo_group for the four 64-row groups of a 256-row wave whose entries
+// [p[0], p[4]) fit one 64-entry pass (group g holds [p[g], p[g + 1]), rows
+// row0 + 64 g ...): one load for all, then the same per-row sums in entry
+// order.  s_lo and s_hi hold 256 entries.
+template <typename TV>
+__device__ __forceinline__ void mixed_coo_quad(const int* p, int64_t row0, int lane, const double* __restrict__ x,
+                                               const int* __restrict__ cr, const int* __restrict__ cc,
+                                               const TV* __restrict__ cv, float* s_v, float* s_x, int* s_lo,
+                                               int* s_hi, float* cs, float* cabs, int* nterms) {
+  const int i = p[0] + lane;
+  const bool ok = i < p[4];
+  int r = -1;
+  float vv = 0.0f, xx = 0.0f;
+  if (ok) {
+    r = cr[i] - static_cast<int>(row0);
+    vv = lp_to_f32<TV>(cv[i]);
+    xx = static_cast<float>(x[cc[i]]);
+    // an entry outside its own group's rows (unsorted input) is dropped
+    if (r < 0 || r >= 4 * kWave || i < p[r >> 6] || i >= p[(r >> 6) + 1]) r = -1;
+  }
+  const int rp = __shfl_up(r, 1);
+  const int rn = __shfl_down(r, 1);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    s_lo[g * kWave + lane] = 0;
+    s_hi[g * kWave + lane] = 0;
+  }
+  s_v[lane] = vv;
+  s_x[lane] = xx;
+  wave_sync();
+  if (ok && r >= 0 && (lane == 0 || rp != r)) s_lo[r] = lane;
+  if (ok && r >= 0 && (lane == kWave - 1 || i + 1 >= p[4] || rn != r)) s_hi[r] = lane + 1;
+  wave_sync();
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int a = s_lo[h * kWave + lane], e = s_hi[h * kWave + lane];
+    nterms[h] += e - a;
+    float c = 0.0f, ca = 0.0f;
+    for (int k = a; k < e; ++k) {
+      c = __builtin_fmaf(s_v[k], s_x[k], c);
+      ca = __builtin_fmaf(__builtin_fabsf(s_v[k]), __builtin_fabsf(s_x[k]), ca);
+    }
+    cs[h] = c;
+    cabs[h] = ca;
+  }
+  wave_sync();
+}
+
 template <typename TV, bool HAS_COO>
-__global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
+__global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
     const TV* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb, int bw,
     const int* __restrict__ cr, const int* __restrict__ cc, const TV* __restrict__ cv,
@@ -2317,138 +2332,144 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row2(
     const uint32_t* __restrict__ runs3_blocks, int kshift) {
   __shared__ float s_v[kWavesPerWG][kWave];
   __shared__ float s_x[kWavesPerWG][kWave];
-  __shared__ int s_lo[kWavesPerWG][2 * kWave];
-  __shared__ int s_hi[kWavesPerWG][2 * kWave];
-  __shared__ float s_y[kWavesPerWG][4 * kWave];
+  __shared__ int s_lo[kWavesPerWG][4 * kWave];
+  __shared__ int s_hi[kWavesPerWG][4 * kWave];
+  __shared__ __attribute__((aligned(16))) float s_y[kWavesPerWG][8 * kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const unsigned work = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
-  const int64_t w2 = static_cast<int64_t>(work) * kWavesPerWG + wv;  // 128-row wave
-  const int64_t row0 = w2 * 2 * kWave;
+  const int64_t w4 = static_cast<int64_t>(work) * kWavesPerWG + wv;  // 256-row wave
+  const int64_t row0 = w4 * 4 * kWave;
   if (row0 >= nrows) return;
-  const int64_t bA = row0 / bw, bB = (row0 + kWave) / bw;
-  const bool hasB = row0 + kWave < nrows;
-  int sA = 0, cA = 0, sB = 0, cB = 0;
-  if (bA < nb) {
-    sA = blk[bA];
-    cA = blk[bA + 1] - sA;
+  const int64_t ngroups = (nrows + kWave - 1) / kWave;
+  const int nq = static_cast<int>(min<int64_t>(4, ngroups - 4 * w4));  // 64-row groups present
+  int64_t bq[4];
+  int sq[4], cq[4];
+  bool rq[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    bq[q] = (row0 + q * kWave) / bw;
+    const bool have = q < nq && bq[q] < nb;
+    sq[q] = have ? blk[bq[q]] : 0;
+    cq[q] = have ? blk[bq[q] + 1] - sq[q] : 0;
+    rq[q] = have && ((runs3_blocks[bq[q] >> 5] >> (bq[q] & 31)) & 1u);
   }
-  if (hasB && bB < nb) {
-    sB = blk[bB];
-    cB = blk[bB + 1] - sB;
-  }
-  const bool runsA = bA < nb && ((runs3_blocks[bA >> 5] >> (bA & 31)) & 1u);
-  const bool runsB = hasB && bB < nb && ((runs3_blocks[bB >> 5] >> (bB & 31)) & 1u);
-  bool same = false;
-  if (runsA && cA > 0 && cA <= kWave) {
-    if (bA == bB) {
-      same = true;
-    } else if (hasB && cB == cA) {
-      const int oa = lane < cA ? offs[sA + lane] : 0;
-      const int ob = lane < cA ? offs[sB + lane] : 0;
-      same = __ballot(oa != ob) == 0;
-    }
-  }
-  float sumA, absA, sumB = 0.0f, absB = 0.0f;
+  // one offset list of runs for the whole wave: its blocks (consecutive) hold
+  // the same count and offsets
+  const int c0 = cq[0];
+  bool same = nq == 4 && rq[0] && c0 > 0 && c0 <= kWave && c0 % 3 == 0;
   if (same) {
-    const int64_t row = row0 + 2 * lane;
-    const bool act_a = row < nrows, act_b = row + 1 < nrows;
-    const int64_t last = nrows - 1;
-    const TV* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
-    const int vlane = 2 * lane + ((lane >= kWave / 2 && bB != bA) ? (cA - 1) * bw : 0);
-    const int* op = offs + sA;
-    const uint32_t n32 = static_cast<uint32_t>(nrows);
-    float sa = 0.0f, aa = 0.0f, sb = 0.0f, ab = 0.0f;
-    for (int o0 = 0; o0 < cA; o0 += kChunk) {
-      int off[kChunk];
+    const int oa = lane < c0 ? offs[sq[0] + lane] : 0;
+    bool diff = false;
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + k];  // padded array
-      float va[kChunk], vbv[kChunk];
-#pragma unroll
-      for (int k = 0; k < kChunk; ++k)
-        lp_pair(vb + static_cast<int64_t>(min(o0 + k, cA - 1)) * bw + vlane, va[k], vbv[k]);
-      // x in FP32: each run's centre pair converted once, its neighbours shifted in
-      // as FP32 (the same values as converting the shifted FP64 ones)
-      float xa[kChunk], xb[kChunk];
-#pragma unroll
-      for (int w = 0; w < kChunk / 3; ++w) {
-        // runs past the block's count read padding: clamped, their products unused
-        const int64_t o = off[3 * w + 1];
-        dbl2a8 m;
-        if (row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last) {  // wave-uniform
-          m = *reinterpret_cast<const dbl2a8*>(x + (row + o));
-        } else {
-          int64_t c0 = row + o, c1 = row + 1 + o;
-          c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
-          c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
-          m.x = x[c0];
-          m.y = x[c1];
-        }
-        int64_t cf = row0 + o - 1, ce = row0 + o + 2 * kWave;
-        cf = cf < 0 ? 0 : (cf > last ? last : cf);
-        ce = ce < 0 ? 0 : (ce > last ? last : ce);
-        const float first = static_cast<float>(x[cf]), after = static_cast<float>(x[ce]);
-        const float mx = static_cast<float>(m.x), my = static_cast<float>(m.y);
-        xa[3 * w] = wave_shr1(my, first);
-        xa[3 * w + 1] = mx;
-        xa[3 * w + 2] = my;
-        xb[3 * w] = mx;
-        xb[3 * w + 1] = my;
-        xb[3 * w + 2] = wave_shl1(mx, after);
-      }
-#pragma unroll
-      for (int k = 0; k < kChunk; ++k) {
-        // rows and columns are below 2^31: row + offset modulo 2^32 is below nrows
-        // exactly when the column is in [0, nrows); rows past the end are cleared
-        // after the loop instead of per entry
-        const uint32_t ca = static_cast<uint32_t>(row) + static_cast<uint32_t>(off[k]), cb = ca + 1u;
-        const bool oka = (o0 + k < cA) & (va[k] != 0.0f) & (ca < n32);
-        const bool okb = (o0 + k < cA) & (vbv[k] != 0.0f) & (cb < n32);
-        sa = oka ? __builtin_fmaf(va[k], xa[k], sa) : sa;
-        aa = oka ? __builtin_fmaf(__builtin_fabsf(va[k]), __builtin_fabsf(xa[k]), aa) : aa;
-        sb = okb ? __builtin_fmaf(vbv[k], xb[k], sb) : sb;
-        ab = okb ? __builtin_fmaf(__builtin_fabsf(vbv[k]), __builtin_fabsf(xb[k]), ab) : ab;
+    for (int q = 1; q < 4; ++q) {
+      if (bq[q] != bq[0]) {
+        same = same && cq[q] == c0;
+        if (cq[q] == c0) diff = diff | (oa != (lane < c0 ? offs[sq[q] + lane] : 0));
       }
     }
-    if (!act_a) sa = aa = 0.0f;
-    if (!act_b) sb = ab = 0.0f;
+    same = same && __ballot(diff) == 0;
+  }
+  float sum[4], sab[4];
+  if (same) {
+    const int64_t row = row0 + 4 * lane;
+    const int64_t last = nrows - 1;
+    const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
+    const int64_t bl = row / bw;  // the lane's block: its values follow the wave's first block's
+    const TV* vb = vals + static_cast<int64_t>(sq[0]) * bw +
+                   ((bl - bq[0]) * c0 * bw + (row - bl * bw));
+    const int* op = offs + sq[0];
+    float s[4] = {0.0f, 0.0f, 0.0f, 0.0f}, a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int o0 = 0; o0 < c0; o0 += 3) {  // one run of three per step
+      int off[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) off[k] = op[o0 + k];
+      float v[3][4];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) lp_quad(vb + static_cast<int64_t>(o0 + k) * bw, v[k]);
+      const int64_t o = off[1];
+      const int64_t lo = row0 + o - 1, hi = row0 + o + 4 * kWave;  // wave-uniform
+      const bool inside = lo >= 0 && hi <= last;
+      const int64_t cl = lo < 0 ? 0 : (lo > last ? last : lo), ch = hi < 0 ? 0 : (hi > last ? last : hi);
+      double m[4];
+      if (inside) {
+        const dbl2a8 p0 = *reinterpret_cast<const dbl2a8*>(x + (row + o));
+        const dbl2a8 p1 = *reinterpret_cast<const dbl2a8*>(x + (row + o + 2));
+        m[0] = p0.x;
+        m[1] = p0.y;
+        m[2] = p1.x;
+        m[3] = p1.y;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          int64_t c = row + i + o;
+          c = c < 0 ? 0 : (c > last ? last : c);
+          m[i] = x[c];
+        }
+        // columns outside the matrix: their values as zeros
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t c = row32 + static_cast<uint32_t>(off[k]);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[k][i] = c + static_cast<uint32_t>(i) < n32 ? v[k][i] : 0.0f;
+        }
+      }
+      const float el = static_cast<float>(x[cl]), er = static_cast<float>(x[ch]);  // scalar loads
+      float xf[6];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xf[i + 1] = static_cast<float>(m[i]);
+      xf[0] = wave_shr1(xf[4], el);  // x[row + o - 1]
+      xf[5] = wave_shl1(xf[1], er);  // x[row + o + 4]
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          s[i] = __builtin_fmaf(v[k][i], xf[i + k], s[i]);
+          a[i] = __builtin_fmaf(__builtin_fabsf(v[k][i]), __builtin_fabsf(xf[i + k]), a[i]);
+        }
+      }
+    }
+    // back to row-per-lane order: group q's row r0 + lane at [64 q + lane]
     float* sy = s_y[wv];
-    sy[2 * lane] = sa;
-    sy[2 * lane + 1] = sb;
-    sy[2 * kWave + 2 * lane] = aa;
-    sy[2 * kWave + 2 * lane + 1] = ab;
+    *reinterpret_cast<float4*>(sy + 4 * lane) = make_float4(s[0], s[1], s[2], s[3]);
+    *reinterpret_cast<float4*>(sy + 4 * kWave + 4 * lane) = make_float4(a[0], a[1], a[2], a[3]);
     wave_sync();
-    sumA = sy[lane];
-    sumB = sy[kWave + lane];
-    absA = sy[2 * kWave + lane];
-    absB = sy[3 * kWave + lane];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      sum[q] = sy[q * kWave + lane];
+      sab[q] = sy[4 * kWave + q * kWave + lane];
+    }
     wave_sync();
   } else {
-    mixed_half<TV>(vals, offs, x, row0, bA, sA, bA < nb ? cA : 0, bw, lane, nrows, runsA, sumA, absA);
-    if (hasB) mixed_half<TV>(vals, offs, x, row0 + kWave, bB, sB, cB, bw, lane, nrows, runsB, sumB, absB);
-  }
-  // fallback sums of both groups: one pass when their entries fit one
-  float cs[2] = {0.0f, 0.0f}, cabs[2] = {0.0f, 0.0f};
-  int nterms[2] = {cA, cB};
-  if constexpr (HAS_COO) {
-    const int p0 = gptr[2 * w2], p1 = gptr[2 * w2 + 1];
-    const int p2 = hasB ? gptr[2 * w2 + 2] : p1;
-    if (p2 - p0 <= kWave) {
-      if (p0 < p2)
-        mixed_coo_pair<TV>(p0, p1, p2, row0, lane, x, cr, cc, cv, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], cs, cabs,
-                           nterms);
-    } else {
-      mixed_coo_group<TV>(p0, p1, row0, lane, x, cr, cc, cv, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], cs[0], cabs[0],
-                          nterms[0]);
-      mixed_coo_group<TV>(p1, p2, row0 + kWave, lane, x, cr, cc, cv, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], cs[1],
-                          cabs[1], nterms[1]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      sum[q] = sab[q] = 0.0f;
+      if (q < nq)
+        mixed_half<TV>(vals, offs, x, row0 + q * kWave, bq[q], sq[q], cq[q], bw, lane, nrows, rq[q], sum[q], sab[q]);
     }
   }
-  mixed_store<HAS_COO>(2 * w2, row0, lane, sumA, absA, cA, cs[0], cabs[0], nterms[0], y, nrows, ratio, u_store,
-                       flag_mask);
-  if (hasB)
-    mixed_store<HAS_COO>(2 * w2 + 1, row0 + kWave, lane, sumB, absB, cB, cs[1], cabs[1], nterms[1], y, nrows, ratio,
-                         u_store, flag_mask);
+  float cs[4] = {0.0f, 0.0f, 0.0f, 0.0f}, cabs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  int nterms[4] = {cq[0], cq[1], cq[2], cq[3]};
+  if constexpr (HAS_COO) {
+    int p[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) p[q] = gptr[4 * w4 + min(q, nq)];
+    if (p[4] - p[0] <= kWave) {
+      if (p[0] < p[4])
+        mixed_coo_quad<TV>(p, row0, lane, x, cr, cc, cv, s_v[wv], s_x[wv], s_lo[wv], s_hi[wv], cs, cabs, nterms);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        mixed_coo_group<TV>(p[q], p[q + 1], row0 + q * kWave, lane, x, cr, cc, cv, s_v[wv], s_x[wv], s_lo[wv],
+                            s_hi[wv], cs[q], cabs[q], nterms[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q < nq)
+      mixed_store<HAS_COO>(4 * w4 + q, row0 + q * kWave, lane, sum[q], sab[q], cq[q], cs[q], cabs[q], nterms[q], y,
+                           nrows, ratio, u_store, flag_mask);
+  }
 }
 
 // FP64 recomputation of one row with the FP64 kernel's arithmetic: blocked
@@ -2771,12 +2792,12 @@ int launch_mixed(int vdtype, const void* vals_lp, const double* vals64, const in
   const float u_store = vdtype == HBDIA_VALUES_F32 ? 5.9604645e-08f : 3.9062500e-03f + 5.9604645e-08f;
   const int ks = xcd_chunk_shift(grid.x);
   if (runs3) {
-    // 128-row waves, two rows per lane
-    const int64_t wgs2 = (nrows + 2 * kWG - 1) / (2 * kWG);
-    const dim3 grid2(static_cast<unsigned>(wgs2));
-    const int ks2 = xcd_chunk_shift(wgs2);
+    // 256-row waves, four rows per lane
+    const int64_t wgs4 = (nrows + 4 * kWG - 1) / (4 * kWG);
+    const dim3 grid2(static_cast<unsigned>(wgs4));
+    const int ks2 = xcd_chunk_shift(wgs4);
 #define HBDIA_LAUNCH_MX2(TV, C)                                                                                   \
-  hipLaunchKernelGGL((k_hbdia_mixed_row2<TV, C>), grid2, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, x, \
+  hipLaunchKernelGGL((k_hbdia_mixed_row4<TV, C>), grid2, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, x, \
                      y, nrows, nb, bw, cr, cc, static_cast<const TV*>(cv_lp), gptr, r, u_store, mask, runs3, ks2)
     if (vdtype == HBDIA_VALUES_F32) { if (with_coo) HBDIA_LAUNCH_MX2(float, true); else HBDIA_LAUNCH_MX2(float, false); }
     else { if (with_coo) HBDIA_LAUNCH_MX2(uint16_t, true); else HBDIA_LAUNCH_MX2(uint16_t, false); }
