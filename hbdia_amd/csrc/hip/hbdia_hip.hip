@@ -1664,6 +1664,208 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi_row2(
   }
 }
 
+// The columns of k_hbdia_multi_res for a wave whose 27 offsets (nine runs of
+// three, centres cen) are held as values v (columns outside the matrix as
+// zero values).  INSIDE: every run's window, lane 0's left and lane 63's right
+// neighbour included, lies inside x, so each centre pair is one 16-byte load;
+// otherwise each element is a clamped load.  The loads of a column are issued
+// with no branch between them, and the next column's go out before this
+// column's store, which would otherwise hold them back on the in-order memory
+// counter.
+template <bool X32, bool INSIDE>
+__device__ __forceinline__ void multi_res_columns(const dbl2a8* v, const int* cen, const double* __restrict__ vb,
+                                                  int64_t vlane, const int* __restrict__ op, int bw,
+                                                  const double* __restrict__ X, int64_t ldx,
+                                                  double* __restrict__ Y, int64_t ldy, int nrhs, int64_t row0,
+                                                  int64_t row, int64_t nrows, int lane) {
+  const int64_t last = nrows - 1;
+  const bool act_a = row < nrows, act_b = row + 1 < nrows;
+  const bool lane0 = lane == 0, lane63 = lane == kWave - 1;
+  dbl2a8 m[kChunk];       // per run: x at its centre for both rows
+  double el[kChunk], er[kChunk];  // per run: x left of lane 0's rows, right of lane 63's
+  auto load_x = [&](int j) {
+    const double* xj = X + static_cast<int64_t>(j) * ldx;
+    // an opaque copy of the row: the x addresses are formed per column rather
+    // than hoisted out of the loop and held live beside the values
+    int64_t rj = row;
+    asm volatile("" : "+v"(rj));
+#pragma unroll
+    for (int w = 0; w < kChunk; ++w) {
+      const int64_t o = cen[w];
+      if constexpr (INSIDE) {
+        m[w] = *reinterpret_cast<const dbl2a8*>(x_at<X32>(xj, rj + o));
+      } else {
+        int64_t c0 = rj + o, c1 = rj + 1 + o;
+        c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+        c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+        m[w].x = *x_at<X32>(xj, c0);
+        m[w].y = *x_at<X32>(xj, c1);
+      }
+    }
+#pragma unroll
+    for (int w = 0; w < kChunk; ++w) {
+      const int64_t o = cen[w];
+      int64_t l = row0 + o - 1, h = row0 + o + 2 * kWave;  // wave-uniform: scalar loads
+      if constexpr (!INSIDE) {
+        l = l < 0 ? 0 : (l > last ? last : l);
+        h = h < 0 ? 0 : (h > last ? last : h);
+      }
+      el[w] = *x_at<X32>(xj, l);
+      er[w] = *x_at<X32>(xj, h);
+    }
+  };
+  load_x(0);
+#pragma unroll 1
+  for (int j = 0; j < nrhs; ++j) {
+    double sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int w = 0; w < kChunk; ++w) {
+      double left = wave_shr1_zero(m[w].y), right = wave_shl1_zero(m[w].x);
+      left = lane0 ? el[w] : left;     // x[row + o - 1]
+      right = lane63 ? er[w] : right;  // x[row + o + 2]
+      sa = __builtin_fma(v[3 * w].x, left, sa);
+      sa = __builtin_fma(v[3 * w + 1].x, m[w].x, sa);
+      sa = __builtin_fma(v[3 * w + 2].x, m[w].y, sa);
+      sb = __builtin_fma(v[3 * w].y, m[w].x, sb);
+      sb = __builtin_fma(v[3 * w + 1].y, m[w].y, sb);
+      sb = __builtin_fma(v[3 * w + 2].y, right, sb);
+    }
+    // a finite non-zero sum is the exact sum bit for bit; a column with
+    // another (a zero, an infinity or NaN) in a row it stores is summed again
+    // from the same registers with every skip applied per term (a zero value,
+    // which includes every column outside the matrix)
+    const bool redo = (act_a & !__builtin_amdgcn_class(sa, kFiniteNonzero)) |
+                      (act_b & !__builtin_amdgcn_class(sb, kFiniteNonzero));
+    if (__ballot(redo) != 0) {
+      sa = sb = 0.0;
+#pragma unroll
+      for (int w = 0; w < kChunk; ++w) {
+        double left = wave_shr1_zero(m[w].y), right = wave_shl1_zero(m[w].x);
+        left = lane0 ? el[w] : left;
+        right = lane63 ? er[w] : right;
+        const double xa[3] = {left, m[w].x, m[w].y}, xb[3] = {m[w].x, m[w].y, right};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          // opaque copies: the skip masks are formed here, per column,
+          // instead of being hoisted out of the column loop and held live
+          double a = v[3 * w + k].x, b = v[3 * w + k].y;
+          asm volatile("" : "+v"(a), "+v"(b));
+          sa = masked_fma(a != 0.0, a, xa[k], sa);
+          sb = masked_fma(b != 0.0, b, xb[k], sb);
+        }
+      }
+    }
+    if (j + 1 < nrhs) load_x(j + 1);
+    double* yp = Y + static_cast<int64_t>(j) * ldy + row;
+    if (act_b) {
+      dbl2a8 t;
+      t.x = sa;
+      t.y = sb;
+      __builtin_nontemporal_store(t, reinterpret_cast<dbl2a8*>(yp));
+    } else if (act_a) {
+      __builtin_nontemporal_store(sa, yp);
+    }
+  }
+}
+
+// FP64 multi-RHS with more columns than one pass of k_hbdia_multi_row2 holds,
+// no fallback, runs-of-three hints: 128-row waves, two adjacent rows per lane,
+// every column in one wave.  A wave whose halves share one offset list of 27
+// offsets (the interior of a 27-point stencil) loads its rows' values once
+// (non-temporal, 108 VGPRs) and keeps them in registers while it walks the
+// columns (multi_res_columns), so the value stream is read once per product
+// instead of once per column pass.  Per column and run: the centre pair is one
+// 16-byte load, the outer neighbours come by zero-fill DPP shifts, and lane 0's
+// left / lane 63's right neighbour is selected from a scalar load of the value
+// beyond the wave's end.  The exactness argument is multi_row2_fast's: a
+// skipped term adds a signed zero, so a stored sum that is finite and non-zero
+// is the exact sum bit for bit, and a column whose sums are not all so is
+// summed again with every skip applied.  Other waves run the 64-row code per
+// half, eight columns at a time.
+template <bool X32>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k_hbdia_multi_res(
+    const double* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
+    const double* __restrict__ X, int64_t ldx, double* __restrict__ Y, int64_t ldy, int nrhs, int64_t nrows,
+    int nb, int bw, const uint32_t* __restrict__ runs3_blocks, int kshift) {
+  constexpr int kOff = 3 * kChunk;  // 27
+  __shared__ double s_v[kWavesPerWG][kWave];
+  __shared__ double s_x[kWavesPerWG][kWave];
+  __shared__ int s_lo[kWavesPerWG][kWave];
+  __shared__ int s_hi[kWavesPerWG][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned work = xcd_chunk_work(blockIdx.x, gridDim.x, kshift);
+  const int64_t w2 = static_cast<int64_t>(work) * kWavesPerWG + wv;  // 128-row wave
+  const int64_t row0 = w2 * 2 * kWave;
+  if (row0 >= nrows) return;
+  const int64_t bA = row0 / bw, bB = (row0 + kWave) / bw;
+  const bool hasB = row0 + kWave < nrows;
+  int sA = 0, cA = 0, sB = 0, cB = 0;
+  if (bA < nb) {
+    sA = blk[bA];
+    cA = blk[bA + 1] - sA;
+  }
+  if (hasB && bB < nb) {
+    sB = blk[bB];
+    cB = blk[bB + 1] - sB;
+  }
+  const bool runsA = bA < nb && ((runs3_blocks[bA >> 5] >> (bA & 31)) & 1u);
+  bool same = false;
+  if (runsA && cA == kOff) {
+    if (bA == bB) {
+      same = true;
+    } else if (hasB && cB == cA) {
+      const int oa = lane < cA ? offs[sA + lane] : 0;
+      const int ob = lane < cA ? offs[sB + lane] : 0;
+      same = __ballot(oa != ob) == 0;
+    }
+  }
+  if (!same) {
+    constexpr int KB = 8;
+    for (int j0 = 0; j0 < nrhs; j0 += KB) {
+      const int kb = min(KB, nrhs - j0);
+      multi_group<double, KB, false, X32, true>(vals, offs, blk, X, ldx, Y, ldy, nrows, nb, bw, nullptr, nullptr,
+                                                nullptr, nullptr, runs3_blocks, 2 * w2, j0, kb, lane, s_v[wv],
+                                                s_x[wv], s_lo[wv], s_hi[wv]);
+      if (hasB)
+        multi_group<double, KB, false, X32, true>(vals, offs, blk, X, ldx, Y, ldy, nrows, nb, bw, nullptr, nullptr,
+                                                  nullptr, nullptr, runs3_blocks, 2 * w2 + 1, j0, kb, lane, s_v[wv],
+                                                  s_x[wv], s_lo[wv], s_hi[wv]);
+    }
+    return;
+  }
+  const int64_t last = nrows - 1;
+  const int64_t row = row0 + 2 * lane;
+  const uint32_t row32 = static_cast<uint32_t>(row), n32 = static_cast<uint32_t>(nrows);
+  const double* vb = vals + static_cast<int64_t>(sA) * bw + (row0 - bA * bw);
+  // lanes of half B read block bB's lines, which follow block bA's
+  const int64_t vlane = 2 * lane + ((lane >= kWave / 2 && bB != bA) ? static_cast<int64_t>(cA - 1) * bw : 0);
+  const int* op = offs + sA;
+  // the wave's values, once; columns outside the matrix as zero values
+  dbl2a8 v[kOff];
+#pragma unroll
+  for (int k = 0; k < kOff; ++k) {
+    const dbl2a8 t = __builtin_nontemporal_load(
+        reinterpret_cast<const dbl2a8*>(vb + static_cast<int64_t>(k) * bw + vlane));
+    const uint32_t ca = row32 + static_cast<uint32_t>(op[k]);
+    v[k].x = ca < n32 ? t.x : 0.0;
+    v[k].y = ca + 1u < n32 ? t.y : 0.0;
+  }
+  // the runs' centres, and whether every run's window lies inside x
+  int cen[kChunk];
+  bool inside = true;
+#pragma unroll
+  for (int w = 0; w < kChunk; ++w) {
+    const int64_t o = op[3 * w + 1];
+    cen[w] = static_cast<int>(o);
+    inside = inside && row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last;  // wave-uniform
+  }
+  if (inside)
+    multi_res_columns<X32, true>(v, cen, vb, vlane, op, bw, X, ldx, Y, ldy, nrhs, row0, row, nrows, lane);
+  else
+    multi_res_columns<X32, false>(v, cen, vb, vlane, op, bw, X, ldx, Y, ldy, nrhs, row0, row, nrows, lane);
+}
+
 // ---------------------------------------------------------------------------
 // Many matrices in one persistent launch.  A tile is 64 consecutive rows of one
 // matrix; it is processed with exactly the arithmetic of k_hbdia<T, true, ...>
@@ -2743,6 +2945,20 @@ int launch_multi(const T* vals, const int32_t* offs, const int32_t* blk, int32_t
       // its values from L2.  The launch is bound by L2 requests (the values
       // once per pass, x nine times per column): 8 columns per pass measured
       // k = 32 at 0.617 ms against 0.668 ms with 4.
+      if (!with_coo && nrhs > 8) {
+        // more columns than one pass: every column in one wave, the values
+        // held in registers and read once
+        const int64_t wgs2 = (nrows + 2 * kWG - 1) / (2 * kWG);
+        const dim3 grid2(static_cast<unsigned>(wgs2));
+        const int ks2 = xcd_chunk_shift(wgs2);
+        if (x32)
+          hipLaunchKernelGGL((k_hbdia_multi_res<true>), grid2, dim3(kWG), 0, s, vals, offs, blk, X, ldx, Y, ldy, nrhs,
+                             nrows, nb, bw, runs3, ks2);
+        else
+          hipLaunchKernelGGL((k_hbdia_multi_res<false>), grid2, dim3(kWG), 0, s, vals, offs, blk, X, ldx, Y, ldy,
+                             nrhs, nrows, nb, bw, runs3, ks2);
+        return check_hip(hipGetLastError(), name);
+      }
       const int kb2 = kb;
       const int64_t items2 = (nrows + 2 * kWG - 1) / (2 * kWG) * ((nrhs + kb2 - 1) / kb2);
       const dim3 grid2(static_cast<unsigned>(items2));

@@ -38,11 +38,12 @@ def single(A: HBDIA, x: np.ndarray, y0: np.ndarray) -> np.ndarray:
     return Y.download()
 
 
-@pytest.mark.parametrize("k", [1, 3, 4, 8, 16, 20, 32])
-def test_multi_rhs_columns_equal_single_vector(gpu, k):
-    print(f"case: multi-RHS k={k}", flush=True)
+@pytest.mark.parametrize("k,scatter", [(1, 0.01), (3, 0.01), (4, 0.01), (8, 0.01), (16, 0.01), (20, 0.01), (32, 0.01),
+                                       (9, 0.0), (16, 0.0), (32, 0.0)])
+def test_multi_rhs_columns_equal_single_vector(gpu, k, scatter):
+    print(f"case: multi-RHS k={k} scatter={scatter}", flush=True)
     n = 20
-    A = HBDIA.stencil(n, 27, 0.01, seed=21)
+    A = HBDIA.stencil(n, 27, scatter, seed=21)
     A.convertToHBDIAFormat(64, 2)
     N = n ** 3
     ld = N + 37  # a leading dimension larger than N
@@ -61,7 +62,8 @@ def test_multi_rhs_columns_equal_single_vector(gpu, k):
         assert np.all(Yh[j, N:] == 0.25)  # padding rows untouched
 
 
-def test_multi_rhs_without_fallback_keeps_empty_block_rows(gpu):
+@pytest.mark.parametrize("k", [4, 12])
+def test_multi_rhs_without_fallback_keeps_empty_block_rows(gpu, k):
     # rows past the last block keep Y (blocked part only, as hbdia_bdia_spmv)
     rows = np.array([0, 0, 1, 63, 64, 64], np.int32)
     cols = np.array([0, 1, 1, 63, 64, 63], np.int32)
@@ -69,7 +71,7 @@ def test_multi_rhs_without_fallback_keeps_empty_block_rows(gpu):
     A = HBDIA(rows, cols, vals, 200, 100)  # 200 rows, blocks cover only min(200,100) rows
     A.convertToHBDIAFormat(64, 1)
     assert A.info()["num_coo"] == 0
-    k, N = 4, 200
+    N = 200
     Xh = np.vstack([np.arange(N, dtype=np.float64) + j for j in range(k)])
     Y0 = np.full((k, N), -3.0)
     X, Y = DeviceArray.from_host(Xh.ravel()), DeviceArray.from_host(Y0.ravel())

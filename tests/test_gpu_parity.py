@@ -501,7 +501,9 @@ def test_dense_fallback_groups_are_bit_identical(gpu, N, bw, dtype):
 
 @pytest.mark.parametrize("N,bw,with_coo,k,dtype", [(1000, 64, True, 3, np.float64), (1000, 64, False, 8, np.float64),
                                                    (4099, 128, True, 5, np.float64), (777, 256, True, 16, np.float64),
-                                                   (1000, 64, True, 8, np.float32), (777, 256, False, 5, np.float32)])
+                                                   (1000, 64, True, 8, np.float32), (777, 256, False, 5, np.float32),
+                                                   (1000, 64, False, 12, np.float64), (4099, 128, False, 17, np.float64),
+                                                   (777, 256, False, 24, np.float64)])
 def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k, dtype):
     """The multi-RHS kernel's runs-of-three path: every column equals the hint-free multi-RHS result,
     the single-vector kernel and the oracle."""
@@ -533,7 +535,7 @@ def test_runs3_hint_multi_rhs_is_bit_identical(gpu, N, bw, with_coo, k, dtype):
                                       err_msg=f"column {j}")
 
 
-@pytest.mark.parametrize("k", [5, 8])
+@pytest.mark.parametrize("k", [5, 8, 12, 32])
 def test_multi_rhs_zero_and_nonfinite_sums_bit_exact(gpu, k):
     """The FP64 multi-RHS kernel sums without per-term skips and redoes a wave with them when a stored
     sum is zero, infinite or NaN.  Columns whose sums underflow to -0, are +0, pass through an
@@ -553,7 +555,7 @@ def test_multi_rhs_zero_and_nonfinite_sums_bit_exact(gpu, k):
     cols[4][N // 3] = np.nan
     cols[5][[7, N // 2]] = [np.inf, -np.inf]
     cols[7][: N // 2] = -1e-200
-    Xh = np.vstack(cols[:k])
+    Xh = np.vstack([cols[j % len(cols)] for j in range(k)])
     Y0 = np.full((k, N), 0.5)
     X = DeviceArray.from_host(Xh.ravel())
     outs = []
