@@ -84,8 +84,8 @@ _HIP_SIGS = {
     "hbdia_mixed_workspace_bytes": (c_size_t, [c_int64]),
     "hbdia_batch_make_tiles": (_i, [_p, c_int32, _p, _p, _p, c_int32, _p, c_int64]),
     "hbdia_batch_spmv_f64": (_i, [_p, _p, _p, _p, _p]),
-    "hbdia_serve_spmv_f64": (_i, [_p, _p, c_int32, c_int64, c_int32, c_int32, _p, _p, _p]),
-    "hbdia_serve_spmv_f32": (_i, [_p, _p, c_int32, c_int64, c_int32, c_int32, _p, _p, _p]),
+    "hbdia_serve_spmv_f64": (_i, [_p, _p, c_int32, c_int64, c_int32, c_int32, _p]),
+    "hbdia_serve_spmv_f32": (_i, [_p, _p, c_int32, c_int64, c_int32, c_int32, _p]),
     "hbdia_batch_spmv_f32": (_i, [_p, _p, _p, _p, _p]),
     "hbdia_batch_occupancy": (_i, [_p, c_int32, c_int32, POINTER(c_int)]),
     "hbdia_hbdia_spmm_rm_f64": (_i, [_p, _p, _p, c_int32, c_int32, _p, _p, _p, _p, _p, _p, c_int32, c_int64, _p]),

@@ -1956,8 +1956,11 @@ static_assert(sizeof(hbdia_batch_tile_t) == kTileDwords * 4, "tile record layout
 static_assert(HBDIA_BATCH_INLINE_OFFSETS >= kChunk, "a tile record carries the first chunk's offsets");
 
 // JOBS: the work is a list of jobs (serving), each the tiles of one pooled
-// matrix applied to its own x / y slot; the wave's work indices are mapped to
-// (job, tile) by a binary search over the jobs' work_begin.
+// matrix applied to the job's own x and y (device addresses carried by the job,
+// element 0 at the matrix's row 0); the wave's work indices are mapped to (job,
+// tile) by a binary search over the jobs' work_begin.  A job's y is taken to
+// start at zero: every row is written, rows without blocks as 0 (+ fallback),
+// so the caller clears nothing.
 template <typename T, bool HAS_COO, bool X32, bool CONTIG, bool JOBS>
 __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__ tiles,
                                                      const int32_t* __restrict__ wave_tiles,
@@ -1996,7 +1999,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
     else tj = wave * tiles_per_step + (sj / tiles_per_step) * step + sj % tiles_per_step;
     const bool vj = tj < limit;
     int64_t rec = vj ? tj : 0;
-    int shift = 0;
+    uint64_t jx = 0, jy = 0;  // JOBS: the job's x and y addresses
     if constexpr (JOBS) {
       // the last job whose work starts at or before tj
       int lo = 0, hi = num_jobs - 1;
@@ -2006,13 +2009,13 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
         else hi = mid - 1;
       }
       rec = vj ? jobs[lo].tile_begin + (tj - jobs[lo].work_begin) : 0;
-      shift = vj ? jobs[lo].row_shift : 0;
+      jx = vj ? jobs[lo].x : 0;
+      jy = vj ? jobs[lo].y : 0;
     }
     const int32_t* rp = tiles + rec * kTileDwords + ql * 5;
     int rv[5];
 #pragma unroll
     for (int r = 0; r < 5; ++r) rv[r] = rp[r];
-    if constexpr (JOBS) rv[1] += ql == 0 ? shift : 0;  // dword 1 (rows) is held by lane 4j
     const int ng = __popcll(__ballot(vj && ql == 0));
     if (ng == 0) break;
     // fields of tile jj of the group, read from the lanes holding its record;
@@ -2021,6 +2024,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
     struct Fields {
       int value_line, rows, row0, num_rows, bw, cnt, offs_at, p0, p1, fused;
       int off[kChunk];
+      uint64_t xb, yb;  // JOBS: the tile's job's x and y
     };
     auto read_fields = [&](int jj) {
       Fields f;
@@ -2045,6 +2049,16 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
       f.off[7] = TILE_FIELD(17);
       f.off[8] = TILE_FIELD(18);
 #undef TILE_FIELD
+      f.xb = f.yb = 0;
+      if constexpr (JOBS) {
+        auto lane64 = [&](uint64_t v) {
+          const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), 4 * jj);
+          const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), 4 * jj);
+          return (static_cast<uint64_t>(hi) << 32) | lo;
+        };
+        f.xb = lane64(jx);
+        f.yb = lane64(jy);
+      }
       return f;
     };
     Fields nextf = read_fields(0);
@@ -2064,9 +2078,11 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
         const Fields cg = nextf;
         ++jj;
         if (jj + 1 < ng) nextf = read_fields(jj + 1);
-        tile_pair<T, HAS_COO, X32>(cf.value_line, cg.value_line, cf.rows, cf.row0, cf.num_rows, cf.bw, cf.cnt,
-                                   cf.off, cf.p0, cf.p1, cg.p1, cf.fused, vals, cr, cc, cv, xp, yp, lane, sv, sx,
-                                   slo, shi);
+        // a job's x and y start at its matrix's row 0; the batch's at the pool's
+        const T* xq = JOBS ? reinterpret_cast<const T*>(cf.xb) : xp + cf.rows;
+        T* yq = JOBS ? reinterpret_cast<T*>(cf.yb) : yp + cf.rows;
+        tile_pair<T, HAS_COO, X32>(cf.value_line, cg.value_line, 0, cf.row0, cf.num_rows, cf.bw, cf.cnt, cf.off,
+                                   cf.p0, cf.p1, cg.p1, cf.fused, vals, cr, cc, cv, xq, yq, lane, sv, sx, slo, shi);
         continue;
       }
       const int value_line = cf.value_line, rows = cf.rows, row0 = cf.row0, num_rows = cf.num_rows, bw = cf.bw,
@@ -2075,8 +2091,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
       const int64_t nrows = num_rows;
       const int64_t row = static_cast<int64_t>(row0) + lane;  // row within the matrix
       const bool active = row < nrows;
-      const T* x = xp + rows;
-      T* y = yp + rows;
+      const T* x = JOBS ? reinterpret_cast<const T*>(cf.xb) : xp + rows;
+      T* y = JOBS ? reinterpret_cast<T*>(cf.yb) : yp + rows;
       // A matrix's rows are below 2^31: row + offset modulo 2^32 is below n32
       // exactly when the column lies in [0, nrows), and an out-of-range column
       // lies below 0 for a negative offset and past the last row otherwise.
@@ -2168,11 +2184,12 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
           }
         }
         if (active) {
-          const T base = cnt > 0 ? sum : y[row];
+          // a job's y starts at zero: rows without blocks are written too
+          const T base = cnt > 0 ? sum : (JOBS ? T(0) : y[row]);
           __builtin_nontemporal_store(base + cs, y + row);
         }
       } else {
-        if (active && cnt > 0) __builtin_nontemporal_store(sum, y + row);
+        if (active && (cnt > 0 || JOBS)) __builtin_nontemporal_store(cnt > 0 ? sum : T(0), y + row);
       }
     }
     if (ng < kTileGroup) break;
@@ -3078,22 +3095,23 @@ int launch_batch(const hbdia_batch_t* bt, const hbdia_batch_schedule_t* sc, cons
 
 template <typename T>
 int launch_serve(const hbdia_batch_t* bt, const hbdia_serve_job_t* jobs, int32_t num_jobs, int64_t total_tiles,
-                 int32_t num_workgroups, int32_t tiles_per_step, const T* x, T* y, hbdia_stream_t stream,
-                 const char* name) {
+                 int32_t num_workgroups, int32_t tiles_per_step, hbdia_stream_t stream, const char* name) {
   if (!bt || num_jobs < 0 || total_tiles < 0 || num_workgroups < 1 || tiles_per_step < 1)
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": invalid argument");
   if (num_jobs == 0 || total_tiles == 0) return HBDIA_SUCCESS;
-  if (!jobs || !x || !y || !bt->tiles || !bt->values || !bt->offsets)
+  if (!jobs || !bt->tiles || !bt->values || !bt->offsets)
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
   if (bt->has_coo && (!bt->coo_rows || !bt->coo_cols || !bt->coo_values))
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null fallback pool");
   const dim3 grid(static_cast<unsigned>(num_workgroups)), block(kWG);
+  const T* no_x = nullptr;  // each job carries its own x and y
+  T* no_y = nullptr;
   hipStream_t s = hs(stream);
 #define HBDIA_SERVE_LAUNCH(C, X) \
   hipLaunchKernelGGL((k_hbdia_batch<T, C, X, false, true>), grid, block, 0, s,                                \
                      reinterpret_cast<const int32_t*>(bt->tiles), nullptr, static_cast<const T*>(bt->values), \
                      bt->offsets, bt->coo_rows, bt->coo_cols, static_cast<const T*>(bt->coo_values),          \
-                     total_tiles, tiles_per_step, jobs, num_jobs, x, y)
+                     total_tiles, tiles_per_step, jobs, num_jobs, no_x, no_y)
   if (bt->has_coo) {
     if (bt->x32) HBDIA_SERVE_LAUNCH(true, true); else HBDIA_SERVE_LAUNCH(true, false);
   } else {
@@ -3481,16 +3499,14 @@ int hbdia_batch_spmv_f32(const hbdia_batch_t* batch, const hbdia_batch_schedule_
 }
 
 int hbdia_serve_spmv_f64(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t num_jobs,
-                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, const double* x,
-                         double* y, hbdia_stream_t stream) {
-  return launch_serve<double>(pool, jobs, num_jobs, total_tiles, num_workgroups, tiles_per_step, x, y, stream,
+                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, hbdia_stream_t stream) {
+  return launch_serve<double>(pool, jobs, num_jobs, total_tiles, num_workgroups, tiles_per_step, stream,
                               "hbdia_serve_spmv_f64");
 }
 
 int hbdia_serve_spmv_f32(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t num_jobs,
-                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, const float* x,
-                         float* y, hbdia_stream_t stream) {
-  return launch_serve<float>(pool, jobs, num_jobs, total_tiles, num_workgroups, tiles_per_step, x, y, stream,
+                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, hbdia_stream_t stream) {
+  return launch_serve<float>(pool, jobs, num_jobs, total_tiles, num_workgroups, tiles_per_step, stream,
                              "hbdia_serve_spmv_f32");
 }
 

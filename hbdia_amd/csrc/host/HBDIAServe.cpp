@@ -78,17 +78,19 @@ int64_t nowNs() {
 
 template <typename T>
 int serveLaunch(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t n, int64_t tiles, int32_t wg,
-                int32_t tps, const T* x, T* y, hbdia_stream_t s);
+                int32_t tps, hbdia_stream_t s);
 template <>
 int serveLaunch<double>(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t n, int64_t tiles,
-                        int32_t wg, int32_t tps, const double* x, double* y, hbdia_stream_t s) {
-  return hbdia_serve_spmv_f64(pool, jobs, n, tiles, wg, tps, x, y, s);
+                        int32_t wg, int32_t tps, hbdia_stream_t s) {
+  return hbdia_serve_spmv_f64(pool, jobs, n, tiles, wg, tps, s);
 }
 template <>
 int serveLaunch<float>(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t n, int64_t tiles, int32_t wg,
-                       int32_t tps, const float* x, float* y, hbdia_stream_t s) {
-  return hbdia_serve_spmv_f32(pool, jobs, n, tiles, wg, tps, x, y, s);
+                       int32_t tps, hbdia_stream_t s) {
+  return hbdia_serve_spmv_f32(pool, jobs, n, tiles, wg, tps, s);
 }
+
+uint64_t devAddr(const void* p) { return static_cast<uint64_t>(reinterpret_cast<uintptr_t>(p)); }
 
 constexpr int kQueued = 0, kAdmitted = 1, kDone = 2, kFailed = 3;
 
@@ -171,7 +173,7 @@ int64_t HBDIAServer<T>::submit(int matrix, const T* x, T* y, bool deviceVectors)
   if (matrix < 0 || matrix >= numMatrices()) throw std::out_of_range("HBDIAServer::submit: no such matrix");
   if (!x || !y) throw std::invalid_argument("HBDIAServer::submit: null vector");
   const int64_t n = pool_->numRows(matrix);
-  if ((n + 63) / 64 * 64 > arenaElems_)
+  if ((!deviceVectors || !cfg_.batching) && (n + 63) / 64 * 64 > arenaElems_)
     throw std::invalid_argument("HBDIAServer::submit: the matrix's vectors exceed the request arena");
   std::lock_guard<std::mutex> g(mu_);
   if (error_) std::rethrow_exception(error_);
@@ -251,7 +253,11 @@ void HBDIAServer<T>::fail(std::exception_ptr e) {
 
 template <typename T>
 void HBDIAServer<T>::issue(std::vector<int64_t>& ids, int ring) {
-  // job table and the copies in (x into the arena, y cleared)
+  // the job table and the copies in.  Batched, a request whose vectors are on
+  // the device is read and written in place (no arena slot); a host request's
+  // x goes into its arena slot.  The serve kernel writes every row of y, so
+  // nothing is cleared.  The plain form copies every request through the
+  // arena and clears y for the single-matrix kernel.
   hbdia_serve_job_t* hj = hostJobs_[ring];
   int64_t work = 0;
   std::vector<Request> rs;
@@ -260,33 +266,35 @@ void HBDIAServer<T>::issue(std::vector<int64_t>& ids, int ring) {
     for (int64_t id : ids) rs.push_back(requests_[id]);
   }
   for (size_t k = 0; k < rs.size(); ++k) {
-    const hbdia_batch_desc_t& d = pool_->desc(rs[k].matrix);
+    const Request& r = rs[k];
     hj[k].work_begin = work;
-    hj[k].tile_begin = d.first_tile;
-    hj[k].row_shift = static_cast<int32_t>(rs[k].slot - d.rows);
-    hj[k].pad_ = 0;
-    work += pool_->tilesOf(rs[k].matrix);
+    hj[k].tile_begin = pool_->desc(r.matrix).first_tile;
+    hj[k].x = r.slot >= 0 ? devAddr(xArena_ + r.slot) : devAddr(r.x);
+    hj[k].y = r.slot >= 0 ? devAddr(yArena_ + r.slot) : devAddr(r.y);
+    work += pool_->tilesOf(r.matrix);
   }
   const size_t T_ = sizeof(T);
   if (cfg_.batching)
     CHECK_HBDIA(hbdia_memcpy_h2d(devJobs_[ring], hj, rs.size() * sizeof(hbdia_serve_job_t), sIn_));
   for (const Request& r : rs) {
+    if (r.slot < 0) continue;
     const size_t bytes = static_cast<size_t>(pool_->numRows(r.matrix)) * T_;
     if (r.device) CHECK_HBDIA(hbdia_memcpy_d2d(xArena_ + r.slot, r.x, bytes, sIn_));
     else CHECK_HBDIA(hbdia_memcpy_h2d(xArena_ + r.slot, r.x, bytes, sIn_));
-    CHECK_HBDIA(hbdia_memset(yArena_ + r.slot, 0, bytes, sIn_));
+    if (!cfg_.batching) CHECK_HBDIA(hbdia_memset(yArena_ + r.slot, 0, bytes, sIn_));
   }
   CHECK_HBDIA(hbdia_event_record(evIn_[ring], sIn_));
   CHECK_HBDIA(hbdia_stream_wait_event(sComp_, evIn_[ring]));
   if (cfg_.batching) {
     CHECK_HBDIA(serveLaunch<T>(&pool_->deviceBatch(), devJobs_[ring], static_cast<int32_t>(rs.size()), work,
-                               workgroups_, cfg_.tilesPerStep, xArena_, yArena_, sComp_));
+                               workgroups_, cfg_.tilesPerStep, sComp_));
   } else {
     for (const Request& r : rs) pool_->spmvMatrix(r.matrix, xArena_ + r.slot, yArena_ + r.slot, sComp_);
   }
   CHECK_HBDIA(hbdia_event_record(evComp_[ring], sComp_));
   CHECK_HBDIA(hbdia_stream_wait_event(sOut_, evComp_[ring]));
   for (const Request& r : rs) {
+    if (r.slot < 0) continue;
     const size_t bytes = static_cast<size_t>(pool_->numRows(r.matrix)) * T_;
     if (r.device) CHECK_HBDIA(hbdia_memcpy_d2d(r.y, yArena_ + r.slot, bytes, sOut_));
     else CHECK_HBDIA(hbdia_memcpy_d2h(r.y, yArena_ + r.slot, bytes, sOut_));
@@ -312,7 +320,7 @@ void HBDIAServer<T>::loop() {
             Request& r = requests_[id];
             r.state = kDone;
             r.t.completeNs = t;
-            arena_.release(r.slot);
+            if (r.slot >= 0) arena_.release(r.slot);
             ++completed_;
           }
         }
@@ -326,8 +334,11 @@ void HBDIAServer<T>::loop() {
         std::lock_guard<std::mutex> g(mu_);
         while (!pending_.empty() && static_cast<int>(next.size()) < cfg_.maxJobsPerLaunch) {
           Request& r = requests_[pending_.front()];
-          const int64_t slot = arena_.allocate(pool_->numRows(r.matrix));
-          if (slot < 0) break;  // strict order: later requests wait behind this one
+          int64_t slot = -1;  // batched device requests need no arena slot
+          if (!cfg_.batching || !r.device) {
+            slot = arena_.allocate(pool_->numRows(r.matrix));
+            if (slot < 0) break;  // strict order: later requests wait behind this one
+          }
           r.slot = slot;
           r.state = kAdmitted;
           r.t.admitNs = nowNs();

@@ -6,9 +6,12 @@
 // loop.  Here a pool of converted matrices is uploaded once (an HBDIABatch
 // pool); clients submit requests (matrix, x, y) from any thread.  A worker
 // thread admits queued requests in arrival order while a launch may take more
-// jobs and the request arena (device memory for the requests' x and y, the
-// part of the HBM budget not used by the resident matrices) has room, and
-// issues them as one persistent launch of the batch kernel over a job table.
+// jobs and, for requests whose vectors are in host memory, the request arena
+// (device memory for their x and y, the part of the HBM budget not used by the
+// resident matrices) has room, and issues them as one persistent launch of the
+// batch kernel over a job table.  A request whose vectors are already on the
+// device is read and written in place: each job carries its own x and y, and
+// the kernel writes every row of y, so nothing is copied or cleared for it.
 // Requests that arrive while launches are on the GPU are admitted into the
 // next launch, which is queued behind them; copies in (x), the kernel and
 // copies out (y) run on three streams, so one launch's copies overlap
@@ -55,9 +58,9 @@ class VectorArena {
 
 // Elements of each of the server's x and y arenas for an HBM budget and the
 // resident matrices' bytes: half of what remains per vector, a multiple of 64,
-// and at most kMaxServeArenaElems, because the serve kernel addresses a
-// request's rows as int32 (pool rows + slot shift).  Throws when the budget
-// leaves no room.
+// and at most kMaxServeArenaElems (one allocation per vector of at most 16 GiB
+// in FP64; a request's vectors are addressed through 64-bit job addresses, its
+// rows as int32 within its matrix).  Throws when the budget leaves no room.
 constexpr int64_t kMaxServeArenaElems = (int64_t{1} << 31) - 64;
 int64_t serveArenaElems(int64_t budgetBytes, int64_t residentBytes, int64_t elemBytes);
 
@@ -86,7 +89,9 @@ class HBDIAServer {
 
   // y = A_matrix x.  x, y: host memory (pinned for overlapped copies) or, with
   // deviceVectors, device memory; both stay valid until the request
-  // completes.  Throws if the matrix's vectors can never fit the arena.
+  // completes.  Device vectors are used in place by the batched server, so
+  // requests in flight together must not share a y.  Throws if the matrix's
+  // vectors can never fit the arena (host vectors, or the plain form).
   int64_t submit(int matrix, const T* x, T* y, bool deviceVectors = false);
   void wait(int64_t id);  // rethrows a failure of the worker
   bool done(int64_t id) const;
@@ -148,7 +153,8 @@ class HBDIAServer {
 // Open-loop load: `requests` arrivals with exponential gaps of mean
 // 1 / rate seconds (rate <= 0: all at once), each on a matrix drawn uniformly
 // from the pool and one of its x variants (xs[m * variants + v]); results go
-// to ys[m] (shared by the requests of a matrix).  Latency of a request =
+// to ys[m], shared by the requests of a matrix: a load generator for timing,
+// whose outputs overlapping requests overwrite.  Latency of a request =
 // completion - submission.  Returns per-request latencies (us) and the wall
 // time from the first submission to the last completion.
 struct LoadReport {

@@ -340,27 +340,25 @@ int hbdia_batch_spmv_f32(const hbdia_batch_t* batch, const hbdia_batch_schedule_
                          float* y, hbdia_stream_t stream);
 /* Serving: one persistent launch over a list of jobs, each job one matrix of a
  * resident pool (tiles [tile_begin, tile_begin + its tile count) of the pool's
- * tile records) applied to its own slot of the x and y arenas.  The launch's
- * work is the concatenation of the jobs' tiles: job k owns work indices
- * [work_begin_k, work_begin_{k+1}) (work_begin ascending, starting at 0;
- * total_tiles = the sum of all jobs' tiles).  A tile of job k reads x and
- * writes y at (record rows + row_shift_k) + row, so a request whose slot
- * starts at S for a matrix whose records carry rows R uses row_shift = S - R.
- * Interleaved schedule (runs of tiles_per_step tiles, num_workgroups
- * workgroups); every job's rows come out bit for bit as in the batch and
- * single-matrix kernels. */
+ * tile records) applied to its own x and y: device addresses of the job's
+ * vectors, element 0 at the matrix's row 0 (a client's own device vectors, or
+ * slots of a staging arena).  The launch's work is the concatenation of the
+ * jobs' tiles: job k owns work indices [work_begin_k, work_begin_{k+1})
+ * (work_begin ascending, starting at 0; total_tiles = the sum of all jobs'
+ * tiles).  y is written in full, as hbdiaSpMV on the matrix with y starting at
+ * zero: nothing needs clearing.  Interleaved schedule (runs of tiles_per_step
+ * tiles, num_workgroups workgroups); every job's rows come out bit for bit as
+ * in the batch and single-matrix kernels. */
 typedef struct hbdia_serve_job {
   int64_t work_begin;
   int64_t tile_begin;
-  int32_t row_shift;
-  int32_t pad_;
+  uint64_t x; /* const T* */
+  uint64_t y; /* T* */
 } hbdia_serve_job_t;
 int hbdia_serve_spmv_f64(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t num_jobs,
-                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, const double* x,
-                         double* y, hbdia_stream_t stream);
+                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, hbdia_stream_t stream);
 int hbdia_serve_spmv_f32(const hbdia_batch_t* pool, const hbdia_serve_job_t* jobs, int32_t num_jobs,
-                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, const float* x,
-                         float* y, hbdia_stream_t stream);
+                         int64_t total_tiles, int32_t num_workgroups, int32_t tiles_per_step, hbdia_stream_t stream);
 
 /* Workgroups of the batch kernel that one compute unit holds at once (the
  * kernel variant that `batch`, `kind` and value_size (4 or 8) select); a grid

@@ -210,3 +210,53 @@ def test_submit_rejects_short_vectors(gpu):
     with pytest.raises(TypeError):
         S.submit(2, np.zeros(n, np.float32), np.zeros(n))
     S.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_device_vectors_in_place_in_the_mix(gpu, dtype):
+    """Batched requests whose vectors are on the device are read and written in place, with y starting
+    as NaN: the serve kernel writes every row (rows past a matrix's blocks as zero plus their fallback),
+    so nothing is cleared.  Every request equals the oracle with y starting at zero, alone and in a
+    burst from four threads, each request with its own y."""
+    print(f"case: serve device vectors in place {np.dtype(dtype).name}", flush=True)
+    ms = pool(dtype)
+    # 300 x 300 with entries in the first 100 rows only: the blocks of rows 128.. hold no offsets,
+    # and R's lone entry in row 250 is a fallback entry in such a block
+    rows = np.array([0, 0, 1, 63, 64, 64, 99, 99, 250], np.int32)
+    cols = np.array([0, 1, 1, 63, 64, 63, 99, 98, 7], np.int32)
+    vals = np.array([2.0, 1.0, 3.0, 4.0, 5.0, 6.0, 1.5, -2.0, 0.5], dtype)
+    R = HBDIA(rows, cols, vals, 300, 300, dtype=dtype)
+    R.convertToHBDIAFormatOnHost(64, 2)
+    assert R.info()["num_coo"] > 0
+    ms.append(R)
+    R2 = HBDIA(rows[:-1], cols[:-1], vals[:-1], 300, 300, dtype=dtype)
+    R2.convertToHBDIAFormatOnHost(64, 1)
+    assert R2.info()["num_coo"] == 0
+    ms.append(R2)
+    xs = xs_for(ms, dtype, variants=2)
+    want = {(m, v): expected(ms[m], xs[m][v]) for m in range(len(ms)) for v in range(2)}
+    S = HBDIAServer(ms, max_jobs=48, max_in_flight=2)
+    dx = {(m, v): DeviceArray.from_host(xs[m][v]) for m in range(len(ms)) for v in range(2)}
+    for (m, v), w in want.items():
+        Y = DeviceArray.from_host(np.full(ms[m].getNumRows(), np.nan, dtype))
+        S.wait(S.submit(m, dx[m, v], Y, device=True))
+        assert Y.download().tobytes() == w.tobytes(), f"matrix {m} alone differs from the oracle"
+    rng = np.random.default_rng(31)
+    picks = [(int(rng.integers(len(ms))), int(rng.integers(2))) for _ in range(160)]
+    outs = [DeviceArray.from_host(np.full(ms[m].getNumRows(), np.nan, dtype)) for m, _ in picks]
+
+    def submitter(k):
+        for i in range(k, len(picks), 4):
+            m, v = picks[i]
+            S.submit(m, dx[m, v], outs[i], device=True)
+
+    th = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    S.drain()
+    for i, (m, v) in enumerate(picks):
+        assert outs[i].download().tobytes() == want[m, v].tobytes(), f"request {i} (matrix {m}) differs in the mix"
+    assert S.info()["largest_launch"] > 1
+    S.close()
