@@ -568,7 +568,7 @@ def bench_batch(args) -> dict:
              "per_matrix_launches_ms": round(plain, 5), "speedup_vs_per_matrix": round(plain / kern, 3),
              "rows": N, "nnz_bdia": info["bdia_nnz"], "nnz_coo": info["coo_nnz"], "algorithmic_bytes": abytes,
              "stored_bytes": info["stored_bytes"], "block_width": best["block_width"],
-             "schedule": {"kind": ["interleaved", "contiguous"][best["kind"]],
+             "schedule": {"kind": ["interleaved", "contiguous", "xcd"][best["kind"]],
                           "tiles_per_step": best["tiles_per_step"], "workgroups": best["workgroups"]},
              "candidates": len(B.table), "worst_candidate_ms": round(worst["ms"], 5), "parity": "bit-exact"}
         if name == "fp64":

@@ -1976,12 +1976,20 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
   __shared__ int s_hi[kWavesPerWG][2 * kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t wave = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
-  const int64_t step = static_cast<int64_t>(gridDim.x) * kWavesPerWG * tiles_per_step;
+  int64_t wave = static_cast<int64_t>(blockIdx.x) * kWavesPerWG + wv;
+  int64_t step = static_cast<int64_t>(gridDim.x) * kWavesPerWG * tiles_per_step;
   int64_t first = 0, limit = num_tiles;
   if constexpr (CONTIG) {
     first = wave_tiles[wave];
     limit = wave_tiles[wave + 1];
+  } else if (!JOBS && wave_tiles != nullptr) {
+    // XCD split: the workgroups of XCD k (blockIdx % 8 == k) interleave over
+    // the tiles [wave_tiles[k], wave_tiles[k + 1])
+    const unsigned xcd = blockIdx.x & 7u;
+    wave = static_cast<int64_t>(blockIdx.x >> 3) * kWavesPerWG + wv;
+    step = static_cast<int64_t>(gridDim.x >> 3) * kWavesPerWG * tiles_per_step;
+    first = wave_tiles[xcd];
+    limit = wave_tiles[xcd + 1];
   }
   T* sv = s_v[wv];
   T* sx = s_x[wv];
@@ -1996,7 +2004,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
     const int64_t sj = s0 + jl;
     int64_t tj;
     if constexpr (CONTIG) tj = first + sj;
-    else tj = wave * tiles_per_step + (sj / tiles_per_step) * step + sj % tiles_per_step;
+    else tj = first + wave * tiles_per_step + (sj / tiles_per_step) * step + sj % tiles_per_step;
     const bool vj = tj < limit;
     int64_t rec = vj ? tj : 0;
     uint64_t jx = 0, jy = 0;  // JOBS: the job's x and y addresses
@@ -3059,21 +3067,24 @@ int launch_batch(const hbdia_batch_t* bt, const hbdia_batch_schedule_t* sc, cons
   if (!bt || !sc) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null batch or schedule");
   if (bt->num_tiles < 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": negative sizes");
   if (sc->num_workgroups < 1 || sc->tiles_per_step < 1 ||
-      (sc->kind != HBDIA_BATCH_INTERLEAVED && sc->kind != HBDIA_BATCH_CONTIGUOUS))
+      (sc->kind != HBDIA_BATCH_INTERLEAVED && sc->kind != HBDIA_BATCH_CONTIGUOUS && sc->kind != HBDIA_BATCH_XCD) ||
+      (sc->kind == HBDIA_BATCH_XCD && sc->num_workgroups % 8 != 0))
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": invalid schedule");
   if (bt->num_tiles == 0) return HBDIA_SUCCESS;
   if (!x || !y || !bt->tiles || !bt->values || !bt->offsets)
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null pointer");
   if (bt->has_coo && (!bt->coo_rows || !bt->coo_cols || !bt->coo_values))
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": null fallback pool");
-  if (sc->kind == HBDIA_BATCH_CONTIGUOUS && !bt->wave_tiles)
-    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": contiguous schedule without wave_tiles");
+  if (sc->kind != HBDIA_BATCH_INTERLEAVED && !bt->wave_tiles)
+    return fail(HBDIA_ERROR_INVALID_ARGUMENT, std::string(name) + ": schedule without wave_tiles");
   const dim3 grid(static_cast<unsigned>(sc->num_workgroups)), block(kWG);
   hipStream_t s = hs(stream);
   const bool coo = bt->has_coo != 0, x32 = bt->x32 != 0, contig = sc->kind == HBDIA_BATCH_CONTIGUOUS;
+  // interleaved: no split array (the kernel reads wave_tiles as the XCD split when given one)
+  const int32_t* split = sc->kind == HBDIA_BATCH_INTERLEAVED ? nullptr : bt->wave_tiles;
 #define HBDIA_BATCH_LAUNCH(C, X, G) \
   hipLaunchKernelGGL((k_hbdia_batch<T, C, X, G, false>), grid, block, 0, s,                                   \
-                     reinterpret_cast<const int32_t*>(bt->tiles), bt->wave_tiles,                           \
+                     reinterpret_cast<const int32_t*>(bt->tiles), split,                                    \
                      static_cast<const T*>(bt->values), bt->offsets, bt->coo_rows, bt->coo_cols,            \
                      static_cast<const T*>(bt->coo_values), bt->num_tiles, sc->tiles_per_step, nullptr, 0, x, y)
   if (coo) {
@@ -3512,7 +3523,7 @@ int hbdia_serve_spmv_f32(const hbdia_batch_t* pool, const hbdia_serve_job_t* job
 
 int hbdia_batch_occupancy(const hbdia_batch_t* batch, int32_t kind, int32_t value_size, int* workgroups_per_cu) {
   if (!batch || !workgroups_per_cu || (value_size != 4 && value_size != 8) ||
-      (kind != HBDIA_BATCH_INTERLEAVED && kind != HBDIA_BATCH_CONTIGUOUS))
+      (kind != HBDIA_BATCH_INTERLEAVED && kind != HBDIA_BATCH_CONTIGUOUS && kind != HBDIA_BATCH_XCD))
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_batch_occupancy: invalid argument");
   const bool coo = batch->has_coo != 0, x32 = batch->x32 != 0, contig = kind == HBDIA_BATCH_CONTIGUOUS;
   return value_size == 8 ? batch_occupancy<double>(coo, x32, contig, workgroups_per_cu)

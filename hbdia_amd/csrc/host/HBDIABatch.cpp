@@ -229,14 +229,16 @@ int HBDIABatch<T>::residentWorkgroups(int kind) const {
 
 template <typename T>
 void HBDIABatch<T>::setSchedule(const BatchSchedule& s) {
-  if (s.kind != HBDIA_BATCH_INTERLEAVED && s.kind != HBDIA_BATCH_CONTIGUOUS)
+  if (s.kind != HBDIA_BATCH_INTERLEAVED && s.kind != HBDIA_BATCH_CONTIGUOUS && s.kind != HBDIA_BATCH_XCD)
     throw std::invalid_argument("HBDIABatch::setSchedule: unknown kind");
   if (s.tilesPerStep < 1 || s.workgroups < 0)
     throw std::invalid_argument("HBDIABatch::setSchedule: tilesPerStep >= 1 and workgroups >= 0");
-  const int wg = s.workgroups > 0 ? s.workgroups : residentWorkgroups(s.kind);
-  if (s.kind == HBDIA_BATCH_CONTIGUOUS) {
-    // wave w takes the tiles whose work prefix falls in [w, w+1) * total / W
-    const int64_t W = static_cast<int64_t>(wg) * 4;
+  int wg = s.workgroups > 0 ? s.workgroups : residentWorkgroups(s.kind);
+  if (s.kind == HBDIA_BATCH_XCD) wg = std::max(8, wg / 8 * 8);  // the same number on every XCD
+  if (s.kind != HBDIA_BATCH_INTERLEAVED) {
+    // contiguous: wave w takes the tiles whose work prefix falls in
+    // [w, w+1) * total / W; XCD split: the same cut into W = 8 ranges
+    const int64_t W = s.kind == HBDIA_BATCH_XCD ? 8 : static_cast<int64_t>(wg) * 4;
     std::vector<int32_t> bounds(static_cast<size_t>(W) + 1);
     const long double total = static_cast<long double>(tileWork_.back());
     for (int64_t w = 0; w <= W; ++w) {
@@ -395,10 +397,10 @@ std::unique_ptr<HBDIABatch<T>> autotuneBatch(const std::vector<HBDIA<T>*>& matri
       std::vector<const HBDIA<T>*> cm(matrices.begin(), matrices.end());
       auto batch = std::make_unique<HBDIABatch<T>>(cm);
       std::vector<BatchSchedule> cands;
-      for (int kind : {HBDIA_BATCH_INTERLEAVED, HBDIA_BATCH_CONTIGUOUS}) {
+      for (int kind : {HBDIA_BATCH_INTERLEAVED, HBDIA_BATCH_XCD, HBDIA_BATCH_CONTIGUOUS}) {
         const int R = batch->residentWorkgroups(kind);
         for (int wg : {R, std::max(1, R / 2)}) {
-          if (kind == HBDIA_BATCH_INTERLEAVED)
+          if (kind != HBDIA_BATCH_CONTIGUOUS)
             for (int tps : {1, 2, 4, 8}) cands.push_back(BatchSchedule{kind, tps, wg});
           else
             cands.push_back(BatchSchedule{kind, 1, wg});

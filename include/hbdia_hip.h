@@ -267,7 +267,13 @@ int hbdia_hbdia_spmv_mixed_noalloc(int vdtype, const void* values_lp, const doub
  *   HBDIA_BATCH_INTERLEAVED: runs of tiles_per_step consecutive tiles, wave w
  *     taking runs w, w + W, w + 2W, ... (W = 4 num_workgroups);
  *   HBDIA_BATCH_CONTIGUOUS: tiles [wave_tiles[w], wave_tiles[w+1]), a split
- *     that the host balances by work (wave_tiles has W + 1 entries).
+ *     that the host balances by work (wave_tiles has W + 1 entries);
+ *   HBDIA_BATCH_XCD: the tiles cut into eight contiguous ranges balanced by
+ *     work, [wave_tiles[k], wave_tiles[k+1]) (wave_tiles has 9 entries), range
+ *     k taken by the workgroups that run on XCD k (workgroup b runs on XCD
+ *     b % 8; num_workgroups a multiple of 8) with the interleaved order inside
+ *     it.  A matrix then runs on one XCD, whose L2 keeps its x lines for the
+ *     band loads of neighbouring tiles and the fallback's gathers.
  * A tile record carries its block's first offsets, so a wave needs one
  * scalar load per tile, issued for the next tile while the current tile's
  * loads are in flight.  Pools need padding past their last entry: offsets
@@ -275,6 +281,7 @@ int hbdia_hbdia_spmv_mixed_noalloc(int vdtype, const void* values_lp, const doub
  * HBDIA_BATCH_COO_PAD entries (any values; columns must be 0), values 64. */
 #define HBDIA_BATCH_INTERLEAVED 0
 #define HBDIA_BATCH_CONTIGUOUS 1
+#define HBDIA_BATCH_XCD 2
 #define HBDIA_BATCH_COO_PAD 64
 
 /* One matrix of a batch, as placed in the pools (host-side description). */
@@ -322,7 +329,7 @@ typedef struct hbdia_batch {
 } hbdia_batch_t;
 
 typedef struct hbdia_batch_schedule {
-  int32_t kind;           /* HBDIA_BATCH_INTERLEAVED or HBDIA_BATCH_CONTIGUOUS */
+  int32_t kind;           /* HBDIA_BATCH_INTERLEAVED, HBDIA_BATCH_CONTIGUOUS or HBDIA_BATCH_XCD */
   int32_t tiles_per_step; /* interleaved: tiles per run, >= 1 */
   int32_t num_workgroups; /* persistent grid size, >= 1 */
 } hbdia_batch_schedule_t;
