@@ -393,27 +393,27 @@ __device__ __forceinline__ T bdia_sum_runs3(const T* __restrict__ vb, const int*
 // The same for a row whose block is per lane (block width not a multiple of
 // 64): `vp` is the lane's address in its block's first offset line, `op` the
 // lane's offset list; lanes leave the loop at their own count.
-template <typename T, bool X32>
+template <typename T, bool X32, int CH = kChunk>
 __device__ __forceinline__ T bdia_sum_lane(const T* __restrict__ vp, const int* __restrict__ op, int cnt,
                                            int64_t bw, const T* __restrict__ x, int64_t row, int64_t nrows) {
   T sum = T(0);
   const int64_t last = nrows - 1;
-  for (int o0 = 0; o0 < cnt; o0 += kChunk) {
-    int off[kChunk];
-    T v[kChunk], xl[kChunk];
+  for (int o0 = 0; o0 < cnt; o0 += CH) {
+    int off[CH];
+    T v[CH], xl[CH];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) off[k] = op[min(o0 + k, cnt - 1)];
+    for (int k = 0; k < CH; ++k) off[k] = op[min(o0 + k, cnt - 1)];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k)
+    for (int k = 0; k < CH; ++k)
       v[k] = __builtin_nontemporal_load(vp + static_cast<int64_t>(min(o0 + k, cnt - 1)) * bw);
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) {
+    for (int k = 0; k < CH; ++k) {
       int64_t c = row + off[k];
       c = c < 0 ? 0 : (c > last ? last : c);
       xl[k] = load_x<T, X32>(x, c);
     }
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) {
+    for (int k = 0; k < CH; ++k) {
       const int64_t c = row + off[k];
       const bool ok = (o0 + k < cnt) & (v[k] != T(0)) & (c >= 0) & (c < nrows);
       sum = masked_fma(ok, v[k], xl[k], sum);
@@ -2244,13 +2244,60 @@ __global__ void k_f64_to_bf16(const double* __restrict__ src, uint16_t* __restri
 
 __global__ void k_zero_i32(int* __restrict__ p) { *p = 0; }
 
-template <typename TV, bool HAS_COO, bool RUNS3>
+// FP64 recomputation of one row with the FP64 kernel's arithmetic: blocked
+// part in ascending offset order (FMA, zero and range skips), fallback entries
+// of the row in entry order (FMA), y = (bdia or old y) + coo.  The blocked
+// part takes CH offsets per round of loads: nine in the fix-up pass, three in
+// kernels that recompute their own flagged rows (few registers there).
+template <int CH>
+__device__ __forceinline__ void fix_row(int64_t row, const double* __restrict__ vals, const int* __restrict__ offs,
+                                        const int* __restrict__ blk, const double* __restrict__ x,
+                                        double* __restrict__ y, int64_t nrows, int nb, int bw,
+                                        const int* __restrict__ cr, const int* __restrict__ cc,
+                                        const double* __restrict__ cv, const int* __restrict__ gptr) {
+  const int64_t b = row / bw;
+  int start = 0, cnt = 0;
+  if (b < nb) {
+    start = blk[b];
+    cnt = blk[b + 1] - start;
+  }
+  double sum = 0.0;
+  if (cnt > 0) {
+    const double* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
+    sum = bdia_sum_lane<double, false, CH>(vp, offs + start, cnt, bw, x, row, nrows);
+  }
+  if (gptr) {
+    const int64_t g = row / kWave;
+    int lo = gptr[g], hi = gptr[g + 1];
+    const int end = hi;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cr[mid] < row) lo = mid + 1;
+      else hi = mid;
+    }
+    double cs = 0.0;
+    for (int i = lo; i < end && cr[i] == row; ++i) cs = __builtin_fma(cv[i], x[cc[i]], cs);
+    y[row] = (cnt > 0 ? sum : y[row]) + cs;
+  } else if (cnt > 0) {
+    y[row] = sum;
+  }
+}
+
+// FP64 arrays for recomputing flagged rows inside the reduced-precision
+// kernels (FIX_INLINE): with them a flagged row is recomputed by its own lane
+// before the kernel ends, and no fix-up pass follows.
+struct MixedFp64 {
+  const double* vals;
+  const double* cv;
+};
+
+template <typename TV, bool HAS_COO, bool RUNS3, bool FIX_INLINE>
 __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
     const TV* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb, int bw,
     const int* __restrict__ cr, const int* __restrict__ cc, const TV* __restrict__ cv,
     const int* __restrict__ gptr, float ratio, float u_store, uint64_t* __restrict__ flag_mask,
-    const uint32_t* __restrict__ runs3_blocks, int kshift) {
+    const uint32_t* __restrict__ runs3_blocks, int kshift, MixedFp64 f64) {
   __shared__ float s_v[kWavesPerWG][kWave];
   __shared__ float s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][kWave];
@@ -2352,6 +2399,8 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed(
   // every group stores its mask (zero included): no reset between calls
   const uint64_t m = __ballot(fix);
   if (lane == 0) flag_mask[grp] = m;
+  if (FIX_INLINE && fix)
+    fix_row<3>(row, f64.vals, offs, blk, x, y, nrows, nb, bw, cr, cc, f64.cv, HAS_COO ? gptr : nullptr);
   if (!active || fix) return;
   if constexpr (HAS_COO) {
     const double base = cnt > 0 ? static_cast<double>(sum) : y[row];
@@ -2446,10 +2495,12 @@ __device__ __forceinline__ void mixed_coo_group(int p0, int p1, int64_t r0, int 
 
 // The cancellation flag and the store of one 64-row group (rows r0 + lane)
 // from its blocked and fallback sums, exactly k_hbdia_mixed's
+// Returns the group's flag mask (the rows left for the FP64 recomputation).
 template <bool HAS_COO>
-__device__ __forceinline__ void mixed_store(int64_t grp, int64_t r0, int lane, float sum, float sabs, int cnt,
-                                            float cs, float cabs, int nterms, double* __restrict__ y, int64_t nrows,
-                                            float ratio, float u_store, uint64_t* __restrict__ flag_mask) {
+__device__ __forceinline__ uint64_t mixed_store(int64_t grp, int64_t r0, int lane, float sum, float sabs, int cnt,
+                                                float cs, float cabs, int nterms, double* __restrict__ y,
+                                                int64_t nrows, float ratio, float u_store,
+                                                uint64_t* __restrict__ flag_mask) {
   const int64_t row = r0 + lane;
   const bool active = row < nrows;
   const float tot = sum + cs;
@@ -2460,13 +2511,14 @@ __device__ __forceinline__ void mixed_store(int64_t grp, int64_t r0, int lane, f
                    (cnt > 0 || cs != 0.0f || cabs != 0.0f);
   const uint64_t m = __ballot(fix);
   if (lane == 0) flag_mask[grp] = m;
-  if (!active || fix) return;
+  if (!active || fix) return m;
   if constexpr (HAS_COO) {
     const double base = cnt > 0 ? static_cast<double>(sum) : y[row];
     __builtin_nontemporal_store(base + static_cast<double>(cs), y + row);
   } else {
     if (cnt > 0) __builtin_nontemporal_store(static_cast<double>(sum), y + row);
   }
+  return m;
 }
 
 // Four adjacent rows per lane for the reduced-precision kernel (block width a
@@ -2550,13 +2602,13 @@ __device__ __forceinline__ void mixed_coo_quad(const int* p, int64_t row0, int l
   wave_sync();
 }
 
-template <typename TV, bool HAS_COO>
+template <typename TV, bool HAS_COO, bool FIX_INLINE>
 __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
     const TV* __restrict__ vals, const int* __restrict__ offs, const int* __restrict__ blk,
     const double* __restrict__ x, double* __restrict__ y, int64_t nrows, int nb, int bw,
     const int* __restrict__ cr, const int* __restrict__ cc, const TV* __restrict__ cv,
     const int* __restrict__ gptr, float ratio, float u_store, uint64_t* __restrict__ flag_mask,
-    const uint32_t* __restrict__ runs3_blocks, int kshift) {
+    const uint32_t* __restrict__ runs3_blocks, int kshift, MixedFp64 f64) {
   __shared__ float s_v[kWavesPerWG][kWave];
   __shared__ float s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][4 * kWave];
@@ -2691,49 +2743,24 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
                             s_hi[wv], cs[q], cabs[q], nterms[q]);
     }
   }
+  uint64_t fm[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (q < nq)
-      mixed_store<HAS_COO>(4 * w4 + q, row0 + q * kWave, lane, sum[q], sab[q], cq[q], cs[q], cabs[q], nterms[q], y,
-                           nrows, ratio, u_store, flag_mask);
+      fm[q] = mixed_store<HAS_COO>(4 * w4 + q, row0 + q * kWave, lane, sum[q], sab[q], cq[q], cs[q], cabs[q],
+                                   nterms[q], y, nrows, ratio, u_store, flag_mask);
+  }
+  if constexpr (FIX_INLINE) {
+    // flagged rows, recomputed in FP64 by their own lanes once every sum is stored
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (fm[q] != 0 && ((fm[q] >> lane) & 1u))
+        fix_row<3>(row0 + q * kWave + lane, f64.vals, offs, blk, x, y, nrows, nb, bw, cr, cc, f64.cv,
+                       HAS_COO ? gptr : nullptr);
+    }
   }
 }
 
-// FP64 recomputation of one row with the FP64 kernel's arithmetic: blocked
-// part in ascending offset order (FMA, zero and range skips), fallback entries
-// of the row in entry order (FMA), y = (bdia or old y) + coo.
-__device__ __forceinline__ void fix_row(int64_t row, const double* __restrict__ vals, const int* __restrict__ offs,
-                                        const int* __restrict__ blk, const double* __restrict__ x,
-                                        double* __restrict__ y, int64_t nrows, int nb, int bw,
-                                        const int* __restrict__ cr, const int* __restrict__ cc,
-                                        const double* __restrict__ cv, const int* __restrict__ gptr) {
-  const int64_t b = row / bw;
-  int start = 0, cnt = 0;
-  if (b < nb) {
-    start = blk[b];
-    cnt = blk[b + 1] - start;
-  }
-  double sum = 0.0;
-  if (cnt > 0) {
-    const double* vp = vals + static_cast<int64_t>(start) * bw + (row - b * bw);
-    sum = bdia_sum_lane<double, false>(vp, offs + start, cnt, bw, x, row, nrows);
-  }
-  if (gptr) {
-    const int64_t g = row / kWave;
-    int lo = gptr[g], hi = gptr[g + 1];
-    const int end = hi;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (cr[mid] < row) lo = mid + 1;
-      else hi = mid;
-    }
-    double cs = 0.0;
-    for (int i = lo; i < end && cr[i] == row; ++i) cs = __builtin_fma(cv[i], x[cc[i]], cs);
-    y[row] = (cnt > 0 ? sum : y[row]) + cs;
-  } else if (cnt > 0) {
-    y[row] = sum;
-  }
-}
 
 // Recomputes every row whose bit is set in flag_mask (one 64-bit mask per
 // 64-row group).  Each wave scans the masks of 64 consecutive groups
@@ -2766,7 +2793,7 @@ __global__ __launch_bounds__(256) void k_fix_groups(const double* __restrict__ v
       qn += __popcll(m);
       if (qn >= kWave) {  // a full batch: recompute q[0..63], move the rest down
         wave_sync();
-        fix_row(q[lane], vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+        fix_row<kChunk>(q[lane], vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
         const int64_t rest = lane + kWave < qn ? q[lane + kWave] : 0;
         wave_sync();
         if (lane + kWave < qn) q[lane] = rest;
@@ -2776,7 +2803,7 @@ __global__ __launch_bounds__(256) void k_fix_groups(const double* __restrict__ v
     }
   }
   wave_sync();
-  if (lane < qn) fix_row(q[lane], vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
+  if (lane < qn) fix_row<kChunk>(q[lane], vals, offs, blk, x, y, nrows, nb, bw, cr, cc, cv, gptr);
 }
 
 // COO fallback alone: res[r] = row sum for rows with entries (others keep res).
@@ -3032,32 +3059,49 @@ int launch_mixed(int vdtype, const void* vals_lp, const double* vals64, const in
   const float r = static_cast<float>(ratio);
   const float u_store = vdtype == HBDIA_VALUES_F32 ? 5.9604645e-08f : 3.9062500e-03f + 5.9604645e-08f;
   const int ks = xcd_chunk_shift(grid.x);
+  // FP32 storage flags few rows: each is recomputed in FP64 by its own lane
+  // at the end of the kernel (a short stall of that wave, hidden by the
+  // others), and no fix-up pass follows (192^3: 0.1775 -> 0.1735 ms per
+  // SpMV).  BF16 storage flags many more (a third of the waves at ratio 1e4 at
+  // 192^3): in-kernel they measured 0.142 -> 0.185 ms, so they go to the
+  // fix-up pass, which recomputes them 64 at a time.
+  const bool inl = vdtype == HBDIA_VALUES_F32;
+  const MixedFp64 f64{vals64, cv64};
   if (runs3) {
     // 256-row waves, four rows per lane
     const int64_t wgs4 = (nrows + 4 * kWG - 1) / (4 * kWG);
     const dim3 grid2(static_cast<unsigned>(wgs4));
     const int ks2 = xcd_chunk_shift(wgs4);
-#define HBDIA_LAUNCH_MX2(TV, C)                                                                                   \
-  hipLaunchKernelGGL((k_hbdia_mixed_row4<TV, C>), grid2, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, x, \
-                     y, nrows, nb, bw, cr, cc, static_cast<const TV*>(cv_lp), gptr, r, u_store, mask, runs3, ks2)
-    if (vdtype == HBDIA_VALUES_F32) { if (with_coo) HBDIA_LAUNCH_MX2(float, true); else HBDIA_LAUNCH_MX2(float, false); }
-    else { if (with_coo) HBDIA_LAUNCH_MX2(uint16_t, true); else HBDIA_LAUNCH_MX2(uint16_t, false); }
+#define HBDIA_LAUNCH_MX2(TV, C, I)                                                                                \
+  hipLaunchKernelGGL((k_hbdia_mixed_row4<TV, C, I>), grid2, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, \
+                     x, y, nrows, nb, bw, cr, cc, static_cast<const TV*>(cv_lp), gptr, r, u_store, mask, runs3, ks2, f64)
+    if (vdtype == HBDIA_VALUES_F32) {
+      if (with_coo) { if (inl) HBDIA_LAUNCH_MX2(float, true, true); else HBDIA_LAUNCH_MX2(float, true, false); }
+      else { if (inl) HBDIA_LAUNCH_MX2(float, false, true); else HBDIA_LAUNCH_MX2(float, false, false); }
+    } else {
+      if (with_coo) { if (inl) HBDIA_LAUNCH_MX2(uint16_t, true, true); else HBDIA_LAUNCH_MX2(uint16_t, true, false); }
+      else { if (inl) HBDIA_LAUNCH_MX2(uint16_t, false, true); else HBDIA_LAUNCH_MX2(uint16_t, false, false); }
+    }
 #undef HBDIA_LAUNCH_MX2
   } else {
-#define HBDIA_LAUNCH_MX(TV, C, R)                                                                             \
-  hipLaunchKernelGGL((k_hbdia_mixed<TV, C, R>), grid, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, x, \
-                     y, nrows, nb, bw, cr, cc, static_cast<const TV*>(cv_lp), gptr, r, u_store, mask, runs3, ks)
-  if (vdtype == HBDIA_VALUES_F32) {
-    if (with_coo) HBDIA_LAUNCH_MX(float, true, false); else HBDIA_LAUNCH_MX(float, false, false);
-  } else {
-    if (with_coo) HBDIA_LAUNCH_MX(uint16_t, true, false); else HBDIA_LAUNCH_MX(uint16_t, false, false);
-  }
+#define HBDIA_LAUNCH_MX(TV, C, I)                                                                             \
+  hipLaunchKernelGGL((k_hbdia_mixed<TV, C, false, I>), grid, block, 0, s, static_cast<const TV*>(vals_lp), offs, blk, \
+                     x, y, nrows, nb, bw, cr, cc, static_cast<const TV*>(cv_lp), gptr, r, u_store, mask, runs3, ks, f64)
+    if (vdtype == HBDIA_VALUES_F32) {
+      if (with_coo) { if (inl) HBDIA_LAUNCH_MX(float, true, true); else HBDIA_LAUNCH_MX(float, true, false); }
+      else { if (inl) HBDIA_LAUNCH_MX(float, false, true); else HBDIA_LAUNCH_MX(float, false, false); }
+    } else {
+      if (with_coo) { if (inl) HBDIA_LAUNCH_MX(uint16_t, true, true); else HBDIA_LAUNCH_MX(uint16_t, true, false); }
+      else { if (inl) HBDIA_LAUNCH_MX(uint16_t, false, true); else HBDIA_LAUNCH_MX(uint16_t, false, false); }
+    }
 #undef HBDIA_LAUNCH_MX
   }
-  const int64_t scan_waves = (groups + kWave - 1) / kWave;
-  const int64_t fix_blocks = std::min<int64_t>((scan_waves + 3) / 4, 2048);
-  hipLaunchKernelGGL(k_fix_groups, dim3(static_cast<unsigned>(fix_blocks)), dim3(256), 0, s, vals64, offs, blk, x, y,
-                     nrows, nb, bw, cr, cc, cv64, gptr, mask);
+  if (!inl) {
+    const int64_t scan_waves = (groups + kWave - 1) / kWave;
+    const int64_t fix_blocks = std::min<int64_t>((scan_waves + 3) / 4, 2048);
+    hipLaunchKernelGGL(k_fix_groups, dim3(static_cast<unsigned>(fix_blocks)), dim3(256), 0, s, vals64, offs, blk, x,
+                       y, nrows, nb, bw, cr, cc, cv64, gptr, mask);
+  }
   return check_hip(hipGetLastError(), name);
 }
 
