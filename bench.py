@@ -255,11 +255,10 @@ def bench_spmv(args) -> dict | int:
 
     stream = A.device_ptrs()["bdia_stream"]
     ev0, ev1 = Event(), Event()
+    # settle, then the W warm-ups and the K back-to-back steps that `value` and the roofline are
+    # measured on; the reference's own method (each call alone after a synchronize, which lets the
+    # clocks sag between calls) runs after them
     settle_calls = _settle(step, A.synchronize)
-    # the reference's own method first (each call alone after a synchronize), then the W warm-ups
-    # and the K back-to-back steps that `value` and the roofline are measured on
-    refm = _reference_method(A, X, Y, gpu_coo)
-    log(f"reference timing method (synchronized per call, 100 rounds, first dropped): {refm}")
     for _ in range(args.warmup):
         step()
     A.synchronize()
@@ -277,6 +276,8 @@ def bench_spmv(args) -> dict | int:
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     log(f"{args.steps} steps: {wall_ms:.4f} ms/step wall, {kern_ms:.4f} ms/launch on the stream, "
         f"{achieved:.1f} GB/s algorithmic")
+    refm = _reference_method(A, X, Y, gpu_coo)
+    log(f"reference timing method (synchronized per call, 100 rounds, first dropped): {refm}")
 
     traffic = None
     if args.workload == "c2" and (args.grid or 256) == 256 and args.dtype == "f64":
