@@ -123,13 +123,12 @@ HBDIAServer<T>::HBDIAServer(const std::vector<const HBDIA<T>*>& matrices, const 
     const int rings = cfg_.maxLaunchesInFlight;
     evIn_.assign(rings, nullptr);
     evComp_.assign(rings, nullptr);
-    evDone_.assign(rings, nullptr);
+
     hostJobs_.assign(rings, nullptr);
     devJobs_.assign(rings, nullptr);
     for (int r = 0; r < rings; ++r) {
       CHECK_HBDIA(hbdia_event_create(&evIn_[r]));
       CHECK_HBDIA(hbdia_event_create(&evComp_[r]));
-      CHECK_HBDIA(hbdia_event_create(&evDone_[r]));
       const size_t bytes = static_cast<size_t>(cfg_.maxJobsPerLaunch) * sizeof(hbdia_serve_job_t);
       CHECK_HBDIA(hbdia_malloc_host(&p, bytes));
       hostJobs_[r] = static_cast<hbdia_serve_job_t*>(p);
@@ -140,7 +139,7 @@ HBDIAServer<T>::HBDIAServer(const std::vector<const HBDIA<T>*>& matrices, const 
   } catch (...) {
     for (auto* q : hostJobs_) if (q) (void)hbdia_free_host(q);
     for (auto* q : devJobs_) if (q) (void)hbdia_free(q);
-    for (auto* v : {&evIn_, &evComp_, &evDone_})
+    for (auto* v : {&evIn_, &evComp_})
       for (auto e : *v) if (e) (void)hbdia_event_destroy(e);
     for (auto s : {sIn_, sComp_, sOut_}) if (s) (void)hbdia_stream_destroy(s);
     if (xArena_) (void)hbdia_free(xArena_);
@@ -161,7 +160,10 @@ HBDIAServer<T>::~HBDIAServer() {
   for (auto s : {sIn_, sComp_, sOut_}) (void)hbdia_stream_synchronize(s);
   for (auto* q : hostJobs_) (void)hbdia_free_host(q);
   for (auto* q : devJobs_) (void)hbdia_free(q);
-  for (auto* v : {&evIn_, &evComp_, &evDone_})
+  for (auto& L : inflight_)
+    for (size_t i = L.retired; i < L.done.size(); ++i)
+      if (L.done[i] != evComp_[L.ring]) (void)hbdia_event_destroy(L.done[i]);
+  for (auto* v : {&evIn_, &evComp_, &eventPool_})
     for (auto e : *v) (void)hbdia_event_destroy(e);
   for (auto s : {sIn_, sComp_, sOut_}) (void)hbdia_stream_destroy(s);
   (void)hbdia_free(xArena_);
@@ -252,7 +254,19 @@ void HBDIAServer<T>::fail(std::exception_ptr e) {
 }
 
 template <typename T>
-void HBDIAServer<T>::issue(std::vector<int64_t>& ids, int ring) {
+hbdia_event_t HBDIAServer<T>::takeEvent() {
+  if (!eventPool_.empty()) {
+    hbdia_event_t e = eventPool_.back();
+    eventPool_.pop_back();
+    return e;
+  }
+  hbdia_event_t e = nullptr;
+  CHECK_HBDIA(hbdia_event_create(&e));
+  return e;
+}
+
+template <typename T>
+void HBDIAServer<T>::issue(std::vector<int64_t>& ids, int ring, std::vector<hbdia_event_t>& done) {
   // the job table and the copies in.  Batched, a request whose vectors are on
   // the device is read and written in place (no arena slot); a host request's
   // x goes into its arena slot.  The serve kernel writes every row of y, so
@@ -293,13 +307,21 @@ void HBDIAServer<T>::issue(std::vector<int64_t>& ids, int ring) {
   }
   CHECK_HBDIA(hbdia_event_record(evComp_[ring], sComp_));
   CHECK_HBDIA(hbdia_stream_wait_event(sOut_, evComp_[ring]));
+  // each request completes when its own y is final: one copied out does not
+  // wait for the copies of the requests after it in the launch, and one
+  // written in place completes with the kernel
+  done.clear();
   for (const Request& r : rs) {
-    if (r.slot < 0) continue;
+    if (r.slot < 0) {
+      done.push_back(evComp_[ring]);
+      continue;
+    }
     const size_t bytes = static_cast<size_t>(pool_->numRows(r.matrix)) * T_;
     if (r.device) CHECK_HBDIA(hbdia_memcpy_d2d(r.y, yArena_ + r.slot, bytes, sOut_));
     else CHECK_HBDIA(hbdia_memcpy_d2h(r.y, yArena_ + r.slot, bytes, sOut_));
+    done.push_back(takeEvent());
+    CHECK_HBDIA(hbdia_event_record(done.back(), sOut_));
   }
-  CHECK_HBDIA(hbdia_event_record(evDone_[ring], sOut_));
 }
 
 template <typename T>
@@ -309,25 +331,35 @@ void HBDIAServer<T>::loop() {
     while (true) {
       bool progress = false;
       // retire finished launches, oldest first
+      // (a launch's requests complete in order: their events are recorded in
+      // order on one stream)
       while (!inflight_.empty()) {
-        int fin = 0;
-        CHECK_HBDIA(hbdia_event_query(evDone_[inflight_.front().ring], &fin));
-        if (!fin) break;
-        const int64_t t = nowNs();
-        {
-          std::lock_guard<std::mutex> g(mu_);
-          for (int64_t id : inflight_.front().ids) {
-            Request& r = requests_[id];
+        Launch& L = inflight_.front();
+        bool any = false;
+        while (L.retired < L.ids.size()) {
+          int fin = 0;
+          CHECK_HBDIA(hbdia_event_query(L.done[L.retired], &fin));
+          if (!fin) break;
+          const int64_t t = nowNs();
+          {
+            std::lock_guard<std::mutex> g(mu_);
+            Request& r = requests_[L.ids[L.retired]];
             r.state = kDone;
             r.t.completeNs = t;
             if (r.slot >= 0) arena_.release(r.slot);
             ++completed_;
           }
+          if (L.done[L.retired] != evComp_[L.ring]) eventPool_.push_back(L.done[L.retired]);
+          ++L.retired;
+          any = true;
         }
-        ringFree_.push_back(inflight_.front().ring);
+        if (any) {
+          cvDone_.notify_all();
+          progress = true;
+        }
+        if (L.retired < L.ids.size()) break;
+        ringFree_.push_back(L.ring);
         inflight_.pop_front();
-        cvDone_.notify_all();
-        progress = true;
       }
       // admit queued requests in arrival order while the launch and the arena have room
       {
@@ -351,13 +383,16 @@ void HBDIAServer<T>::loop() {
       if (!next.empty() && !ringFree_.empty()) {
         const int ring = ringFree_.back();
         ringFree_.pop_back();
-        issue(next, ring);
+        Launch L;
+        L.ring = ring;
+        issue(next, ring, L.done);
         {
           std::lock_guard<std::mutex> g(mu_);
           ++launches_;
           largest_ = std::max<int64_t>(largest_, static_cast<int64_t>(next.size()));
         }
-        inflight_.push_back(Launch{next, ring});
+        L.ids = next;
+        inflight_.push_back(std::move(L));
         next.clear();
         progress = true;
       }

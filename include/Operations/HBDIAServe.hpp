@@ -119,11 +119,17 @@ class HBDIAServer {
   };
   struct Launch {
     std::vector<int64_t> ids;
+    // per request: the event after which its y is final.  A request whose y is
+    // copied out has its own (from eventPool_); one written in place shares the
+    // launch's kernel event evComp_[ring].
+    std::vector<hbdia_event_t> done;
+    size_t retired = 0;               // requests of the launch already completed, in order
     int ring = 0;
   };
 
   void loop();
-  void issue(std::vector<int64_t>& ids, int ring);
+  void issue(std::vector<int64_t>& ids, int ring, std::vector<hbdia_event_t>& done);
+  hbdia_event_t takeEvent();
   void fail(std::exception_ptr e);
 
   ServeConfig cfg_;
@@ -134,7 +140,8 @@ class HBDIAServer {
   T* yArena_ = nullptr;
   int workgroups_ = 1;
   hbdia_stream_t sIn_ = nullptr, sComp_ = nullptr, sOut_ = nullptr;
-  std::vector<hbdia_event_t> evIn_, evComp_, evDone_;
+  std::vector<hbdia_event_t> evIn_, evComp_;
+  std::vector<hbdia_event_t> eventPool_;  // per-request completion events, reused
   std::vector<hbdia_serve_job_t*> hostJobs_;  // pinned, one table per ring slot
   std::vector<hbdia_serve_job_t*> devJobs_;
   std::vector<int> ringFree_;
