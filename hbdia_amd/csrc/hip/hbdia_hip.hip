@@ -1673,11 +1673,9 @@ __global__ __launch_bounds__(kWG) void k_hbdia_multi_row2(
 // column's store, which would otherwise hold them back on the in-order memory
 // counter.
 template <bool X32, bool INSIDE>
-__device__ __forceinline__ void multi_res_columns(const dbl2a8* v, const int* cen, const double* __restrict__ vb,
-                                                  int64_t vlane, const int* __restrict__ op, int bw,
-                                                  const double* __restrict__ X, int64_t ldx,
-                                                  double* __restrict__ Y, int64_t ldy, int nrhs, int64_t row0,
-                                                  int64_t row, int64_t nrows, int lane) {
+__device__ __forceinline__ void multi_res_columns(const dbl2a8* v, const int* cen, const double* __restrict__ X,
+                                                  int64_t ldx, double* __restrict__ Y, int64_t ldy, int nrhs,
+                                                  int64_t row0, int64_t row, int64_t nrows, int lane) {
   const int64_t last = nrows - 1;
   const bool act_a = row < nrows, act_b = row + 1 < nrows;
   const bool lane0 = lane == 0, lane63 = lane == kWave - 1;
@@ -1861,9 +1859,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(3))) void k
     inside = inside && row0 + o - 1 >= 0 && row0 + o + 2 * kWave <= last;  // wave-uniform
   }
   if (inside)
-    multi_res_columns<X32, true>(v, cen, vb, vlane, op, bw, X, ldx, Y, ldy, nrhs, row0, row, nrows, lane);
+    multi_res_columns<X32, true>(v, cen, X, ldx, Y, ldy, nrhs, row0, row, nrows, lane);
   else
-    multi_res_columns<X32, false>(v, cen, vb, vlane, op, bw, X, ldx, Y, ldy, nrhs, row0, row, nrows, lane);
+    multi_res_columns<X32, false>(v, cen, X, ldx, Y, ldy, nrhs, row0, row, nrows, lane);
 }
 
 // ---------------------------------------------------------------------------
