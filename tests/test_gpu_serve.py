@@ -260,3 +260,39 @@ def test_device_vectors_in_place_in_the_mix(gpu, dtype):
         assert outs[i].download().tobytes() == want[m, v].tobytes(), f"request {i} (matrix {m}) differs in the mix"
     assert S.info()["largest_launch"] > 1
     S.close()
+
+
+def test_host_and_device_requests_mixed_in_launches(gpu):
+    """Requests with host vectors (copied through the arena) and with device vectors (used in place)
+    submitted together from four threads share launches; each completes with its own y final and
+    equal to the oracle, and the server's completion order stays the submission order."""
+    print("case: serve host and device requests mixed", flush=True)
+    ms = pool()
+    xs = xs_for(ms, np.float64, variants=2)
+    want = {(m, v): expected(ms[m], xs[m][v]) for m in range(len(ms)) for v in range(2)}
+    dx = {(m, v): DeviceArray.from_host(xs[m][v]) for m in range(len(ms)) for v in range(2)}
+    S = HBDIAServer(ms, max_jobs=24, max_in_flight=2)
+    rng = np.random.default_rng(41)
+    picks = [(int(rng.integers(len(ms))), int(rng.integers(2)), bool(rng.integers(2))) for _ in range(200)]
+    outs = [DeviceArray.from_host(np.full(ms[m].getNumRows(), np.nan)) if dev else
+            np.full(ms[m].getNumRows(), np.nan) for m, _, dev in picks]
+    ids = [None] * len(picks)
+
+    def submitter(k):
+        for i in range(k, len(picks), 4):
+            m, v, dev = picks[i]
+            ids[i] = S.submit(m, dx[m, v] if dev else xs[m][v], outs[i], device=dev)
+
+    th = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    S.drain()
+    for i, (m, v, dev) in enumerate(picks):
+        got = outs[i].download() if dev else outs[i]
+        assert got.tobytes() == want[m, v].tobytes(), f"request {i} (matrix {m}, device={dev}) differs"
+    done_at = sorted((S.times(r)[2], r) for r in ids)
+    assert [r for _, r in done_at] == sorted(ids)  # completion in submission order
+    assert S.info()["largest_launch"] > 1
+    S.close()
