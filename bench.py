@@ -456,7 +456,7 @@ def bench_precision(args) -> dict:
 
 def bench_multirhs(args) -> dict:
     """k right-hand sides in one launch vs k single-vector launches, 128^3 27-point stencil."""
-    from hbdia_amd import HBDIA, DeviceArray, HBDIAVector, gen_vector, hbdiaSpMV, hbdiaSpMVMulti
+    from hbdia_amd import HBDIA, DeviceArray, HBDIAVector, gen_vector, hbdiaSpMV, hbdiaSpMVMulti, multi_ld
     n = args.grid or 128
     dt = np.float32 if args.dtype == "f32" else np.float64
     s_v = np.dtype(dt).itemsize
@@ -469,19 +469,26 @@ def bench_multirhs(args) -> dict:
     X1, Y1 = HBDIAVector(gen_vector(N, 1).astype(dt)), HBDIAVector(np.zeros(N, dt))
     _, t1 = _time_launches(lambda: hbdiaSpMV(A, X1, Y1, False, True), A.synchronize, stream, args.steps, args.warmup)
     rows = {}
+    # column-major X and Y with a leading dimension one 128-byte line past N (N * 8 bytes is a power
+    # of two)
+    ld = multi_ld(N, dt)
     for k in (1, 4, 8, 16, 32):
         Xh = np.concatenate([gen_vector(N, 10 + j) for j in range(k)]).astype(dt)
-        X, Y = DeviceArray.from_host(Xh), DeviceArray(k * N, dt, zero=True)
-        hbdiaSpMVMulti(A, X, Y, k)
+        Xp = np.zeros((k, ld), dt)
+        Xp[:, :N] = Xh.reshape(k, N)
+        X, Y = DeviceArray.from_host(Xp.ravel()), DeviceArray(k * ld, dt, zero=True)
+        hbdiaSpMVMulti(A, X, Y, k, ldx=ld, ldy=ld)
         A.synchronize()
+        Yc = Y.download().reshape(k, ld)[:, :N].ravel()
         # column k-1 must equal the single-vector result
         Xs = HBDIAVector(Xh[(k - 1) * N:])
         Ys = HBDIAVector(np.zeros(N, dt))
         hbdiaSpMV(A, Xs, Ys, False, True)
         A.synchronize()
-        if not np.array_equal(Y.download()[(k - 1) * N:], Ys.download()):
+        if not np.array_equal(Yc[(k - 1) * N:], Ys.download()):
             raise BenchError({"error": f"multi-RHS column mismatch at k={k}"})
-        _, tk = _time_launches(lambda: hbdiaSpMVMulti(A, X, Y, k), A.synchronize, stream, args.steps, args.warmup)
+        _, tk = _time_launches(lambda: hbdiaSpMVMulti(A, X, Y, k, ldx=ld, ldy=ld), A.synchronize, stream,
+                               args.steps, args.warmup)
         b = nnz * s_v + k * N * 2 * s_v
         # the same columns row-major (row r's k values adjacent): every column checked against the
         # column-major result, which the single-vector check above ties to hbdiaSpMV
@@ -489,7 +496,7 @@ def bench_multirhs(args) -> dict:
         Yr = DeviceArray(k * N, dt, zero=True)
         hbdiaSpMVMulti(A, Xr, Yr, k, layout="row")
         A.synchronize()
-        if not np.array_equal(Yr.download().reshape(N, k).T.ravel(), Y.download()):
+        if not np.array_equal(Yr.download().reshape(N, k).T.ravel(), Yc):
             raise BenchError({"error": f"row-major multi-RHS differs from column-major at k={k}"})
         _, tr = _time_launches(lambda: hbdiaSpMVMulti(A, Xr, Yr, k, layout="row"), A.synchronize, stream,
                                args.steps, args.warmup)
@@ -506,7 +513,7 @@ def bench_multirhs(args) -> dict:
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
             "data": "synthetic (seeded generator)",
             "config": {"workload": f"stencil27_{n}^3_multirhs", "grid": n, "rows": N, "block_width": 64,
-                       "single_vector_ms": round(t1, 5)},
+                       "ld": ld, "single_vector_ms": round(t1, 5)},
             "by_k": rows}
     return line
 

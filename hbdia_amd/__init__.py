@@ -11,11 +11,11 @@ if either is absent; there is no CPU fallback.
 from ._native import HBDIAError, NativeLibraryMissing, gpu_available
 from .api import (DEFAULT_BLOCK_WIDTH, DEFAULT_THRESHOLD, MAX_COO_ENTRIES, ExecutionMode, HBDIA, HBDIABatch, HBDIAMixed,
                   HBDIAServer, HBDIAVector, VectorArena,
-                  gen_vector, hbdiaSpMV, hbdiaSpMVMulti, set_num_threads)
+                  gen_vector, hbdiaSpMV, hbdiaSpMVMulti, multi_ld, set_num_threads)
 from .device import DeviceArray, Event, Graph, Stream, device_count, device_name, mem_info, synchronize
 
 __all__ = [
-    "HBDIA", "HBDIABatch", "HBDIAMixed", "HBDIAServer", "VectorArena", "HBDIAVector", "hbdiaSpMV", "hbdiaSpMVMulti", "ExecutionMode", "DEFAULT_BLOCK_WIDTH", "DEFAULT_THRESHOLD",
+    "HBDIA", "HBDIABatch", "HBDIAMixed", "HBDIAServer", "VectorArena", "HBDIAVector", "hbdiaSpMV", "hbdiaSpMVMulti", "multi_ld", "ExecutionMode", "DEFAULT_BLOCK_WIDTH", "DEFAULT_THRESHOLD",
     "MAX_COO_ENTRIES", "DeviceArray", "Stream", "Event", "Graph", "device_count", "device_name", "mem_info",
     "synchronize", "gen_vector", "set_num_threads", "gpu_available", "HBDIAError", "NativeLibraryMissing",
 ]

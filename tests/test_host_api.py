@@ -215,3 +215,16 @@ def test_kernel_calls_without_gpu_report_errors_not_crashes():
     assert _native.hip.hbdia_last_error()
     # invalid arguments are rejected before any device call
     assert _native.hip.hbdia_bdia_spmv_f64(None, None, None, None, None, 10, 1, 0, None) == 1
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 1000, 4096, 1 << 21, 128 ** 3 + 1, 3_000_000])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_multi_ld_pads_power_of_two_strides(n, dtype):
+    from hbdia_amd import multi_ld
+    size = np.dtype(dtype).itemsize
+    ld = multi_ld(n, dtype)
+    assert ld >= n and (ld * size) % 128 == 0          # whole 128-byte lines
+    assert (ld * size) % 4096 != 0                      # never a multiple of 4 KiB
+    assert ld - n < 2 * 128 // size                     # at most one line past the rounding
+    if (n * size) % 4096 == 0:
+        assert ld == n + 128 // size
