@@ -1950,6 +1950,88 @@ __device__ __forceinline__ void tile_pair(int value_line_a, int value_line_b, in
   }
 }
 
+// tile_pair for two tiles whose offset list is longer than one chunk (the
+// tile records name the same list: hbdia_batch_make_tiles gives equal lists of
+// consecutive tiles one offsets index).  The list is walked in chunks of
+// kChunk in ascending order, the FMA chain running on across chunks as in the
+// single-tile path; the next chunk's offsets (wave-uniform loads) are loaded
+// before the current chunk is summed.
+template <typename T, bool HAS_COO, bool X32>
+__device__ __forceinline__ void tile_pair_long(int value_line_a, int value_line_b, int row0, int num_rows, int bw,
+                                               int cnt, const int* first_off, const int* __restrict__ op, int p0,
+                                               int p1, int p2, int fused, const T* __restrict__ vals,
+                                               const int* __restrict__ cr, const int* __restrict__ cc,
+                                               const T* __restrict__ cv, const T* __restrict__ x, T* __restrict__ y,
+                                               int lane, T* sv, T* sx, int* slo, int* shi) {
+  using V2 = typename Vec2<T>::type;
+  const int64_t nrows = num_rows;
+  const int64_t row = static_cast<int64_t>(row0) + 2 * lane;
+  const bool act_a = row < nrows, act_b = row + 1 < nrows;
+  const int64_t vline = lane < kWave / 2 ? static_cast<int64_t>(value_line_a) * kWave + 2 * lane
+                                         : static_cast<int64_t>(value_line_b) * kWave + 2 * (lane - kWave / 2);
+  const T* vb = vals + vline;
+  const int64_t last = nrows - 1;
+  const uint32_t n32 = static_cast<uint32_t>(num_rows);
+  T ca = T(0), cb = T(0);
+  if constexpr (HAS_COO) {
+    if (fused && p0 < p2)
+      coo_pair_sum<T, true>(cr, cc, cv, x, p0, p1, p2, row0, lane, sv, sx, slo, shi, ca, cb);
+  }
+  int off[kChunk];
+#pragma unroll
+  for (int k = 0; k < kChunk; ++k) off[k] = first_off[k];
+  T sa = T(0), sb = T(0);
+  for (int o0 = 0; o0 < cnt; o0 += kChunk) {
+    const int lastk = min(kChunk, cnt - o0) - 1;
+    V2 v[kChunk], xv[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      v[k] = __builtin_nontemporal_load(
+          reinterpret_cast<const V2*>(vb + static_cast<int64_t>(o0 + min(k, lastk)) * bw));
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const int64_t o = off[k];
+      if (row0 + o >= 0 && row0 + o + 2 * kWave <= last + 1) {  // wave-uniform: every column inside x
+        xv[k] = *reinterpret_cast<const V2*>(x + (row + o));
+      } else {
+        int64_t c0 = row + o, c1 = row + 1 + o;
+        c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+        c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+        xv[k].x = x[c0];
+        xv[k].y = x[c1];
+      }
+    }
+    int cur[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) cur[k] = off[k];
+    if (o0 + kChunk < cnt) {  // the next chunk's offsets (the pool is padded past every list)
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) off[k] = op[o0 + kChunk + k];
+    }
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      const uint32_t c = static_cast<uint32_t>(row) + static_cast<uint32_t>(cur[k]);
+      const bool ink = k <= lastk;
+      const bool oka = ink & act_a & (v[k].x != T(0)) & (c < n32);
+      const bool okb = ink & act_b & (v[k].y != T(0)) & (c + 1u < n32);
+      sa = masked_fma(oka, v[k].x, xv[k].x, sa);
+      sb = masked_fma(okb, v[k].y, xv[k].y, sb);
+    }
+  }
+  if (HAS_COO && fused) {
+    sa = sa + ca;
+    sb = sb + cb;
+  }
+  if (act_b) {
+    V2 t;
+    t.x = sa;
+    t.y = sb;
+    __builtin_nontemporal_store(t, reinterpret_cast<V2*>(y + row));
+  } else if (act_a) {
+    __builtin_nontemporal_store(sa, y + row);
+  }
+}
+
 constexpr int kTileGroup = 16;  // tile records fetched together by one wave
 constexpr int kTileDwords = 20;  // 80-byte tile record
 static_assert(sizeof(hbdia_batch_tile_t) == kTileDwords * 4, "tile record layout");
@@ -2078,10 +2160,25 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
       // two adjacent rows per lane, 16-byte value loads; the arithmetic per row
       // is the single tile's below
       bool pair = jj + 1 < ng && nextf.rows == cf.rows && nextf.row0 == cf.row0 + kWave && nextf.bw == cf.bw &&
-                  nextf.cnt == cf.cnt && cf.cnt > 0 && cf.cnt <= kChunk && nextf.fused == cf.fused &&
+                  nextf.cnt == cf.cnt && cf.cnt > 0 && nextf.fused == cf.fused &&
                   (!HAS_COO || !cf.fused || (nextf.p0 == cf.p1 && nextf.p1 - cf.p0 <= kWave));
+      if (cf.cnt <= kChunk) {
 #pragma unroll
-      for (int k = 0; k < kChunk; ++k) pair = pair && (k >= cf.cnt || nextf.off[k] == cf.off[k]);
+        for (int k = 0; k < kChunk; ++k) pair = pair && (k >= cf.cnt || nextf.off[k] == cf.off[k]);
+      } else {
+        pair = pair && nextf.offs_at == cf.offs_at;  // one list (equal lists share an index)
+      }
+      if (pair && cf.cnt > kChunk) {
+        const Fields cg = nextf;
+        ++jj;
+        if (jj + 1 < ng) nextf = read_fields(jj + 1);
+        const T* xq = JOBS ? reinterpret_cast<const T*>(cf.xb) : xp + cf.rows;
+        T* yq = JOBS ? reinterpret_cast<T*>(cf.yb) : yp + cf.rows;
+        tile_pair_long<T, HAS_COO, X32>(cf.value_line, cg.value_line, cf.row0, cf.num_rows, cf.bw, cf.cnt, cf.off,
+                                        offs + cf.offs_at, cf.p0, cf.p1, cg.p1, cf.fused, vals, cr, cc, cv, xq, yq,
+                                        lane, sv, sx, slo, shi);
+        continue;
+      }
       if (pair) {
         const Fields cg = nextf;
         ++jj;
@@ -3517,6 +3614,7 @@ int hbdia_batch_make_tiles(const hbdia_batch_desc_t* desc, int32_t num_matrices,
     if (d.first_tile < 0 || d.first_tile + groups > num_tiles)
       return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_batch_make_tiles: tiles outside the tile array");
     const int32_t* bs = block_start + d.block_start;
+    int32_t prev_at = -1, prev_cnt = -1;  // the previous tile's list (canonical index)
     for (int64_t g = 0; g < groups; ++g) {
       hbdia_batch_tile_t& tl = tiles[d.first_tile + g];
       const int64_t row0 = g * kWave;
@@ -3538,7 +3636,15 @@ int hbdia_batch_make_tiles(const hbdia_batch_desc_t* desc, int32_t num_matrices,
       tl.num_rows = d.num_rows;
       tl.stride = d.block_width;
       tl.count = cnt;
-      tl.offsets = d.offsets + start;
+      // equal lists of consecutive tiles get one index, so the kernel can tell
+      // that two tiles share their offsets from the records alone
+      int32_t at = d.offsets + start;
+      if (cnt > 0 && cnt == prev_cnt && at != prev_at &&
+          std::equal(offsets + at, offsets + at + cnt, offsets + prev_at))
+        at = prev_at;
+      tl.offsets = at;
+      prev_at = at;
+      prev_cnt = cnt;
       for (int k = 0; k < HBDIA_BATCH_INLINE_OFFSETS; ++k)
         tl.first_offsets[k] = k < cnt ? offsets[d.offsets + start + k] : 0;
       if (d.coo_nnz > 0) {

@@ -308,7 +308,9 @@ typedef struct hbdia_batch_tile {
   int32_t num_rows;   /* rows of the matrix */
   int32_t stride;     /* block width: distance between offset lines, in values */
   int32_t count;      /* offsets of the tile's block (0: empty block) */
-  int32_t offsets;    /* offsets-pool index of the block's first offset */
+  int32_t offsets;    /* offsets-pool index of the block's offset list, or of the previous tile's
+                         when the two lists are equal (so equal lists of consecutive tiles share
+                         one index) */
   int32_t coo_begin;  /* fallback-pool range [coo_begin, coo_end) of the tile's rows */
   int32_t coo_end;
   int32_t fused;      /* 1: every row gets (blocked sum or old y) + fallback; 0: blocked part only */

@@ -71,6 +71,52 @@ def run_batch(batch, xs, y0s, graph=False):
     return Y.download()
 
 
+def long_list_matrix(dtype):
+    """320 rows, width-64 blocks of 11 offsets each: blocks 0, 1 and 4 hold one list, blocks 2 and 3
+    another of the same length (tiles 1 -> 2 must not pair)."""
+    a, b = list(range(-1, 10)), list(range(0, 11))
+    rows, cols = [], []
+    for blk, lst in enumerate([a, a, b, b, a]):
+        for r in range(64 * blk, 64 * blk + 64):
+            for o in lst:
+                if 0 <= r + o < 320:
+                    rows.append(r)
+                    cols.append(r + o)
+    vals = np.linspace(-2.0, 3.0, len(rows))
+    return HBDIA(np.array(rows, np.int32), np.array(cols, np.int32), vals.astype(dtype), 320, 320)
+
+
+def stencil_set(dtype, widths):
+    """Lists longer than one chunk: 27-point stencils with and without a fallback, next to a 7-point
+    stencil and the matrix of equal-length unequal lists."""
+    ms = [HBDIA.stencil(12, 27, 0.0, seed=21, dtype=dtype), HBDIA.stencil(16, 27, 0.01, seed=22, dtype=dtype),
+          HBDIA.stencil(9, 27, 0.02, seed=23, dtype=dtype), HBDIA.stencil(20, 7, 0.005, seed=24, dtype=dtype),
+          HBDIA.stencil(14, 27, 0.0, seed=25, dtype=dtype), long_list_matrix(dtype)]
+    for i, m in enumerate(ms):
+        m.convertToHBDIAFormatOnHost(widths[i % len(widths)], 2)
+    return ms
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("widths", [[64], [128, 64, 256]])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_long_offset_lists_pair_exactly(gpu, dtype, widths, kind):
+    """Tiles whose offset lists span several chunks run as pairs (two rows per lane) when their
+    records name one list; every matrix equals the oracle and hbdiaSpMV bit for bit on every
+    schedule kind."""
+    print(f"case: batch long lists {np.dtype(dtype).name} widths={widths} kind={kind}", flush=True)
+    ms = stencil_set(dtype, widths)
+    xs, y0s = pools(ms, dtype, seed=31)
+    B = HBDIABatch(ms)
+    B.set_schedule(kind, 2, 0)
+    y = run_batch(B, xs, y0s)
+    off = B.row_offsets
+    for i, m in enumerate(ms):
+        got = y[off[i]:off[i + 1]]
+        np.testing.assert_array_equal(got, oracle_y(m, xs[i], y0s[i]), err_msg=f"matrix {i} vs oracle")
+        np.testing.assert_array_equal(got, single(m, xs[i], y0s[i]), err_msg=f"matrix {i} vs hbdiaSpMV")
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_batch_equals_single_matrix_and_oracle(gpu, dtype):
     print(f"case: batch parity {np.dtype(dtype).name}", flush=True)

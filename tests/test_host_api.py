@@ -228,3 +228,34 @@ def test_multi_ld_pads_power_of_two_strides(n, dtype):
     assert ld - n < 2 * 128 // size                     # at most one line past the rounding
     if (n * size) % 4096 == 0:
         assert ld == n + 128 // size
+
+
+def test_batch_tiles_share_an_index_for_equal_offset_lists():
+    """hbdia_batch_make_tiles (host code) gives consecutive tiles whose blocks hold equal offset
+    lists one offsets index, so the batch kernel can pair them from their records alone; a
+    different list of the same length keeps its own index."""
+    import ctypes
+
+    class Desc(ctypes.Structure):
+        _fields_ = [("values", ctypes.c_int64), ("rows", ctypes.c_int64), ("first_tile", ctypes.c_int64),
+                    ("offsets", ctypes.c_int32), ("block_start", ctypes.c_int32), ("coo", ctypes.c_int32),
+                    ("coo_nnz", ctypes.c_int32), ("group_ptr", ctypes.c_int32), ("num_rows", ctypes.c_int32),
+                    ("num_blocks", ctypes.c_int32), ("block_width", ctypes.c_int32)]
+
+    a = list(range(-1, 10))          # 11 offsets
+    b = list(range(0, 11))           # 11 offsets, another list
+    lists = [a, a, b, b, a]          # blocks 0..4, width 64: one tile each
+    offs = np.array(sum(lists, []) + [0] * 16, np.int32)
+    starts = np.cumsum([0] + [len(l) for l in lists]).astype(np.int32)
+    d = Desc(values=0, rows=0, first_tile=0, offsets=0, block_start=0, coo=0, coo_nnz=0, group_ptr=0,
+             num_rows=320, num_blocks=5, block_width=64)
+    tiles = np.zeros((5, 20), np.int32)
+    rc = _native.hip.hbdia_batch_make_tiles(ctypes.byref(d), 1, offs.ctypes.data, starts.ctypes.data, None, 8,
+                                            tiles.ctypes.data, 5)
+    assert rc == 0, _native.hip.hbdia_last_error()
+    at = tiles[:, 6]                 # the offsets index of each tile
+    assert list(at) == [0, 0, 22, 22, 44]
+    assert list(tiles[:, 5]) == [11] * 5
+    for t, l in zip(tiles, lists):   # every index names the tile's own list
+        assert list(offs[t[6]:t[6] + 11]) == l
+        assert list(t[10:20]) == l[:10]
