@@ -1955,7 +1955,9 @@ __device__ __forceinline__ void tile_pair(int value_line_a, int value_line_b, in
 // consecutive tiles one offsets index).  The list is walked in chunks of
 // kChunk in ascending order, the FMA chain running on across chunks as in the
 // single-tile path; the next chunk's offsets (wave-uniform loads) are loaded
-// before the current chunk is summed.
+// before the current chunk is summed.  A full chunk of three runs of
+// consecutive offsets inside x loads each run's centre pair once and shifts
+// the outer neighbours in across lanes.
 template <typename T, bool HAS_COO, bool X32>
 __device__ __forceinline__ void tile_pair_long(int value_line_a, int value_line_b, int row0, int num_rows, int bw,
                                                int cnt, const int* first_off, const int* __restrict__ op, int p0,
@@ -1988,17 +1990,41 @@ __device__ __forceinline__ void tile_pair_long(int value_line_a, int value_line_
     for (int k = 0; k < kChunk; ++k)
       v[k] = __builtin_nontemporal_load(
           reinterpret_cast<const V2*>(vb + static_cast<int64_t>(o0 + min(k, lastk)) * bw));
+    // a full chunk of three runs of consecutive offsets whose windows lie
+    // inside x: one 16-byte load per run, the outer neighbours by lane shifts
+    // (the same x values as the per-offset loads)
+    bool runs = lastk == kChunk - 1;
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) {
-      const int64_t o = off[k];
-      if (row0 + o >= 0 && row0 + o + 2 * kWave <= last + 1) {  // wave-uniform: every column inside x
-        xv[k] = *reinterpret_cast<const V2*>(x + (row + o));
-      } else {
-        int64_t c0 = row + o, c1 = row + 1 + o;
-        c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
-        c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
-        xv[k].x = x[c0];
-        xv[k].y = x[c1];
+    for (int w = 0; w < kChunk / 3; ++w) {
+      const int64_t o = off[3 * w + 1];
+      runs = runs && off[3 * w] == o - 1 && off[3 * w + 2] == o + 1 && row0 + o - 1 >= 0 &&
+             row0 + o + 2 * kWave <= last;
+    }
+    if (runs) {
+#pragma unroll
+      for (int w = 0; w < kChunk / 3; ++w) {
+        const int64_t o = off[3 * w + 1];
+        const V2 m = *reinterpret_cast<const V2*>(x + (row + o));
+        const T el = x[row0 + o - 1], er = x[row0 + o + 2 * kWave];  // wave-uniform: scalar loads
+        xv[3 * w].x = wave_shr1(m.y, el);  // x[row + o - 1]
+        xv[3 * w].y = m.x;
+        xv[3 * w + 1] = m;
+        xv[3 * w + 2].x = m.y;
+        xv[3 * w + 2].y = wave_shl1(m.x, er);  // x[row + o + 2]
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) {
+        const int64_t o = off[k];
+        if (row0 + o >= 0 && row0 + o + 2 * kWave <= last + 1) {  // wave-uniform: every column inside x
+          xv[k] = *reinterpret_cast<const V2*>(x + (row + o));
+        } else {
+          int64_t c0 = row + o, c1 = row + 1 + o;
+          c0 = c0 < 0 ? 0 : (c0 > last ? last : c0);
+          c1 = c1 < 0 ? 0 : (c1 > last ? last : c1);
+          xv[k].x = x[c0];
+          xv[k].y = x[c1];
+        }
       }
     }
     int cur[kChunk];
