@@ -541,8 +541,9 @@ def bench_batch(args) -> dict:
         info = B.info()
         best = min(B.table, key=lambda r: r["ms"])
         worst = max(B.table, key=lambda r: r["ms"])
-        log(f"{name}: autotuned in {time.time() - t1:.1f}s over {len(B.table)} candidates: best {best}, "
-            f"worst {worst['ms']:.4f} ms")
+        log(f"{name}: autotuned in {time.time() - t1:.1f}s over {len(B.table)} candidates: chose block width "
+            f"{ms[0].getBlockWidth()}, kind {info['kind']}, tiles_per_step {info['tiles_per_step']}, workgroups "
+            f"{info['workgroups']}; fastest first timing {best}, slowest {worst['ms']:.4f} ms")
         N = B.rows
         if x64 is None:
             x64 = gen_vector(N, seed + 1)
@@ -575,9 +576,9 @@ def bench_batch(args) -> dict:
                                                                                    HBM_PEAK_GBS, 4),
              "per_matrix_launches_ms": round(plain, 5), "speedup_vs_per_matrix": round(plain / kern, 3),
              "rows": N, "nnz_bdia": info["bdia_nnz"], "nnz_coo": info["coo_nnz"], "algorithmic_bytes": abytes,
-             "stored_bytes": info["stored_bytes"], "block_width": best["block_width"],
-             "schedule": {"kind": ["interleaved", "contiguous", "xcd"][best["kind"]],
-                          "tiles_per_step": best["tiles_per_step"], "workgroups": best["workgroups"]},
+             "stored_bytes": info["stored_bytes"], "block_width": ms[0].getBlockWidth(),
+             "schedule": {"kind": ["interleaved", "contiguous", "xcd"][info["kind"]],
+                          "tiles_per_step": info["tiles_per_step"], "workgroups": info["workgroups"]},
              "candidates": len(B.table), "worst_candidate_ms": round(worst["ms"], 5), "parity": "bit-exact"}
         if name == "fp64":
             y64 = y

@@ -564,7 +564,7 @@ class HBDIABatch:
 
     INTERLEAVED, CONTIGUOUS = 0, 1
     _INFO = ["matrices", "rows", "tiles", "bdia_nnz", "coo_nnz", "algorithmic_bytes", "stored_bytes", "flops",
-             "kind", "tiles_per_step", "workgroups", "dtype"]
+             "kind", "tiles_per_step", "workgroups", "dtype", "long_lists"]
 
     def __init__(self, matrices: list[HBDIA] | None = None, _handle=None, table=None):
         if _handle is None:

@@ -2069,7 +2069,7 @@ static_assert(HBDIA_BATCH_INLINE_OFFSETS >= kChunk, "a tile record carries the f
 // tile) by a binary search over the jobs' work_begin.  A job's y is taken to
 // start at zero: every row is written, rows without blocks as 0 (+ fallback),
 // so the caller clears nothing.
-template <typename T, bool HAS_COO, bool X32, bool CONTIG, bool JOBS>
+template <typename T, bool HAS_COO, bool X32, bool CONTIG, bool JOBS, bool LONG>
 __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__ tiles,
                                                      const int32_t* __restrict__ wave_tiles,
                                                      const T* __restrict__ vals, const int* __restrict__ offs,
@@ -2078,6 +2078,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
                                                      const hbdia_serve_job_t* __restrict__ jobs, int num_jobs,
                                                      const T* __restrict__ xp, T* __restrict__ yp) {
   static_assert(!(JOBS && CONTIG), "jobs run on the interleaved schedule");
+  static_assert(kChunk == HBDIA_BATCH_CHUNK, "chunk of the batch kernel");
   __shared__ T s_v[kWavesPerWG][kWave];
   __shared__ T s_x[kWavesPerWG][kWave];
   __shared__ int s_lo[kWavesPerWG][2 * kWave];  // 128 rows for a pair of tiles
@@ -2194,7 +2195,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
       } else {
         pair = pair && nextf.offs_at == cf.offs_at;  // one list (equal lists share an index)
       }
-      if (pair && cf.cnt > kChunk) {
+      if (LONG && pair && cf.cnt > kChunk) {
         const Fields cg = nextf;
         ++jj;
         if (jj + 1 < ng) nextf = read_fields(jj + 1);
@@ -2205,7 +2206,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
                                         lane, sv, sx, slo, shi);
         continue;
       }
-      if (pair) {
+      if (pair && cf.cnt <= kChunk) {
         const Fields cg = nextf;
         ++jj;
         if (jj + 1 < ng) nextf = read_fields(jj + 1);
@@ -3259,13 +3260,24 @@ int launch_batch(const hbdia_batch_t* bt, const hbdia_batch_schedule_t* sc, cons
   const dim3 grid(static_cast<unsigned>(sc->num_workgroups)), block(kWG);
   hipStream_t s = hs(stream);
   const bool coo = bt->has_coo != 0, x32 = bt->x32 != 0, contig = sc->kind == HBDIA_BATCH_CONTIGUOUS;
+  const bool lng = bt->long_lists != 0;  // the contiguous schedule runs long lists unpaired
   // interleaved: no split array (the kernel reads wave_tiles as the XCD split when given one)
   const int32_t* split = sc->kind == HBDIA_BATCH_INTERLEAVED ? nullptr : bt->wave_tiles;
-#define HBDIA_BATCH_LAUNCH(C, X, G) \
-  hipLaunchKernelGGL((k_hbdia_batch<T, C, X, G, false>), grid, block, 0, s,                                   \
-                     reinterpret_cast<const int32_t*>(bt->tiles), split,                                    \
-                     static_cast<const T*>(bt->values), bt->offsets, bt->coo_rows, bt->coo_cols,            \
-                     static_cast<const T*>(bt->coo_values), bt->num_tiles, sc->tiles_per_step, nullptr, 0, x, y)
+#define HBDIA_BATCH_LAUNCH(C, X, G)                                                                       \
+  do {                                                                                                    \
+    if (lng && !G)                                                                                        \
+      hipLaunchKernelGGL((k_hbdia_batch<T, C, X, false, false, true>), grid, block, 0, s,                 \
+                         reinterpret_cast<const int32_t*>(bt->tiles), split,                              \
+                         static_cast<const T*>(bt->values), bt->offsets, bt->coo_rows, bt->coo_cols,      \
+                         static_cast<const T*>(bt->coo_values), bt->num_tiles, sc->tiles_per_step,        \
+                         nullptr, 0, x, y);                                                               \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_hbdia_batch<T, C, X, G, false, false>), grid, block, 0, s,                    \
+                         reinterpret_cast<const int32_t*>(bt->tiles), split,                              \
+                         static_cast<const T*>(bt->values), bt->offsets, bt->coo_rows, bt->coo_cols,      \
+                         static_cast<const T*>(bt->coo_values), bt->num_tiles, sc->tiles_per_step,        \
+                         nullptr, 0, x, y);                                                               \
+  } while (0)
   if (coo) {
     if (x32) {
       if (contig) HBDIA_BATCH_LAUNCH(true, true, true); else HBDIA_BATCH_LAUNCH(true, true, false);
@@ -3297,11 +3309,21 @@ int launch_serve(const hbdia_batch_t* bt, const hbdia_serve_job_t* jobs, int32_t
   const T* no_x = nullptr;  // each job carries its own x and y
   T* no_y = nullptr;
   hipStream_t s = hs(stream);
-#define HBDIA_SERVE_LAUNCH(C, X) \
-  hipLaunchKernelGGL((k_hbdia_batch<T, C, X, false, true>), grid, block, 0, s,                                \
-                     reinterpret_cast<const int32_t*>(bt->tiles), nullptr, static_cast<const T*>(bt->values), \
-                     bt->offsets, bt->coo_rows, bt->coo_cols, static_cast<const T*>(bt->coo_values),          \
-                     total_tiles, tiles_per_step, jobs, num_jobs, no_x, no_y)
+#define HBDIA_SERVE_LAUNCH(C, X)                                                                            \
+  do {                                                                                                      \
+    if (bt->long_lists)                                                                                     \
+      hipLaunchKernelGGL((k_hbdia_batch<T, C, X, false, true, true>), grid, block, 0, s,                    \
+                         reinterpret_cast<const int32_t*>(bt->tiles), nullptr,                              \
+                         static_cast<const T*>(bt->values), bt->offsets, bt->coo_rows, bt->coo_cols,        \
+                         static_cast<const T*>(bt->coo_values), total_tiles, tiles_per_step, jobs,          \
+                         num_jobs, no_x, no_y);                                                             \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_hbdia_batch<T, C, X, false, true, false>), grid, block, 0, s,                   \
+                         reinterpret_cast<const int32_t*>(bt->tiles), nullptr,                              \
+                         static_cast<const T*>(bt->values), bt->offsets, bt->coo_rows, bt->coo_cols,        \
+                         static_cast<const T*>(bt->coo_values), total_tiles, tiles_per_step, jobs,          \
+                         num_jobs, no_x, no_y);                                                             \
+  } while (0)
   if (bt->has_coo) {
     if (bt->x32) HBDIA_SERVE_LAUNCH(true, true); else HBDIA_SERVE_LAUNCH(true, false);
   } else {
@@ -3312,10 +3334,15 @@ int launch_serve(const hbdia_batch_t* bt, const hbdia_serve_job_t* jobs, int32_t
 }
 
 template <typename T>
-int batch_occupancy(bool coo, bool x32, bool contig, int* out) {
+int batch_occupancy(bool coo, bool x32, bool contig, bool lng, int* out) {
   hipError_t e;
-#define HBDIA_BATCH_OCC(C, X, G) \
-  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_hbdia_batch<T, C, X, G, false>, kWG, 0)
+#define HBDIA_BATCH_OCC(C, X, G)                                                                                \
+  do {                                                                                                          \
+    if (lng && !G)                                                                                              \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_hbdia_batch<T, C, X, false, false, true>, kWG, 0); \
+    else                                                                                                        \
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_hbdia_batch<T, C, X, G, false, false>, kWG, 0);    \
+  } while (0)
   if (coo) {
     if (x32) {
       if (contig) HBDIA_BATCH_OCC(true, true, true); else HBDIA_BATCH_OCC(true, true, false);
@@ -3714,8 +3741,9 @@ int hbdia_batch_occupancy(const hbdia_batch_t* batch, int32_t kind, int32_t valu
       (kind != HBDIA_BATCH_INTERLEAVED && kind != HBDIA_BATCH_CONTIGUOUS && kind != HBDIA_BATCH_XCD))
     return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_batch_occupancy: invalid argument");
   const bool coo = batch->has_coo != 0, x32 = batch->x32 != 0, contig = kind == HBDIA_BATCH_CONTIGUOUS;
-  return value_size == 8 ? batch_occupancy<double>(coo, x32, contig, workgroups_per_cu)
-                         : batch_occupancy<float>(coo, x32, contig, workgroups_per_cu);
+  const bool lng = batch->long_lists != 0;
+  return value_size == 8 ? batch_occupancy<double>(coo, x32, contig, lng, workgroups_per_cu)
+                         : batch_occupancy<float>(coo, x32, contig, lng, workgroups_per_cu);
 }
 int hbdia_convert_values(const double* src, void* dst, int64_t n, int vdtype, hbdia_stream_t stream) {
   if (n < 0) return fail(HBDIA_ERROR_INVALID_ARGUMENT, "hbdia_convert_values: negative size");

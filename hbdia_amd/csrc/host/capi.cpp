@@ -825,6 +825,7 @@ int hbdia_batch_info(hbdia_batch_handle_t b, int64_t* info) {
       info[HBDIA_BATCH_INFO_TILES_PER_STEP] = B.schedule().tilesPerStep;
       info[HBDIA_BATCH_INFO_WORKGROUPS] = B.schedule().workgroups;
       info[HBDIA_BATCH_INFO_DTYPE] = b->dtype;
+      info[HBDIA_BATCH_INFO_LONG_LISTS] = B.deviceBatch().long_lists;
       return 0;
     });
   });

@@ -104,8 +104,10 @@ struct BatchTuneResult {
 
 // Converts every matrix at each candidate block width (host conversion with
 // `threshold`), pools them, times each tile schedule over `reps` launches on
-// device-resident x and y, and returns the batch of the fastest
-// (block width, schedule).  The matrices are left converted on the host at the
+// device-resident x and y (after about 50 ms of launches that let the clocks
+// ramp up), times the six fastest again in turn over three rounds, and returns
+// the batch of the fastest (block width, schedule) by its best round.  The
+// pools of every block width are held on the device until the choice is made.  The matrices are left converted on the host at the
 // chosen block width (a previous HBDIA layout, device copies included, is
 // dropped).  `table`, if given, receives every candidate's time.
 template <typename T>

@@ -249,7 +249,8 @@ enum {
   HBDIA_BATCH_INFO_TILES_PER_STEP = 9,
   HBDIA_BATCH_INFO_WORKGROUPS = 10,
   HBDIA_BATCH_INFO_DTYPE = 11,
-  HBDIA_BATCH_INFO_COUNT = 12
+  HBDIA_BATCH_INFO_LONG_LISTS = 12, /* 1: the kernel that pairs tiles of long offset lists */
+  HBDIA_BATCH_INFO_COUNT = 13
 };
 int hbdia_batch_create(hbdia_batch_handle_t* b, const hbdia_matrix_t* matrices, int count);
 /* Tries every block width in block_widths (host conversion with `threshold`,

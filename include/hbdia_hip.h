@@ -283,6 +283,7 @@ int hbdia_hbdia_spmv_mixed_noalloc(int vdtype, const void* values_lp, const doub
 #define HBDIA_BATCH_CONTIGUOUS 1
 #define HBDIA_BATCH_XCD 2
 #define HBDIA_BATCH_COO_PAD 64
+#define HBDIA_BATCH_CHUNK 9 /* offsets the batch kernel takes per step */
 
 /* One matrix of a batch, as placed in the pools (host-side description). */
 typedef struct hbdia_batch_desc {
@@ -328,6 +329,9 @@ typedef struct hbdia_batch {
   int64_t num_tiles;
   int32_t has_coo; /* any tile with fused == 1 */
   int32_t x32;     /* every matrix: num_rows * sizeof(value) < 2^32 */
+  int32_t long_lists; /* 1: run the kernel that pairs tiles with more than HBDIA_BATCH_CHUNK offsets
+                         (a hint: either value is correct; HBDIABatch sets it when such tiles hold
+                         most of the values, since that kernel holds more registers) */
 } hbdia_batch_t;
 
 typedef struct hbdia_batch_schedule {

@@ -100,14 +100,23 @@ def stencil_set(dtype, widths):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("widths", [[64], [128, 64, 256]])
 @pytest.mark.parametrize("kind", [0, 1, 2])
-def test_long_offset_lists_pair_exactly(gpu, dtype, widths, kind):
+@pytest.mark.parametrize("minority", [False, True])
+def test_long_offset_lists_pair_exactly(gpu, dtype, widths, kind, minority):
     """Tiles whose offset lists span several chunks run as pairs (two rows per lane) when their
     records name one list; every matrix equals the oracle and hbdiaSpMV bit for bit on every
-    schedule kind."""
-    print(f"case: batch long lists {np.dtype(dtype).name} widths={widths} kind={kind}", flush=True)
+    schedule kind.  With the long lists a minority of the values (large banded matrices beside
+    them) the batch runs the kernel without that pair path, and the long tiles one row per lane."""
+    print(f"case: batch long lists {np.dtype(dtype).name} widths={widths} kind={kind} minority={minority}",
+          flush=True)
     ms = stencil_set(dtype, widths)
+    if minority:
+        big = HBDIA.banded_batch(count=3, min_rows=60000, max_rows=90000, seed=41, dtype=dtype)
+        for m in big:
+            m.convertToHBDIAFormatOnHost(widths[0], 2)
+        ms = big + ms
     xs, y0s = pools(ms, dtype, seed=31)
     B = HBDIABatch(ms)
+    assert B.info()["long_lists"] == (0 if minority else 1)
     B.set_schedule(kind, 2, 0)
     y = run_batch(B, xs, y0s)
     off = B.row_offsets
@@ -179,11 +188,13 @@ def test_autotune_picks_a_candidate_and_stays_exact(gpu):
     B = HBDIABatch.autotune(ms, block_widths=(64, 128), threshold=2, reps=3)
     assert len(B.table) >= 2 * 5
     assert {t["block_width"] for t in B.table} == {64, 128}
-    best = min(B.table, key=lambda t: t["ms"])
+    # the pick is one of the six fastest first timings, chosen by a second timing of those six
+    top = sorted(B.table, key=lambda t: t["ms"])[:6]
     info = B.info()
-    assert (info["kind"], info["tiles_per_step"], info["workgroups"]) == (best["kind"], best["tiles_per_step"],
-                                                                         best["workgroups"])
-    assert all(m.getBlockWidth() == best["block_width"] for m in ms)
+    bw = ms[0].getBlockWidth()
+    assert (bw, info["kind"], info["tiles_per_step"], info["workgroups"]) in [
+        (t["block_width"], t["kind"], t["tiles_per_step"], t["workgroups"]) for t in top]
+    assert all(m.getBlockWidth() == bw for m in ms)
     xs, y0s = pools(ms, np.float64, seed=9)
     y = run_batch(B, xs, y0s, graph=True)
     for i, m in enumerate(ms):
