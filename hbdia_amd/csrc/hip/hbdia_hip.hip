@@ -2675,19 +2675,31 @@ typedef float flt4a4m __attribute__((ext_vector_type(4), aligned(4)));
 typedef unsigned u32x2a4 __attribute__((ext_vector_type(2), aligned(4)));
 
 // the four rows' stored values of one offset (p at a multiple of 4 elements)
-__device__ __forceinline__ void lp_quad(const float* p, float* v) {
-  const flt4a4m t = __builtin_nontemporal_load(reinterpret_cast<const flt4a4m*>(p));
+// A lane's four stored values of an offset: the raw load (issued first, so the
+// x loads of the same run go out before anything waits on it) and its FP32
+// values.
+template <typename TV> struct LpRaw;
+template <> struct LpRaw<float> { using type = flt4a4m; };
+template <> struct LpRaw<uint16_t> { using type = u32x2a4; };
+template <typename TV>
+__device__ __forceinline__ typename LpRaw<TV>::type lp_quad_raw(const TV* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const typename LpRaw<TV>::type*>(p));
+}
+__device__ __forceinline__ void lp_quad_cvt(const flt4a4m& t, float* v) {
   v[0] = t.x;
   v[1] = t.y;
   v[2] = t.z;
   v[3] = t.w;
 }
-__device__ __forceinline__ void lp_quad(const uint16_t* p, float* v) {
-  const u32x2a4 t = __builtin_nontemporal_load(reinterpret_cast<const u32x2a4*>(p));
+__device__ __forceinline__ void lp_quad_cvt(const u32x2a4& t, float* v) {
   v[0] = __uint_as_float(t.x << 16);
   v[1] = __uint_as_float(t.x & 0xffff0000u);
   v[2] = __uint_as_float(t.y << 16);
   v[3] = __uint_as_float(t.y & 0xffff0000u);
+}
+template <typename TV>
+__device__ __forceinline__ void lp_quad(const TV* p, float* v) {
+  lp_quad_cvt(lp_quad_raw<TV>(p), v);
 }
 
 // mixed_coo_group for the four 64-row groups of a 256-row wave whose entries
@@ -2799,9 +2811,9 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
       int off[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) off[k] = op[o0 + k];
-      float v[3][4];
+      typename LpRaw<TV>::type rv[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) lp_quad(vb + static_cast<int64_t>(o0 + k) * bw, v[k]);
+      for (int k = 0; k < 3; ++k) rv[k] = lp_quad_raw<TV>(vb + static_cast<int64_t>(o0 + k) * bw);
       const int64_t o = off[1];
       const int64_t lo = row0 + o - 1, hi = row0 + o + 4 * kWave;  // wave-uniform
       const bool inside = lo >= 0 && hi <= last;
@@ -2821,6 +2833,13 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
           c = c < 0 ? 0 : (c > last ? last : c);
           m[i] = x[c];
         }
+      }
+      const double eld = x[cl], erd = x[ch];  // scalar loads
+      // every load of the run is out; now the values
+      float v[3][4];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) lp_quad_cvt(rv[k], v[k]);
+      if (!inside) {
         // columns outside the matrix: their values as zeros
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -2829,7 +2848,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
           for (int i = 0; i < 4; ++i) v[k][i] = c + static_cast<uint32_t>(i) < n32 ? v[k][i] : 0.0f;
         }
       }
-      const float el = static_cast<float>(x[cl]), er = static_cast<float>(x[ch]);  // scalar loads
+      const float el = static_cast<float>(eld), er = static_cast<float>(erd);
       float xf[6];
 #pragma unroll
       for (int i = 0; i < 4; ++i) xf[i + 1] = static_cast<float>(m[i]);
