@@ -2807,13 +2807,23 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
                    ((bl - bq[0]) * c0 * bw + (row - bl * bw));
     const int* op = offs + sq[0];
     float s[4] = {0.0f, 0.0f, 0.0f, 0.0f}, a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int o0 = 0; o0 < c0; o0 += 3) {  // one run of three per step
-      int off[3];
+    // one run of three per step; every run but the last loads the next run's
+    // offsets and values while it waits for its own x (the last run, peeled,
+    // loads nothing ahead, so no step keeps a value it did not load)
+    int offn[3];
+    typename LpRaw<TV>::type rvn[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) off[k] = op[o0 + k];
+    for (int k = 0; k < 3; ++k) offn[k] = op[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rvn[k] = lp_quad_raw<TV>(vb + static_cast<int64_t>(k) * bw);
+    auto run = [&](auto ahead, int o0) {
+      int off[3];
       typename LpRaw<TV>::type rv[3];
 #pragma unroll
-      for (int k = 0; k < 3; ++k) rv[k] = lp_quad_raw<TV>(vb + static_cast<int64_t>(o0 + k) * bw);
+      for (int k = 0; k < 3; ++k) {
+        off[k] = offn[k];
+        rv[k] = rvn[k];
+      }
       const int64_t o = off[1];
       const int64_t lo = row0 + o - 1, hi = row0 + o + 4 * kWave;  // wave-uniform
       const bool inside = lo >= 0 && hi <= last;
@@ -2835,6 +2845,12 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
         }
       }
       const double eld = x[cl], erd = x[ch];  // scalar loads
+      if constexpr (decltype(ahead)::value) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) offn[k] = op[o0 + 3 + k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rvn[k] = lp_quad_raw<TV>(vb + static_cast<int64_t>(o0 + 3 + k) * bw);
+      }
       // every load of the run is out; now the values
       float v[3][4];
 #pragma unroll
@@ -2862,7 +2878,9 @@ __global__ __launch_bounds__(kWG) void k_hbdia_mixed_row4(
           a[i] = __builtin_fmaf(__builtin_fabsf(v[k][i]), __builtin_fabsf(xf[i + k]), a[i]);
         }
       }
-    }
+    };
+    for (int o0 = 0; o0 < c0 - 3; o0 += 3) run(std::true_type{}, o0);
+    run(std::false_type{}, c0 - 3);
     // back to row-per-lane order: group q's row r0 + lane at [64 q + lane]
     float* sy = s_y[wv];
     *reinterpret_cast<float4*>(sy + 4 * lane) = make_float4(s[0], s[1], s[2], s[3]);
