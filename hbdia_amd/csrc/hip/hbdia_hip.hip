@@ -2059,6 +2059,7 @@ __device__ __forceinline__ void tile_pair_long(int value_line_a, int value_line_
 }
 
 constexpr int kTileGroup = 16;  // tile records fetched together by one wave
+constexpr int kJobsInLds = 1024;  // serving: job starts held in LDS (8 KB)
 constexpr int kTileDwords = 20;  // 80-byte tile record
 static_assert(sizeof(hbdia_batch_tile_t) == kTileDwords * 4, "tile record layout");
 static_assert(HBDIA_BATCH_INLINE_OFFSETS >= kChunk, "a tile record carries the first chunk's offsets");
@@ -2100,6 +2101,17 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
     first = wave_tiles[xcd];
     limit = wave_tiles[xcd + 1];
   }
+  // JOBS: the jobs' first work indices in LDS, so the search that maps each
+  // tile group to its job is a few LDS reads instead of a chain of dependent
+  // global loads (up to kJobsInLds jobs; more are searched in memory)
+  __shared__ int64_t s_job[JOBS ? kJobsInLds : 1];
+  const bool jobs_lds = JOBS && num_jobs <= kJobsInLds;  // uniform
+  if constexpr (JOBS) {
+    if (jobs_lds) {
+      for (int i = threadIdx.x; i < num_jobs; i += blockDim.x) s_job[i] = jobs[i].work_begin;
+      __syncthreads();
+    }
+  }
   T* sv = s_v[wv];
   T* sx = s_x[wv];
   int* slo = s_lo[wv];
@@ -2122,7 +2134,7 @@ __global__ __launch_bounds__(kWG) void k_hbdia_batch(const int32_t* __restrict__
       int lo = 0, hi = num_jobs - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].work_begin <= rec) lo = mid;
+        if ((jobs_lds ? s_job[mid] : jobs[mid].work_begin) <= rec) lo = mid;
         else hi = mid - 1;
       }
       rec = vj ? jobs[lo].tile_begin + (tj - jobs[lo].work_begin) : 0;

@@ -435,7 +435,7 @@ std::unique_ptr<HBDIABatch<T>> autotuneBatch(const std::vector<HBDIA<T>*>& matri
     }
     // the fastest few again, in turn over three rounds, each taking its best
     // round: one timing of `reps` launches moves by a few per cent
-    std::sort(all.begin(), all.end(), [](const Cand& p, const Cand& q) { return p.ms < q.ms; });
+    std::stable_sort(all.begin(), all.end(), [](const Cand& p, const Cand& q) { return p.ms < q.ms; });
     const size_t K = std::min<size_t>(6, all.size());
     std::vector<double> again(K, std::numeric_limits<double>::infinity());
     for (int round = 0; round < 3; ++round)

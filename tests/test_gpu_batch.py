@@ -188,12 +188,14 @@ def test_autotune_picks_a_candidate_and_stays_exact(gpu):
     B = HBDIABatch.autotune(ms, block_widths=(64, 128), threshold=2, reps=3)
     assert len(B.table) >= 2 * 5
     assert {t["block_width"] for t in B.table} == {64, 128}
-    # the pick is one of the six fastest first timings, chosen by a second timing of those six
-    top = sorted(B.table, key=lambda t: t["ms"])[:6]
+    # the pick is one of the six fastest first timings (ties included: event times of a few
+    # microseconds repeat), chosen by a second timing of those six
+    sixth = sorted(t["ms"] for t in B.table)[5]
     info = B.info()
     bw = ms[0].getBlockWidth()
-    assert (bw, info["kind"], info["tiles_per_step"], info["workgroups"]) in [
-        (t["block_width"], t["kind"], t["tiles_per_step"], t["workgroups"]) for t in top]
+    picked = [t["ms"] for t in B.table if (t["block_width"], t["kind"], t["tiles_per_step"], t["workgroups"]) ==
+              (bw, info["kind"], info["tiles_per_step"], info["workgroups"])]
+    assert picked and min(picked) <= sixth
     assert all(m.getBlockWidth() == bw for m in ms)
     xs, y0s = pools(ms, np.float64, seed=9)
     y = run_batch(B, xs, y0s, graph=True)
