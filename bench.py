@@ -118,8 +118,8 @@ def _settle(fn, sync, seconds: float = SETTLE_S) -> int:
 
 
 def _reference_method(A, X, Y, gpu_coo: bool, iterations: int = 100) -> dict:
-    """The reference's own timing (benchmarking/benchHBDIA.cpp:30-50): per round y cleared, device
-    synchronized, hbdiaSpMV + device synchronize timed on the host clock; round 0 dropped and the mean
+    """The reference's own timing (benchmarking/benchHBDIA.cpp:30-50): per iteration y cleared, device
+    synchronized, hbdiaSpMV + device synchronize timed on the host clock; the first iteration dropped and the mean
     of the rest taken, as its plots do (benchmarking/Plotting/plot_hbdia_vs_cusparse_runtime.py:30-32,
     :59-61)."""
     from hbdia_amd.api import spmv_time_synced

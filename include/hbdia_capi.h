@@ -165,9 +165,9 @@ int hbdia_vector_device_ptrs(hbdia_vector_t v, void** local, void** coo);
 
 int hbdia_spmv(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_coo_cpu, int exec_coo_gpu);
 /* The reference's timing loop (benchmarking/benchHBDIA.cpp:30-50): each of
- * `iterations` rounds clears y and its COO-result buffer, synchronizes the
+ * `iterations` iterations clears y and its COO-result buffer, synchronizes the
  * device, then times hbdiaSpMV plus a device synchronize on the host clock
- * into times_ms[i].  The reference's plots drop round 0 as warm-up. */
+ * into times_ms[i].  The reference's plots drop the first iteration as warm-up. */
 int hbdia_spmv_time_synced(hbdia_matrix_t m, hbdia_vector_t x, hbdia_vector_t y, int exec_coo_cpu, int exec_coo_gpu,
                            int iterations, double* times_ms);
 
